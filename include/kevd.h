@@ -1,0 +1,236 @@
+/*
+ * kevd.h -- C ABI of libkevd, the MI355X (gfx950) implementation of
+ * accelerated randomly pivoted Cholesky (ARPC), the bistochastic kernel
+ * EVD built on its factor, and the Gaussian-kernel bandwidth sweep.
+ *
+ * The reference exposes these stages as three standalone MPI programs
+ * whose parameters are compile-time constants and whose interfaces are
+ * HDF5 files (ks_arpclm.cu:566-1152, ks_kevd.cu:419-2403,
+ * ks_bandwidth.cu:339-667).  Each entry point below replaces one stage or
+ * one step of a stage; the comment above it cites what it replaces.
+ *
+ * Conventions
+ *   - All device memory is owned by the context.  Host arrays belong to the
+ *     caller and are copied in or out; capacities are the caller's job.
+ *   - Every call returns a kevd_status.  No call exits the process.  The
+ *     drivers map statuses to the reference's exit codes (2 = non-positive
+ *     degree, 3 = potrf failure, 4 = non-positive q, 5 = eigensolver).
+ *   - One context is used by one host thread at a time.  Work is ordered on
+ *     the context's own HIP stream; every call returns with its host outputs
+ *     complete.  Several contexts may share one device.
+ *   - Point sets are row-major n x dim.  The kernel is
+ *     K(i,j) = exp(-||x_i - x_j||^2 / eps).  The reference writes the same
+ *     kernel as exp(-sum(dx^2) / (Nq * bw)) (ks_arpclm.cu:277), so
+ *     eps = Nq * bw.
+ */
+#ifndef KEVD_H
+#define KEVD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kevd_ctx kevd_ctx;
+
+/* Status codes.  2..5 carry the meaning of the reference's MPI_Abort codes
+ * (ks_kevd.cu:1170-1181 -> 2, 1510-1520 -> 3, 1692-1703 -> 4,
+ * 1904-1913 -> 5). */
+typedef enum {
+    KEVD_OK = 0,
+    KEVD_EINVAL = 1,             /* bad argument or call order            */
+    KEVD_ENONPOS_DEGREE = 2,     /* a degree d~_i <= 0                    */
+    KEVD_EPOTRF = 3,             /* Cholesky of the shifted Gram failed   */
+    KEVD_ENONPOS_Q = 4,          /* a q~_i <= 0 (or a Sinkhorn failure)   */
+    KEVD_EEIG = 5,               /* symmetric eigensolver failed          */
+    KEVD_EHIP = 6,               /* HIP runtime or library error          */
+    KEVD_ENOMEM = 7              /* device or host allocation failed      */
+} kevd_status;
+
+typedef enum {
+    KEVD_FP64 = 0,   /* every step in FP64 (the plain form)                 */
+    KEVD_MIXED = 1   /* FP32 MFMA distances / exp / downdate / Gram, FP64
+                        pivot block, residual diagonal, scalings, eigensolve */
+} kevd_precision;
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces the per-rank device setup of ks_arpclm.cu:683-689 (stream and
+ * handle creation) for one GPU. */
+kevd_status kevd_create(int device, kevd_ctx** out);
+void        kevd_destroy(kevd_ctx* ctx);
+const char* kevd_last_error(const kevd_ctx* ctx);
+const char* kevd_status_string(kevd_status s);
+
+/* Upload a point set (row-major n x dim).  Replaces the h5read of /u and
+ * its cudaMemcpy (ks_arpclm.cu:652-655, 693-696).  The points are centred
+ * on the device (the kernel is translation invariant); FP64 and FP32 copies
+ * plus FP64 squared norms are kept resident.  Any factor held by the
+ * context is dropped. */
+kevd_status kevd_set_points(kevd_ctx* ctx, const double* X, int64_t n, int32_t dim);
+kevd_status kevd_set_points_f32(kevd_ctx* ctx, const float* X, int64_t n, int32_t dim);
+
+/* Median of the squared pairwise distances ||x_i - x_j||^2, i != j, in
+ * FP64.  If nsample <= 0 or nsample >= n(n-1)/2 the median is exact over
+ * all pairs i < j (allowed for n <= 16384); otherwise it is the median of
+ * nsample pairs drawn with Philox4x32-10 from `seed`.  An even count takes
+ * the mean of the two central order statistics.  The reference has no
+ * median code: it hard-codes bw = 0.8 (ks_arpclm.cu:602). */
+kevd_status kevd_median_sqdist(kevd_ctx* ctx, uint64_t seed, int64_t nsample,
+                               double* eps_out);
+
+/* Kernel block K(rows, cols), written row-major nr x nc into K_out (host,
+ * FP64).  Replaces compute_kblock<T> (ks_arpclm.cu:242-280,
+ * ks_kevd.cu:302-340).  The mixed precision runs the distance product on
+ * FP32 MFMA with FP64 norms; FP64 runs it on FP64 MFMA. */
+kevd_status kevd_kernel_block(kevd_ctx* ctx, const int64_t* rows, int64_t nr,
+                              const int64_t* cols, int64_t nc, double eps,
+                              kevd_precision prec, double* K_out);
+
+/* ------------------------------------------------------------------ ARPC */
+
+typedef struct {
+    double eps;               /* kernel scale                               */
+    int64_t rank;             /* target rank r (the factor has r columns)   */
+    int32_t block;            /* proposals per round b, 1..128              */
+    uint64_t seed;            /* Philox key                                 */
+    double tol;               /* rejection tolerance on H_ii (ref 1e-10)    */
+    kevd_precision prec;
+    int32_t max_rounds;       /* guard; <= 0 means 4*ceil(r/b)+64           */
+    const int64_t* inject_pivots; /* optional fixed pivot list (host)       */
+    int64_t n_inject;         /* its length (>= rank when given)            */
+} kevd_arpc_params;
+
+typedef struct {
+    int64_t rank;             /* realised rank (<= params.rank)             */
+    int32_t rounds;           /* rounds run                                 */
+    int64_t* pivots;          /* out, capacity >= params.rank, acceptance order */
+    int32_t* accepted;        /* out, per round, capacity >= max_rounds     */
+    double* trace_err;        /* out, per round, tr(K - F F^T) / tr(K)      */
+    int32_t rounds_cap;       /* capacity of accepted / trace_err           */
+} kevd_arpc_result;
+
+/* Accelerated randomly pivoted Cholesky.  Replaces the round loop of
+ * ks_arpclm.cu:793-1046 (propose_block 193-224, the H overlap 833-874,
+ * rejection_sample 300-408, the Step-3 row loop 929-995 and compute_dsum
+ * 536-564).  The factor F (n x rank, K ~ F F^T) stays on the device.
+ * Rounds run until the rank reaches params.rank (the last accepted block is
+ * truncated), the residual mass is zero, or max_rounds is reached. */
+kevd_status kevd_arpc(kevd_ctx* ctx, const kevd_arpc_params* p, kevd_arpc_result* res);
+
+/* Copy the factor to the host: n x rank row-major with leading dimension ld
+ * (>= rank), as FP64 (`as` = KEVD_FP64) or FP32 (`as` = KEVD_MIXED). */
+kevd_status kevd_get_factor(kevd_ctx* ctx, void* F_host, int64_t ld, kevd_precision as);
+
+/* Install an external factor (n x r row-major, leading dimension ld) in the
+ * given precision, for kevd_evd.  n must match the point set. */
+kevd_status kevd_set_factor(kevd_ctx* ctx, const void* F_host, int64_t n, int64_t r,
+                            int64_t ld, kevd_precision prec);
+
+/* The inverse Cholesky factor of K(S,S) in the reference's meaning
+ * (ks_arpclm.cu:1048-1096): r x r lower triangular, column-major, FP64,
+ * with K(S,S) = L L^T and pivots in acceptance order.  Here L = F(S,:). */
+kevd_status kevd_get_linv(kevd_ctx* ctx, double* Linv_host);
+
+/* ------------------------------------------------------------------ kEVD */
+
+typedef enum {
+    KEVD_NORM_REF = 0,       /* two-step d~, q~ form of ks_kevd.cu            */
+    KEVD_NORM_SINKHORN = 1   /* symmetric Sinkhorn scaling s .* (K^ s) = 1    */
+} kevd_norm;
+
+typedef struct {
+    int32_t k_out;            /* leading eigenvectors returned              */
+    int32_t norm;             /* kevd_norm                                  */
+    double sinkhorn_tol;      /* max row-sum deviation target (Sinkhorn)    */
+    int32_t sinkhorn_max_iter;
+    kevd_precision prec;
+} kevd_evd_params;
+
+typedef struct {
+    double* lam;              /* out, rank values, ascending (FP64)         */
+    float* U;                 /* out, n x k_out row-major, columns ascending,
+                                 NULL to skip the copy                      */
+    double rowsum_dev;        /* max_i |(P 1)_i - 1|                        */
+    double shift;             /* the Cholesky shift s = DBL_EPSILON tr(A)   */
+    double min_degree;        /* min d~ (ref) or min s (Sinkhorn)           */
+    int32_t sinkhorn_iters;
+} kevd_evd_result;
+
+/* Bistochastic kernel EVD on the resident factor.  Replaces the four
+ * streaming passes and two dense phases of ks_kevd.cu:755-2308 with the
+ * F-form: d~ = F(F^T 1), q~ = F(F^T d~^-1), A = F^T D~^-2 F,
+ * B = F^T Q~^-1 F, R = chol(A + sI), W = R B R^T, W = V diag(lam) V^T,
+ * U = D~^-1 F R^-1 V_k. */
+kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res);
+
+/* U on the device after kevd_evd, as FP32 n x k_out row-major. */
+kevd_status kevd_get_u(kevd_ctx* ctx, float* U_host);
+
+/* ------------------------------------------------------------- bandwidth */
+
+/* S(eps_j) = sum_{i in rows} sum_{all j'} exp(-||x_i - x_j'||^2 / eps_j),
+ * FP64, all neps values from one fused launch.  Replaces dmat_comp and the
+ * per-bandwidth kernelsum_comp launches (ks_bandwidth.cu:181-325, 537-576).
+ * The distance tile runs on MFMA (FP32 with FP64 norms for KEVD_MIXED). */
+kevd_status kevd_bandwidth_sweep(kevd_ctx* ctx, const int64_t* rows, int64_t nrows,
+                                 const double* eps, int32_t neps,
+                                 kevd_precision prec, double* S_out);
+
+/* Post-processing of a sweep, ks_bandwidth.cu:609-642: logS_i = log(S_i),
+ * the centred-difference slope over log eps, the first maximum of the slope,
+ * eopt = eps[idmax+1] and mdim = 2 * slope[idmax].  slope has neps-2
+ * entries.  Pure host arithmetic. */
+kevd_status kevd_bandwidth_post(const double* eps, const double* S, int32_t neps,
+                                double* logS, double* slope, double* eopt,
+                                double* mdim);
+
+/* --------------------------------------------------------------- KS data */
+
+typedef struct {
+    double L;                 /* domain length (configs: 22, 44, 88)       */
+    int32_t M;                /* grid points (even)                        */
+    double dt;                /* ETDRK4 step                               */
+    int64_t n_discard;        /* steps discarded before the first sample   */
+    int32_t stride;           /* steps between samples                     */
+    uint64_t seed;            /* initial condition 0.1 * N(0,1) per point  */
+    double ic_scale;          /* 0.1                                       */
+} kevd_ks_params;
+
+/* Kuramoto-Sivashinsky snapshots u_t = -u u_x - u_xx - u_xxxx on a periodic
+ * grid, ETDRK4 with 3/2 dealiasing and contour-integral coefficients
+ * (ks_datagen.py:5-89).  Writes n snapshots, row-major n x M, FP64.
+ * Host-side generator (the trajectory is sequential in time). */
+kevd_status kevd_ks_generate(const kevd_ks_params* p, int64_t n, double* out);
+
+/* --------------------------------------------------------------- timing */
+
+typedef struct {
+    double arpc_ms;           /* wall time of the last kevd_arpc          */
+    double evd_ms;            /* wall time of the last kevd_evd           */
+    double downdate_ms;       /* summed device time of the downdate kernel */
+    int64_t downdate_launches;
+    double downdate_flop;     /* algorithmic flop of those launches        */
+    double gram_ms;           /* summed device time of the weighted Gram   */
+    int64_t gram_launches;
+    double gram_flop;
+    double eig_ms;            /* device time of the r x r eigensolve       */
+    double sample_ms;         /* pivot sampling kernels                    */
+    double rejection_ms;      /* H formation + rejection Cholesky          */
+    double matvec_ms;         /* streaming matvec passes                   */
+    double dense_ms;          /* r x r dense phase other than eigensolve   */
+    double u_ms;              /* U GEMM                                    */
+} kevd_timings;
+
+/* Enable per-kernel HIP event timing on the context's stream (adds one
+ * event pair per launch) and read the last call's figures. */
+kevd_status kevd_set_profiling(kevd_ctx* ctx, int enable);
+kevd_status kevd_get_timings(const kevd_ctx* ctx, kevd_timings* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KEVD_H */

@@ -1,0 +1,220 @@
+"""Python face of libkevd: a thin object wrapper over the C ABI.
+
+It mirrors the three reference programs as calls on one GPU context:
+``arpc`` (ks_arpclm.cu main), ``evd`` (ks_kevd.cu main) and
+``bandwidth_sweep`` + ``bandwidth_post`` (ks_bandwidth.cu main).  All
+arithmetic runs in libkevd on the GPU; this module only moves arrays.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import KevdError, KEVD_FP64, KEVD_MIXED, KEVD_NORM_REF, KEVD_NORM_SINKHORN
+
+PREC = {"fp64": KEVD_FP64, "mixed": KEVD_MIXED, KEVD_FP64: KEVD_FP64, KEVD_MIXED: KEVD_MIXED}
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class ArpcOutput:
+    def __init__(self, rank, rounds, pivots, accepted, trace_err):
+        self.rank = rank
+        self.rounds = rounds
+        self.pivots = pivots
+        self.accepted = accepted
+        self.trace_err = trace_err
+
+
+class EvdOutput:
+    def __init__(self, lam, U, rowsum_dev, shift, min_degree, sinkhorn_iters):
+        self.lam = lam
+        self.U = U
+        self.rowsum_dev = rowsum_dev
+        self.shift = shift
+        self.min_degree = min_degree
+        self.sinkhorn_iters = sinkhorn_iters
+
+
+class Context:
+    """One GPU context (device memory, stream).  Use as a context manager."""
+
+    def __init__(self, device=0):
+        self._lib = L.lib()
+        h = C.c_void_p()
+        s = self._lib.kevd_create(device, C.byref(h))
+        if s != 0:
+            raise KevdError(s, self._lib.kevd_status_string(s).decode())
+        self._h = h
+        self.n = 0
+        self.dim = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.kevd_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, s):
+        if s != 0:
+            raise KevdError(s, self._lib.kevd_last_error(self._h).decode())
+
+    # ------------------------------------------------------------ points
+    def set_points(self, X):
+        X = np.asarray(X)
+        if X.ndim != 2:
+            raise ValueError("X must be n x dim")
+        if X.dtype == np.float32:
+            X = np.ascontiguousarray(X)
+            self._check(self._lib.kevd_set_points_f32(self._h, _p(X, C.c_float), X.shape[0], X.shape[1]))
+        else:
+            X = np.ascontiguousarray(X, dtype=np.float64)
+            self._check(self._lib.kevd_set_points(self._h, _p(X, C.c_double), X.shape[0], X.shape[1]))
+        self.n, self.dim = X.shape
+
+    def median_sqdist(self, seed=0, nsample=0):
+        out = C.c_double()
+        self._check(self._lib.kevd_median_sqdist(self._h, seed, nsample, C.byref(out)))
+        return out.value
+
+    def kernel_block(self, rows, cols, eps, prec="fp64"):
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        cols = np.ascontiguousarray(cols, dtype=np.int64)
+        out = np.empty((rows.size, cols.size), dtype=np.float64)
+        self._check(self._lib.kevd_kernel_block(self._h, _p(rows, C.c_int64), rows.size,
+                                                _p(cols, C.c_int64), cols.size, eps,
+                                                PREC[prec], _p(out, C.c_double)))
+        return out
+
+    # -------------------------------------------------------------- ARPC
+    def arpc(self, eps, rank, block=128, seed=0, tol=1e-10, prec="mixed", max_rounds=0,
+             inject_pivots=None):
+        p = L.ArpcParams()
+        p.eps = eps
+        p.rank = rank
+        p.block = block
+        p.seed = seed
+        p.tol = tol
+        p.prec = PREC[prec]
+        mr = max_rounds if max_rounds > 0 else 4 * ((rank + block - 1) // block) + 64
+        p.max_rounds = mr
+        inj = None
+        if inject_pivots is not None:
+            inj = np.ascontiguousarray(inject_pivots, dtype=np.int64)
+            p.inject_pivots = _p(inj, C.c_int64)
+            p.n_inject = inj.size
+        piv = np.zeros(rank, dtype=np.int64)
+        acc = np.zeros(mr, dtype=np.int32)
+        err = np.zeros(mr, dtype=np.float64)
+        r = L.ArpcResult()
+        r.pivots = _p(piv, C.c_int64)
+        r.accepted = _p(acc, C.c_int32)
+        r.trace_err = _p(err, C.c_double)
+        r.rounds_cap = mr
+        self._check(self._lib.kevd_arpc(self._h, C.byref(p), C.byref(r)))
+        nr = r.rounds
+        return ArpcOutput(r.rank, nr, piv[:r.rank].copy(), acc[:nr].copy(), err[:nr].copy())
+
+    def get_factor(self, rank, as_prec="fp64"):
+        dt = np.float32 if PREC[as_prec] == KEVD_MIXED else np.float64
+        F = np.empty((self.n, rank), dtype=dt)
+        self._check(self._lib.kevd_get_factor(self._h, F.ctypes.data_as(C.c_void_p), rank, PREC[as_prec]))
+        return F
+
+    def set_factor(self, F):
+        F = np.asarray(F)
+        prec = KEVD_MIXED if F.dtype == np.float32 else KEVD_FP64
+        F = np.ascontiguousarray(F, dtype=np.float32 if prec == KEVD_MIXED else np.float64)
+        self._check(self._lib.kevd_set_factor(self._h, F.ctypes.data_as(C.c_void_p), F.shape[0],
+                                              F.shape[1], F.shape[1], prec))
+
+    def get_linv(self, rank):
+        out = np.empty(rank * rank, dtype=np.float64)
+        self._check(self._lib.kevd_get_linv(self._h, _p(out, C.c_double)))
+        return out.reshape(rank, rank, order="F")
+
+    # -------------------------------------------------------------- kEVD
+    def evd(self, rank, k_out=100, norm="ref", prec="mixed", sinkhorn_tol=1e-12,
+            sinkhorn_max_iter=1000, want_U=True):
+        p = L.EvdParams()
+        p.k_out = k_out
+        p.norm = KEVD_NORM_SINKHORN if norm == "sinkhorn" else KEVD_NORM_REF
+        p.sinkhorn_tol = sinkhorn_tol
+        p.sinkhorn_max_iter = sinkhorn_max_iter
+        p.prec = PREC[prec]
+        lam = np.zeros(rank, dtype=np.float64)
+        r = L.EvdResult()
+        r.lam = _p(lam, C.c_double)
+        U = None
+        if want_U:
+            U = np.zeros((self.n, k_out), dtype=np.float32)
+            r.U = _p(U, C.c_float)
+        self._check(self._lib.kevd_evd(self._h, C.byref(p), C.byref(r)))
+        return EvdOutput(lam, U, r.rowsum_dev, r.shift, r.min_degree, r.sinkhorn_iters)
+
+    # --------------------------------------------------------- bandwidth
+    def bandwidth_sweep(self, rows, eps, prec="mixed"):
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        S = np.zeros(eps.size, dtype=np.float64)
+        self._check(self._lib.kevd_bandwidth_sweep(self._h, _p(rows, C.c_int64), rows.size,
+                                                   _p(eps, C.c_double), eps.size, PREC[prec],
+                                                   _p(S, C.c_double)))
+        return S
+
+    # ------------------------------------------------------------ timing
+    def set_profiling(self, on=True):
+        self._check(self._lib.kevd_set_profiling(self._h, 1 if on else 0))
+
+    def timings(self):
+        t = L.Timings()
+        self._check(self._lib.kevd_get_timings(self._h, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in L.Timings._fields_}
+
+
+def bandwidth_post(eps, S):
+    """logS, slope, eopt, mdim of a sweep (ks_bandwidth.cu:609-642)."""
+    lib = L.lib()
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    n = eps.size
+    logS = np.zeros(n)
+    slope = np.zeros(max(n - 2, 1))
+    eopt = C.c_double()
+    mdim = C.c_double()
+    s = lib.kevd_bandwidth_post(_p(eps, C.c_double), _p(S, C.c_double), n, _p(logS, C.c_double),
+                                _p(slope, C.c_double), C.byref(eopt), C.byref(mdim))
+    if s != 0:
+        raise KevdError(s, "bandwidth_post: need >= 3 positive values")
+    return logS, slope[:n - 2], eopt.value, mdim.value
+
+
+def ks_generate(n, L_dom=22.0, M=64, dt=0.25, n_discard=10000, stride=4, seed=0, ic_scale=0.1):
+    """Kuramoto-Sivashinsky snapshots, n x M FP64 (ks_datagen.py:5-89)."""
+    lib = L.lib()
+    p = L.KsParams()
+    p.L = L_dom
+    p.M = M
+    p.dt = dt
+    p.n_discard = n_discard
+    p.stride = stride
+    p.seed = seed
+    p.ic_scale = ic_scale
+    out = np.empty((n, M), dtype=np.float64)
+    s = lib.kevd_ks_generate(C.byref(p), n, _p(out, C.c_double))
+    if s != 0:
+        raise KevdError(s, "ks_generate: bad parameters")
+    return out

@@ -1,0 +1,102 @@
+"""Build libkevd.so (HIP, gfx950) and the C++ drivers in-tree.
+
+Every HIP translation unit is compiled by hipcc for gfx950 only; host-only
+C++ (the KS generator, .npy I/O, drivers) by g++.  Outputs:
+  kernel_evd_amd/libkevd.so
+  kernel_evd_amd/bin/kevd-{gen,arpc,evd,bandwidth}
+Object files go to kernel_evd_amd/_build/.
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+DRV = os.path.join(PKG, "drivers")
+OBJ = os.path.join(PKG, "_build")
+BIN = os.path.join(PKG, "bin")
+INC = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libkevd.so")
+
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+
+HIP_SOURCES = ["ctx.hip", "arpc.hip", "factor.hip", "evd.hip", "bandwidth.hip"]
+CPP_SOURCES = ["ksgen.cpp"]
+DRIVERS = ["kevd_gen.cpp", "kevd_arpc.cpp", "kevd_evd.cpp", "kevd_bandwidth.cpp"]
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # packed FP32 ops stay disabled in all device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + INC, "-I" + CSRC,
+]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-I" + INC, "-I" + CSRC,
+             "-Wall", "-Wextra"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed:\n  %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def _compile(src):
+    name = os.path.basename(src)
+    obj = os.path.join(OBJ, name + ".o")
+    if src.endswith(".hip"):
+        cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
+    else:
+        cmd = ["g++"] + CXX_FLAGS + ["-c", src, "-o", obj]
+    _run(cmd)
+    return obj
+
+
+def build_library(jobs=8):
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES + CPP_SOURCES]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_drivers(jobs=8):
+    os.makedirs(BIN, exist_ok=True)
+    common = os.path.join(DRV, "npy.cpp")
+
+    def one(d):
+        exe = os.path.join(BIN, os.path.splitext(d)[0].replace("_", "-"))
+        _run(["g++"] + CXX_FLAGS + ["-o", exe, os.path.join(DRV, d), common,
+              "-L" + PKG, "-lkevd", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+        return exe
+
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        return list(ex.map(one, DRIVERS))
+
+
+def build_oracle():
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    return os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+def build_all():
+    lib = build_library()
+    drivers = build_drivers()
+    orc = build_oracle()
+    return lib, drivers, orc
+
+
+if __name__ == "__main__":
+    for p in build_all():
+        print(p)
+    sys.exit(0)

@@ -1,0 +1,840 @@
+// evd.hip -- bistochastic kernel EVD on the resident factor F (n x r).
+//
+// The reference streams K(R,S) panels through four passes and two dense
+// phases in the L^-1 form (README_kevd.md:22-69; ks_kevd.cu:755-2308).  With
+// F resident, the same quantities are (K^ = F F^T):
+//   t  = F^T 1,        d~ = F t                    pass 1 + pass 2 matvec
+//   cq = F^T d~^-1,    q~ = F cq                   pass 2 / pass 3 matvecs
+//   A  = F^T D~^-2 F,  B = F^T Q~^-1 F             Atil / Btil (one pass, two Grams)
+//   s  = DBL_EPSILON tr(A);  R = chol(A + sI)      dense 1 (ks_kevd.cu:1465-1520)
+//   W  = R B R^T;  W = V diag(lam) V^T             dense 2 (1764-1913)
+//   U  = D~^-1 F R^-1 V_k                          Z step + pass 4 (1973-2275)
+// plus P 1 - 1 (row-sum deviation) from one extra output column of the U GEMM.
+// Sinkhorn mode: s .* (K^ s) = 1 by s <- sqrt(s / K^ s), one Gram F^T S^2 F,
+// U = S F V_k Lambda_k^-1/2.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+#include "gemm.h"
+#include "internal.h"
+
+rocblas_handle blas_handle(kevd_ctx* ctx);
+
+namespace {
+
+// ------------------------------------------------------------- matvecs
+
+constexpr int COLSUM_ROWS = 512;  // rows of F per column-sum block
+
+// part[b][j] = sum_{rows of block b} w_i F(i, j), FP64 (w = 1 if null).
+// Each thread owns 16-byte column groups; a block streams COLSUM_ROWS
+// contiguous rows, so every wave load is one contiguous 4 KiB segment.
+template <typename T>
+__global__ __launch_bounds__(256) void k_colsum(const T* __restrict__ F, int64_t ldf, int64_t n,
+                                                int64_t r, const double* __restrict__ w,
+                                                double* part, int64_t rpad)
+{
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type vec_t;
+    int64_t r0 = (int64_t)blockIdx.x * COLSUM_ROWS;
+    int64_t r1 = r0 + COLSUM_ROWS < n ? r0 + COLSUM_ROWS : n;
+    for (int64_t cg = (int64_t)threadIdx.x * V; cg < r; cg += 256 * V) {
+        double acc[V];
+#pragma unroll
+        for (int q = 0; q < V; ++q) acc[q] = 0.0;
+        for (int64_t i = r0; i < r1; ++i) {
+            double wi = w ? w[i] : 1.0;
+            vec_t v = *reinterpret_cast<const vec_t*>(F + i * ldf + cg);
+            const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+            for (int q = 0; q < V; ++q) acc[q] += wi * (double)e[q];
+        }
+#pragma unroll
+        for (int q = 0; q < V; ++q)
+            if (cg + q < r) part[(int64_t)blockIdx.x * rpad + cg + q] = acc[q];
+    }
+}
+
+// c[j] = sum_b part[b][j] in block order (deterministic).
+__global__ __launch_bounds__(256) void k_colsum_reduce(const double* part, int nb, int64_t r,
+                                                       int64_t rpad, double* c)
+{
+    int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= r) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[(int64_t)b * rpad + j];
+    c[j] = s;
+}
+
+// y_i = F_i . c (FP64), one wave per row.  mode 0: plain; mode 1 (Sinkhorn
+// step): with s the current scaling, also dev_i = |s_i y_i - 1| and
+// s_new_i = sqrt(s_i / y_i).  Per-block min(y) and max(dev) go to bmin/bdev.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rowdot(const T* __restrict__ F, int64_t ldf, int64_t n,
+                                                int64_t r, const double* __restrict__ c,
+                                                double* y, double* bmin, int mode,
+                                                const double* s, double* s_new, double* bdev)
+{
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type vec_t;
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double mn = INFINITY, dv = 0.0;
+    for (int64_t row = (int64_t)blockIdx.x * 16 + wv; row < n && row < (int64_t)blockIdx.x * 16 + 16; row += 4) {
+        double acc = 0.0;
+        for (int64_t cg = (int64_t)lane * V; cg < r; cg += 64 * V) {
+            vec_t v = *reinterpret_cast<const vec_t*>(F + row * ldf + cg);
+            const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+                if (cg + q < r) acc += (double)e[q] * c[cg + q];
+        }
+        acc = wave_sum(acc);
+        mn = fmin(mn, acc);
+        if (lane == 0) {
+            y[row] = acc;
+            if (mode == 1) {
+                double si = s[row];
+                dv = fmax(dv, fabs(si * acc - 1.0));
+                s_new[row] = acc > 0.0 ? sqrt(si / acc) : si;
+            }
+        }
+    }
+    __shared__ double smn[4], sdv[4];
+    if (lane == 0) {
+        smn[wv] = mn;
+        sdv[wv] = dv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bmin[blockIdx.x] = fmin(fmin(smn[0], smn[1]), fmin(smn[2], smn[3]));
+        if (mode == 1) bdev[blockIdx.x] = fmax(fmax(sdv[0], sdv[1]), fmax(sdv[2], sdv[3]));
+    }
+}
+
+__global__ void k_block_minmax(const double* bmin, const double* bdev, int nb, double* out)
+{
+    __shared__ double smn[256], sdv[256];
+    double mn = INFINITY, dv = 0.0;
+    for (int j = threadIdx.x; j < nb; j += 256) {
+        mn = fmin(mn, bmin[j]);
+        if (bdev) dv = fmax(dv, bdev[j]);
+    }
+    smn[threadIdx.x] = mn;
+    sdv[threadIdx.x] = dv;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + o]);
+            sdv[threadIdx.x] = fmax(sdv[threadIdx.x], sdv[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = smn[0];
+        out[1] = sdv[0];
+    }
+}
+
+// Elementwise weights: mode 0: w1 = 1/d^2 (A), 1: w = 1/q (B), 2: w = 1/d,
+// 3: w = s^2, 4: copy.
+__global__ void k_weights(const double* x, int64_t n, int mode, double* w)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double v = x[i];
+    w[i] = mode == 0 ? 1.0 / (v * v) : mode == 1 ? 1.0 / v : mode == 2 ? 1.0 / v
+           : mode == 3 ? v * v : v;
+}
+
+// --------------------------------------------------------- weighted Gram
+
+// G_g = F^T diag(w_g) F for NG weight vectors, lower-triangle tiles.
+// Block tile BI x BJ of the r x r output, K = rows of F.  The BI panel is
+// staged into LDS pre-multiplied by each w_g (so the inner loop is LDS
+// read + MFMA only), the BJ panel as is.  Each lane feeds one K value per
+// MFMA: A(m, k) = Fw(k, I + m), lane (li, h) reads LDS row k0 + h, column
+// li -- consecutive lanes hit consecutive words.  Every PROMOTE rows the
+// FP32/FP64 MFMA accumulators are added into this block's FP64 partial
+// tile (a private slice of `part`), which bounds the length of any single
+// low-precision accumulation.
+template <typename T> struct GramCfg;
+template <> struct GramCfg<float> {
+    static constexpr int BI = 128, BJ = 128, KC = 16, TM = 2, TN = 2, PROMOTE = 1024;
+};
+template <> struct GramCfg<double> {
+    static constexpr int BI = 128, BJ = 64, KC = 8, TM = 4, TN = 2, PROMOTE = 1 << 30;
+};
+
+template <typename T, int NG>
+struct GramTile {
+    typedef GramCfg<T> C;
+    typedef Mfma<T> M;
+    static constexpr int BI = C::BI, BJ = C::BJ, KC = C::KC, TM = C::TM, TN = C::TN;
+    static constexpr int EPV = 16 / (int)sizeof(T);
+    static constexpr int PI = BI + EPV, PJ = BJ + EPV;
+    static constexpr int STAGE = NG * KC * PI + KC * PJ;
+    static constexpr int VI = BI / EPV, VJ = BJ / EPV;     // 16 B vectors per panel row
+    static constexpr int NVI = (KC * VI + 255) / 256, NVJ = (KC * VJ + 255) / 256;
+
+    typename M::acc_t acc[NG][TM][TN];
+    uint4 ri[NVI], rj[NVJ];
+    double wr[NG][NVI];
+    int wm, wn, h, li;
+
+    __device__ inline void zero()
+    {
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+#pragma unroll
+                    for (int v = 0; v < M::ACC; ++v) acc[g][a][b][v] = (T)0;
+    }
+
+    __device__ static inline void mask(uint4& u, int64_t col, int64_t r)
+    {
+        if (col + EPV > r) {
+            T* e = reinterpret_cast<T*>(&u);
+#pragma unroll
+            for (int q = 0; q < EPV; ++q)
+                if (col + q >= r) e[q] = (T)0;
+        }
+    }
+
+    __device__ inline void gload(const T* __restrict__ F, int64_t ldf, int64_t r, int64_t ci,
+                                 int64_t cj, int64_t k0, int64_t k_end,
+                                 const double* __restrict__ w1, const double* __restrict__ w2)
+    {
+#pragma unroll
+        for (int j = 0; j < NVI; ++j) {
+            int lin = threadIdx.x + 256 * j;
+            int kr = lin / VI, cv = lin % VI;
+            int64_t row = k0 + kr, col = ci + cv * EPV;
+            ri[j] = make_uint4(0, 0, 0, 0);
+            wr[0][j] = 0.0;
+            if (NG > 1) wr[NG - 1][j] = 0.0;
+            if (lin < KC * VI && row < k_end && col < r) {
+                ri[j] = *reinterpret_cast<const uint4*>(F + row * ldf + col);
+                mask(ri[j], col, r);
+                wr[0][j] = w1[row];
+                if (NG > 1) wr[NG - 1][j] = w2[row];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NVJ; ++j) {
+            int lin = threadIdx.x + 256 * j;
+            int kr = lin / VJ, cv = lin % VJ;
+            int64_t row = k0 + kr, col = cj + cv * EPV;
+            rj[j] = make_uint4(0, 0, 0, 0);
+            if (lin < KC * VJ && row < k_end && col < r) {
+                rj[j] = *reinterpret_cast<const uint4*>(F + row * ldf + col);
+                mask(rj[j], col, r);
+            }
+        }
+    }
+
+    __device__ inline void sstore(T* st)
+    {
+#pragma unroll
+        for (int j = 0; j < NVI; ++j) {
+            int lin = threadIdx.x + 256 * j;
+            if (lin < KC * VI) {
+                int kr = lin / VI, cv = lin % VI;
+                const T* e = reinterpret_cast<const T*>(&ri[j]);
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    T o[EPV];
+#pragma unroll
+                    for (int q = 0; q < EPV; ++q) o[q] = (T)((double)e[q] * wr[g][j]);
+                    *reinterpret_cast<uint4*>(st + g * KC * PI + kr * PI + cv * EPV) =
+                        *reinterpret_cast<const uint4*>(o);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NVJ; ++j) {
+            int lin = threadIdx.x + 256 * j;
+            if (lin < KC * VJ) {
+                int kr = lin / VJ, cv = lin % VJ;
+                *reinterpret_cast<uint4*>(st + NG * KC * PI + kr * PJ + cv * EPV) = rj[j];
+            }
+        }
+    }
+
+    __device__ inline void compute(const T* cur)
+    {
+#pragma unroll
+        for (int s = 0; s < KC / M::NH; ++s) {
+            int kr = s * M::NH + h;
+            T a[NG][TM], b[TN];
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+#pragma unroll
+                for (int x = 0; x < TM; ++x)
+                    a[g][x] = cur[g * KC * PI + kr * PI + wm * TM * M::MT + x * M::MT + li];
+#pragma unroll
+            for (int y = 0; y < TN; ++y) b[y] = cur[NG * KC * PI + kr * PJ + wn * TN * M::MT + y * M::MT + li];
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+#pragma unroll
+                for (int x = 0; x < TM; ++x)
+#pragma unroll
+                    for (int y = 0; y < TN; ++y) M::mma(acc[g][x][y], a[g][x], b[y]);
+        }
+    }
+
+    __device__ inline void promote(double* out, bool first)
+    {
+        // lane base + compile-time offsets; the empty asm makes the base
+        // opaque so the 64-bit addresses are not hoisted out of the K loop
+        // (which would pin one register pair per accumulator value)
+        double* base = out + (int64_t)(wm * TM * M::MT + M::row_lane(h)) * BJ + wn * TN * M::MT + li;
+        asm volatile("" : "+v"(base));
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TN; ++b)
+#pragma unroll
+                    for (int v = 0; v < M::ACC; ++v) {
+                        int off = g * BI * BJ + (a * M::MT + M::row_off(v)) * BJ + b * M::MT;
+                        double* p = base + off;
+                        double x = (double)acc[g][a][b][v];
+                        *p = first ? x : *p + x;
+                    }
+        zero();
+    }
+};
+
+template <typename T, int NG>
+__global__ __launch_bounds__(256) void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n,
+                                              int64_t r, const double* __restrict__ w1,
+                                              const double* __restrict__ w2,
+                                              const int* __restrict__ tileI,
+                                              const int* __restrict__ tileJ, int ntiles,
+                                              int64_t rows_per_split, double* part)
+{
+    typedef GramTile<T, NG> G;
+    __shared__ __attribute__((aligned(16))) T lds[2 * G::STAGE];
+    int tile = blockIdx.x, split = blockIdx.y;
+    int64_t ci = (int64_t)tileI[tile] * G::BI, cj = (int64_t)tileJ[tile] * G::BJ;
+    int64_t k_beg = (int64_t)split * rows_per_split;
+    int64_t k_end = k_beg + rows_per_split < n ? k_beg + rows_per_split : n;
+    double* out = part + ((int64_t)split * ntiles + tile) * (int64_t)NG * G::BI * G::BJ;
+
+    G g;
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    g.wm = w >> 1;
+    g.wn = w & 1;
+    g.h = lane / Mfma<T>::MT;
+    g.li = lane % Mfma<T>::MT;
+    g.zero();
+    bool first = true;
+    int64_t nk = k_end > k_beg ? (k_end - k_beg + G::KC - 1) / G::KC : 0;
+    if (nk > 0) {
+        g.gload(F, ldf, r, ci, cj, k_beg, k_end, w1, w2);
+        g.sstore(lds);
+        __syncthreads();
+    }
+    int since = 0;
+    for (int64_t kc = 0; kc < nk; ++kc) {
+        T* cur = lds + (kc & 1) * G::STAGE;
+        T* nxt = lds + ((kc + 1) & 1) * G::STAGE;
+        bool more = kc + 1 < nk;
+        if (more) g.gload(F, ldf, r, ci, cj, k_beg + (kc + 1) * G::KC, k_end, w1, w2);
+        g.compute(cur);
+        since += G::KC;
+        if (since >= GramCfg<T>::PROMOTE) {
+            g.promote(out, first);
+            first = false;
+            since = 0;
+        }
+        if (more) g.sstore(nxt);
+        __syncthreads();
+    }
+    if (since > 0 || first) g.promote(out, first);
+}
+
+// G_g(i, j) = sum_split part[split][tile][g](i, j) for the lower triangle,
+// mirrored; column-major r x r FP64.
+template <int BI, int BJ, int NG>
+__global__ __launch_bounds__(256) void k_gram_reduce(const double* part, int nsplit, int ntiles,
+                                                     const int* tileI, const int* tileJ, int64_t r,
+                                                     double* G1, double* G2)
+{
+    int tile = blockIdx.x;
+    int64_t ci = (int64_t)tileI[tile] * BI, cj = (int64_t)tileJ[tile] * BJ;
+    for (int e = threadIdx.x; e < BI * BJ; e += 256) {
+        int row = e / BJ, col = e % BJ;
+        int64_t i = ci + row, j = cj + col;
+        if (i >= r || j >= r || i < j) continue;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            double s = 0.0;
+            for (int sp = 0; sp < nsplit; ++sp)
+                s += part[(((int64_t)sp * ntiles + tile) * NG + g) * BI * BJ + e];
+            double* G = g == 0 ? G1 : G2;
+            G[i + j * r] = s;
+            G[j + i * r] = s;
+        }
+    }
+}
+
+// ------------------------------------------------------------- U GEMM
+
+// U(i, c) = scale_i * (F_i . Y_c) for c < kout, FP32 row-major n x kout;
+// column c == kout (if extra) goes to P1_i in FP64.  Y rows are the B
+// operand (kc x r, leading dimension ldy).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ugemm(const T* __restrict__ F, int64_t ldf, int64_t n,
+                                               int64_t r, const T* __restrict__ Y, int64_t ldy,
+                                               int kc, int kout, const double* __restrict__ scale,
+                                               float* U, double* P1)
+{
+    typedef typename Cfg<T>::G G;
+    __shared__ __attribute__((aligned(16))) T lds[G::LDS_ELEMS];
+    G g;
+    g.init();
+    int64_t r0 = (int64_t)blockIdx.x * Cfg<T>::BM;
+    int c0 = blockIdx.y * Cfg<T>::BN;
+    g.mainloop(lds, F + r0 * ldf, ldf, n - r0, Y + (int64_t)c0 * ldy, ldy, kc - c0, r);
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int b = 0; b < G::NTN; ++b)
+#pragma unroll
+            for (int v = 0; v < Mfma<T>::ACC; ++v) {
+                int64_t row = r0 + g.row_of(a, v);
+                int col = c0 + g.col_of(b);
+                if (row >= n || col >= kc) continue;
+                double x = (double)g.acc[a][b][v] * scale[row];
+                if (col < kout) U[row * kout + col] = (float)x;
+                else if (P1) P1[row] = x;
+            }
+}
+
+__global__ void k_trace_shift(double* A, int64_t r, double* tr_out)
+{
+    __shared__ double s[256];
+    double t = 0.0;
+    for (int64_t i = threadIdx.x; i < r; i += 256) t += A[i + i * r];
+    s[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    double tr = s[0];
+    double sh = DBL_EPSILON * tr;
+    for (int64_t i = threadIdx.x; i < r; i += 256) A[i + i * r] += sh;
+    if (threadIdx.x == 0) {
+        tr_out[0] = tr;
+        tr_out[1] = sh;
+    }
+}
+
+// Y (column-major r x k, FP64) -> rows of the GEMM B operand in T with
+// leading dimension ldy; optionally scaled per column and with one extra
+// row `extra` (length r).
+template <typename T>
+__global__ void k_pack_y(const double* Y, int64_t r, int k, const double* colscale,
+                         const double* extra, T* out, int64_t ldy)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int c = blockIdx.y;
+    if (i >= r) return;
+    if (c < k) {
+        double v = Y[i + (int64_t)c * r];
+        if (colscale) v *= colscale[c];
+        out[(int64_t)c * ldy + i] = (T)v;
+    } else if (extra) {
+        out[(int64_t)c * ldy + i] = (T)extra[i];
+    }
+}
+
+__global__ void k_rowsum_dev(const double* P1, int64_t n, double* bdev)
+{
+    __shared__ double s[256];
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        m = fmax(m, fabs(P1[i] - 1.0));
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bdev[blockIdx.x] = s[0];
+}
+
+__global__ void k_inv_sqrt_tail(const double* lam, int64_t r, int k, double* out)
+{
+    int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= k) return;
+    double l = lam[r - k + c];
+    out[c] = l > 0.0 ? 1.0 / sqrt(l) : 0.0;
+}
+
+// ------------------------------------------------------------- host side
+
+struct EvdBufs {
+    double *part = nullptr, *c = nullptr, *t = nullptr, *y = nullptr, *y2 = nullptr;
+    double *w1 = nullptr, *w2 = nullptr, *bmin = nullptr, *bdev = nullptr, *mm = nullptr;
+};
+
+template <typename T>
+kevd_status colsum(kevd_ctx* ctx, const double* w, double* c)
+{
+    int64_t n = ctx->n, r = ctx->rank;
+    int nb = (int)((n + COLSUM_ROWS - 1) / COLSUM_ROWS);
+    int64_t rpad = KEVD_ROUND_UP(r, 4);
+    double* part = nullptr;
+    KEVD_TRY(ctx_buf_t(ctx, "e_cpart", (size_t)nb * rpad, &part));
+    k_colsum<T><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, w, part, rpad);
+    k_colsum_reduce<<<(unsigned)((r + 255) / 256), 256, 0, ctx->stream>>>(part, nb, r, rpad, c);
+    KEVD_HIP(ctx, hipGetLastError());
+    return KEVD_OK;
+}
+
+// y = F c; returns min(y) (and max dev in Sinkhorn mode) through host_mm.
+template <typename T>
+kevd_status rowdot(kevd_ctx* ctx, const double* c, double* y, int mode, const double* s,
+                   double* s_new, double host_mm[2])
+{
+    int64_t n = ctx->n, r = ctx->rank;
+    int nb = (int)((n + 15) / 16);
+    double *bmin, *bdev, *mm;
+    KEVD_TRY(ctx_buf_t(ctx, "e_bmin", nb, &bmin));
+    KEVD_TRY(ctx_buf_t(ctx, "e_bdev", nb, &bdev));
+    KEVD_TRY(ctx_buf_t(ctx, "e_mm", 2, &mm));
+    k_rowdot<T><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, c, y, bmin, mode, s,
+                                            s_new, bdev);
+    k_block_minmax<<<1, 256, 0, ctx->stream>>>(bmin, mode == 1 ? bdev : nullptr, nb, mm);
+    KEVD_HIP(ctx, hipGetLastError());
+    KEVD_HIP(ctx, hipMemcpyAsync(host_mm, mm, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KEVD_OK;
+}
+
+template <typename T, int NG>
+kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, double* G2)
+{
+    typedef GramCfg<T> C;
+    int64_t n = ctx->n, r = ctx->rank;
+    int ti_n = (int)((r + C::BI - 1) / C::BI), tj_n = (int)((r + C::BJ - 1) / C::BJ);
+    std::vector<int> TI, TJ;
+    for (int I = 0; I < ti_n; ++I)
+        for (int J = 0; J < tj_n; ++J)
+            if ((int64_t)I * C::BI + C::BI - 1 >= (int64_t)J * C::BJ) {
+                TI.push_back(I);
+                TJ.push_back(J);
+            }
+    int ntiles = (int)TI.size();
+    // enough blocks to fill 256 CUs a few times over
+    int nsplit = (int)((2048 + ntiles - 1) / ntiles);
+    int64_t rps = KEVD_ROUND_UP((n + nsplit - 1) / nsplit, C::KC);
+    if (rps < 2048) rps = 2048;
+    nsplit = (int)((n + rps - 1) / rps);
+    int *dTI, *dTJ;
+    double* part;
+    KEVD_TRY(ctx_buf_t(ctx, "e_gTI", ntiles, &dTI));
+    KEVD_TRY(ctx_buf_t(ctx, "e_gTJ", ntiles, &dTJ));
+    KEVD_TRY(ctx_buf_t(ctx, "e_gpart", (size_t)nsplit * ntiles * NG * C::BI * C::BJ, &part));
+    KEVD_HIP(ctx, hipMemcpyAsync(dTI, TI.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    KEVD_HIP(ctx, hipMemcpyAsync(dTJ, TJ.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    {
+        ProfScope ps(ctx, TC_GRAM);
+        k_gram<T, NG><<<dim3(ntiles, nsplit), 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, w1,
+                                                                    w2, dTI, dTJ, ntiles, rps, part);
+    }
+    k_gram_reduce<C::BI, C::BJ, NG><<<ntiles, 256, 0, ctx->stream>>>(part, nsplit, ntiles, dTI, dTJ, r,
+                                                                    G1, G2);
+    KEVD_HIP(ctx, hipGetLastError());
+    ctx->tm.gram_flop += (double)NG * (double)n * (double)r * (double)(r + 1);
+    return KEVD_OK;
+}
+
+template <typename T>
+kevd_status ugemm(kevd_ctx* ctx, const double* Y, int k, const double* colscale, const double* extra,
+                  const double* rowscale, float* U, double* P1)
+{
+    int64_t n = ctx->n, r = ctx->rank;
+    int kc = k + (extra ? 1 : 0);
+    int64_t ldy = KEVD_ROUND_UP(r, 64);
+    T* Yp;
+    KEVD_TRY(ctx_buf_t(ctx, "e_Ypack", (size_t)kc * ldy, &Yp));
+    k_pack_y<T><<<dim3((unsigned)((r + 255) / 256), kc), 256, 0, ctx->stream>>>(Y, r, k, colscale, extra,
+                                                                                Yp, ldy);
+    dim3 grid((unsigned)((n + Cfg<T>::BM - 1) / Cfg<T>::BM), (unsigned)((kc + Cfg<T>::BN - 1) / Cfg<T>::BN));
+    {
+        ProfScope ps(ctx, TC_UGEMM);
+        k_ugemm<T><<<grid, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, Yp, ldy, kc, k, rowscale,
+                                                  U, P1);
+    }
+    KEVD_HIP(ctx, hipGetLastError());
+    return KEVD_OK;
+}
+
+#define BLAS(call)                                                                  \
+    do {                                                                            \
+        rocblas_status b_ = (call);                                                 \
+        if (b_ != rocblas_status_success)                                           \
+            return kevd_fail(ctx, KEVD_EHIP, std::string(#call) + " failed: " +     \
+                                                 rocblas_status_to_string(b_));     \
+    } while (0)
+
+// Symmetric eigensolve of the column-major r x r FP64 matrix W (lower
+// triangle used); V overwrites W, lam ascending.
+kevd_status eigh(kevd_ctx* ctx, double* W, int64_t r, double* lam)
+{
+    rocblas_handle h = blas_handle(ctx);
+    double* E;
+    int* info;
+    KEVD_TRY(ctx_buf_t(ctx, "e_E", r, &E));
+    KEVD_TRY(ctx_buf_t(ctx, "e_info", 1, &info));
+    {
+        ProfScope ps(ctx, TC_EIG);
+        BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)r, W,
+                              (rocblas_int)r, lam, E, info));
+    }
+    int hi = 0;
+    KEVD_HIP(ctx, hipMemcpyAsync(&hi, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (hi != 0) return kevd_fail(ctx, KEVD_EEIG, "syevd failed, info " + std::to_string(hi));
+    return KEVD_OK;
+}
+
+template <typename T>
+kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res)
+{
+    int64_t n = ctx->n, r = ctx->rank;
+    int k = p->k_out;
+    rocblas_handle h = blas_handle(ctx);
+    if (!h) return kevd_fail(ctx, KEVD_EHIP, "rocblas handle creation failed");
+    double *t, *dt, *cq, *qt, *w1, *w2, *A, *B, *W, *lam, *v2, *P1, *trs;
+    int* info;
+    KEVD_TRY(ctx_buf_t(ctx, "e_t", r, &t));
+    KEVD_TRY(ctx_buf_t(ctx, "e_dt", n, &dt));
+    KEVD_TRY(ctx_buf_t(ctx, "e_cq", r, &cq));
+    KEVD_TRY(ctx_buf_t(ctx, "e_qt", n, &qt));
+    KEVD_TRY(ctx_buf_t(ctx, "e_w1", n, &w1));
+    KEVD_TRY(ctx_buf_t(ctx, "e_w2", n, &w2));
+    KEVD_TRY(ctx_buf_t(ctx, "e_A", (size_t)r * r, &A));
+    KEVD_TRY(ctx_buf_t(ctx, "e_B", (size_t)r * r, &B));
+    KEVD_TRY(ctx_buf_t(ctx, "e_W", (size_t)r * r, &W));
+    KEVD_TRY(ctx_buf_t(ctx, "e_lam", r, &lam));
+    KEVD_TRY(ctx_buf_t(ctx, "e_v2", r, &v2));
+    KEVD_TRY(ctx_buf_t(ctx, "e_P1", n, &P1));
+    KEVD_TRY(ctx_buf_t(ctx, "e_trs", 2, &trs));
+    KEVD_TRY(ctx_buf_t(ctx, "e_pinfo", 1, &info));
+    unsigned gn = (unsigned)((n + 255) / 256);
+    double mm[2];
+    {
+        ProfScope ps(ctx, TC_MATVEC);
+        KEVD_TRY(colsum<T>(ctx, nullptr, t));  // t = F^T 1
+    }
+    KEVD_TRY(rowdot<T>(ctx, t, dt, 0, nullptr, nullptr, mm));  // d~ = F t
+    res->min_degree = mm[0];
+    if (!(mm[0] > 0.0))
+        return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive degree d~ (min " + std::to_string(mm[0]) + ")");
+    k_weights<<<gn, 256, 0, ctx->stream>>>(dt, n, 2, w1);  // 1/d~
+    KEVD_TRY(colsum<T>(ctx, w1, cq));                      // cq = F^T d~^-1
+    KEVD_TRY(rowdot<T>(ctx, cq, qt, 0, nullptr, nullptr, mm));  // q~ = F cq
+    if (!(mm[0] > 0.0))
+        return kevd_fail(ctx, KEVD_ENONPOS_Q, "non-positive q~ (min " + std::to_string(mm[0]) + ")");
+    k_weights<<<gn, 256, 0, ctx->stream>>>(dt, n, 0, w1);  // 1/d~^2
+    k_weights<<<gn, 256, 0, ctx->stream>>>(qt, n, 1, w2);  // 1/q~
+    KEVD_HIP(ctx, hipGetLastError());
+    KEVD_TRY((gram<T, 2>(ctx, w1, w2, A, B)));
+    // dense phase 1: A + sI = L L^T (R = L^T)
+    double trsh[2];
+    {
+        ProfScope ps(ctx, TC_DENSE);
+        k_trace_shift<<<1, 256, 0, ctx->stream>>>(A, r, trs);
+        BLAS(rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)r, A, (rocblas_int)r, info));
+    }
+    int pinfo = 0;
+    KEVD_HIP(ctx, hipMemcpyAsync(trsh, trs, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipMemcpyAsync(&pinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    res->shift = trsh[1];
+    if (pinfo != 0)
+        return kevd_fail(ctx, KEVD_EPOTRF, "Cholesky of A + sI failed at minor " + std::to_string(pinfo));
+    // dense phase 2: W = R B R^T = L^T B L
+    double one = 1.0, zero = 0.0;
+    {
+        ProfScope ps(ctx, TC_DENSE);
+        BLAS(rocblas_dtrmm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_none,
+                           rocblas_diagonal_non_unit, (rocblas_int)r, (rocblas_int)r, &one, A,
+                           (rocblas_int)r, B, (rocblas_int)r, W, (rocblas_int)r));  // W = B L
+        // v2 = B cq for the row-sum check, before B is reused
+        BLAS(rocblas_dgemv(h, rocblas_operation_none, (rocblas_int)r, (rocblas_int)r, &one, B,
+                           (rocblas_int)r, cq, 1, &zero, v2, 1));
+        BLAS(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                           rocblas_diagonal_non_unit, (rocblas_int)r, (rocblas_int)r, &one, A,
+                           (rocblas_int)r, W, (rocblas_int)r, B, (rocblas_int)r));  // B = L^T W
+    }
+    KEVD_TRY(eigh(ctx, B, r, lam));  // V in B
+    // Y = L^-T V_k  (the trailing k columns of V, ascending)
+    double* Vk = B + (r - k) * r;
+    {
+        ProfScope ps(ctx, TC_DENSE);
+        BLAS(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                           rocblas_diagonal_non_unit, (rocblas_int)r, (rocblas_int)k, &one, A,
+                           (rocblas_int)r, Vk, (rocblas_int)r));
+    }
+    // U = D~^-1 F Y, and P1 = D~^-1 F (B cq)
+    k_weights<<<gn, 256, 0, ctx->stream>>>(dt, n, 2, w1);
+    KEVD_TRY(ugemm<T>(ctx, Vk, k, nullptr, v2, w1, ctx->U, P1));
+    int nbd = 1024;
+    double* bdev;
+    KEVD_TRY(ctx_buf_t(ctx, "e_rsdev", nbd, &bdev));
+    k_rowsum_dev<<<nbd, 256, 0, ctx->stream>>>(P1, n, bdev);
+    double* mmd;
+    KEVD_TRY(ctx_buf_t(ctx, "e_mm2", 2, &mmd));
+    k_block_minmax<<<1, 256, 0, ctx->stream>>>(bdev, bdev, nbd, mmd);
+    KEVD_HIP(ctx, hipGetLastError());
+    KEVD_HIP(ctx, hipMemcpyAsync(mm, mmd, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipMemcpyAsync(res->lam, lam, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    res->rowsum_dev = mm[1];
+    return KEVD_OK;
+}
+
+template <typename T>
+kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res)
+{
+    int64_t n = ctx->n, r = ctx->rank;
+    int k = p->k_out;
+    rocblas_handle h = blas_handle(ctx);
+    if (!h) return kevd_fail(ctx, KEVD_EHIP, "rocblas handle creation failed");
+    double *s, *s2, *u, *Ks, *w1, *G, *lam, *isq;
+    KEVD_TRY(ctx_buf_t(ctx, "e_s", n, &s));
+    KEVD_TRY(ctx_buf_t(ctx, "e_s2", n, &s2));
+    KEVD_TRY(ctx_buf_t(ctx, "e_u", r, &u));
+    KEVD_TRY(ctx_buf_t(ctx, "e_Ks", n, &Ks));
+    KEVD_TRY(ctx_buf_t(ctx, "e_w1", n, &w1));
+    KEVD_TRY(ctx_buf_t(ctx, "e_A", (size_t)r * r, &G));
+    KEVD_TRY(ctx_buf_t(ctx, "e_lam", r, &lam));
+    KEVD_TRY(ctx_buf_t(ctx, "e_isq", k, &isq));
+    unsigned gn = (unsigned)((n + 255) / 256);
+    // start from s = 1 / sqrt(mean(K^ 1)) so that s K^ s is O(1)
+    double mm[2];
+    KEVD_TRY(colsum<T>(ctx, nullptr, u));
+    KEVD_TRY(rowdot<T>(ctx, u, Ks, 0, nullptr, nullptr, mm));
+    if (!(mm[0] > 0.0))
+        return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
+    std::vector<double> hk((size_t)n);
+    KEVD_HIP(ctx, hipMemcpy(hk.data(), Ks, n * sizeof(double), hipMemcpyDeviceToHost));
+    double mean = 0.0;
+    for (double v : hk) mean += v;
+    mean /= (double)n;
+    std::fill(hk.begin(), hk.end(), 1.0 / std::sqrt(mean));
+    KEVD_HIP(ctx, hipMemcpy(s, hk.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    int it = 0;
+    double dev = INFINITY;
+    int maxit = p->sinkhorn_max_iter > 0 ? p->sinkhorn_max_iter : 1000;
+    double tol = p->sinkhorn_tol > 0.0 ? p->sinkhorn_tol : 1e-12;
+    for (it = 1; it <= maxit; ++it) {
+        {
+            ProfScope ps(ctx, TC_MATVEC);
+            KEVD_TRY(colsum<T>(ctx, s, u));
+        }
+        KEVD_TRY(rowdot<T>(ctx, u, Ks, 1, s, s2, mm));
+        if (!(mm[0] > 0.0))
+            return kevd_fail(ctx, KEVD_ENONPOS_Q, "Sinkhorn: non-positive K^ s");
+        dev = mm[1];
+        if (dev <= tol) break;
+        std::swap(s, s2);
+    }
+    res->sinkhorn_iters = it > maxit ? maxit : it;
+    res->rowsum_dev = dev;
+    res->shift = 0.0;
+    // s holds the scaling whose deviation was measured last
+    k_weights<<<gn, 256, 0, ctx->stream>>>(s, n, 3, w1);  // s^2
+    KEVD_HIP(ctx, hipGetLastError());
+    KEVD_TRY((gram<T, 1>(ctx, w1, nullptr, G, nullptr)));
+    KEVD_TRY(eigh(ctx, G, r, lam));
+    k_inv_sqrt_tail<<<(k + 255) / 256, 256, 0, ctx->stream>>>(lam, r, k, isq);
+    KEVD_TRY(ugemm<T>(ctx, G + (r - k) * r, k, isq, nullptr, s, ctx->U, nullptr));
+    KEVD_HIP(ctx, hipMemcpyAsync(res->lam, lam, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> hs((size_t)n);
+    KEVD_HIP(ctx, hipMemcpy(hs.data(), s, n * sizeof(double), hipMemcpyDeviceToHost));
+    double smin = INFINITY;
+    for (double v : hs) smin = std::min(smin, v);
+    res->min_degree = smin;
+    return KEVD_OK;
+}
+
+}  // namespace
+
+extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res)
+{
+    if (!ctx || !p || !res) return kevd_fail(ctx, KEVD_EINVAL, "evd: null argument");
+    if (!ctx->have_factor || ctx->rank < 1) return kevd_fail(ctx, KEVD_EINVAL, "evd: no factor");
+    if (p->k_out < 1 || p->k_out > ctx->rank || p->k_out > 1024)
+        return kevd_fail(ctx, KEVD_EINVAL, "evd: need 1 <= k_out <= min(rank, 1024)");
+    if (p->norm != KEVD_NORM_REF && p->norm != KEVD_NORM_SINKHORN)
+        return kevd_fail(ctx, KEVD_EINVAL, "evd: unknown normalisation");
+    if (!res->lam) return kevd_fail(ctx, KEVD_EINVAL, "evd: lam output missing");
+    if (p->prec != ctx->fprec)
+        return kevd_fail(ctx, KEVD_EINVAL, "evd: precision must match the factor's precision");
+    if (ctx->rank > 46340) return kevd_fail(ctx, KEVD_EINVAL, "evd: rank too large for the r x r phase");
+    KEVD_HIP(ctx, hipSetDevice(ctx->device));
+    auto t0 = std::chrono::steady_clock::now();
+    ctx->tm.evd_ms = 0;
+    ctx->tm.gram_ms = 0;
+    ctx->tm.gram_launches = 0;
+    ctx->tm.gram_flop = 0;
+    ctx->tm.eig_ms = 0;
+    ctx->tm.matvec_ms = 0;
+    ctx->tm.dense_ms = 0;
+    ctx->tm.u_ms = 0;
+    prof_reset(ctx);
+    size_t ubytes = (size_t)ctx->n * p->k_out * sizeof(float);
+    if (ctx->U_bytes < ubytes) {
+        if (ctx->U) KEVD_HIP(ctx, hipFree(ctx->U));
+        ctx->U = nullptr;
+        KEVD_HIP(ctx, hipMalloc(&ctx->U, ubytes));
+        ctx->U_bytes = ubytes;
+    }
+    ctx->k_out = p->k_out;
+    res->sinkhorn_iters = 0;
+    kevd_status s;
+    if (p->norm == KEVD_NORM_REF)
+        s = ctx->fprec == KEVD_MIXED ? evd_ref<float>(ctx, p, res) : evd_ref<double>(ctx, p, res);
+    else
+        s = ctx->fprec == KEVD_MIXED ? evd_sinkhorn<float>(ctx, p, res) : evd_sinkhorn<double>(ctx, p, res);
+    if (s != KEVD_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        prof_reset(ctx);
+        return s;
+    }
+    if (res->U)
+        KEVD_HIP(ctx, hipMemcpyAsync(res->U, ctx->U, ubytes, hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    ctx->tm.evd_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    prof_resolve(ctx);
+    return KEVD_OK;
+}
+
+extern "C" kevd_status kevd_get_u(kevd_ctx* ctx, float* U_host)
+{
+    if (!ctx || !U_host || !ctx->U || ctx->k_out < 1)
+        return kevd_fail(ctx, KEVD_EINVAL, "get_u: no eigenvectors computed");
+    KEVD_HIP(ctx, hipSetDevice(ctx->device));
+    KEVD_HIP(ctx, hipMemcpyAsync(U_host, ctx->U, (size_t)ctx->n * ctx->k_out * sizeof(float),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KEVD_OK;
+}

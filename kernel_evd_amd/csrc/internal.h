@@ -1,0 +1,102 @@
+// internal.h -- the context object behind the kevd_ctx handle and host-side
+// helpers shared by the translation units of libkevd.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+// One device-side timing record: a named category and an event pair.
+struct TimedLaunch {
+    int cat;
+    hipEvent_t start, stop;
+};
+
+enum TimeCat {
+    TC_DOWNDATE = 0, TC_GRAM, TC_EIG, TC_SAMPLE, TC_REJECT, TC_MATVEC, TC_DENSE, TC_UGEMM,
+    TC_COUNT
+};
+
+struct kevd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // Point set, centred, resident in HBM.  dp = dim rounded up to 32 (zero
+    // padded) so every distance GEMM chunk is whole.
+    int64_t n = 0;
+    int32_t dim = 0;
+    int32_t dp = 0;
+    float* X32 = nullptr;      // n x dp, FP32 copy of the centred points
+    double* X64 = nullptr;     // n x dp, FP64 centred points
+    double* nrm32 = nullptr;   // ||X32_i||^2 accumulated in FP64
+    double* nrm64 = nullptr;   // ||X64_i||^2
+
+    // Factor F (n x rank, row-major, leading dimension ldf) in fprec.
+    void* F = nullptr;
+    size_t F_bytes = 0;
+    kevd_precision fprec = KEVD_FP64;
+    int64_t rank = 0;
+    int64_t ldf = 0;
+    bool have_factor = false;
+    std::vector<int64_t> pivots;  // acceptance order, when F came from kevd_arpc
+    double factor_eps = 0.0;
+
+    // kEVD output held on the device.
+    float* U = nullptr;
+    size_t U_bytes = 0;
+    int32_t k_out = 0;
+
+    // Named grow-only scratch buffers.
+    std::map<std::string, DevBuf> bufs;
+
+    // Pinned host staging for small device->host status copies.
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+
+    void* blas = nullptr;         // rocblas_handle, created on first use
+
+    bool profiling = false;
+    std::vector<TimedLaunch> launches;
+    std::vector<hipEvent_t> event_pool;
+    kevd_timings tm{};
+    double flop_acc[TC_COUNT] = {};
+};
+
+// Grow-only named device buffer.
+kevd_status ctx_buf(kevd_ctx* ctx, const char* name, size_t bytes, void** out);
+template <typename P>
+inline kevd_status ctx_buf_t(kevd_ctx* ctx, const char* name, size_t count, P** out)
+{
+    void* v = nullptr;
+    kevd_status s = ctx_buf(ctx, name, count * sizeof(P), &v);
+    *out = reinterpret_cast<P*>(v);
+    return s;
+}
+kevd_status ctx_pinned(kevd_ctx* ctx, size_t bytes, void** out);
+
+// Profiling: bracket a launch with an event pair on the context stream when
+// profiling is on; resolve the pairs into ctx->tm after the stream syncs.
+struct ProfScope {
+    kevd_ctx* ctx;
+    int cat;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(kevd_ctx* c, int category);
+    ~ProfScope();
+};
+void prof_reset(kevd_ctx* ctx);
+void prof_resolve(kevd_ctx* ctx);
+
+// The gfx950 MFMA paths need a gfx950 device; reject anything else loudly.
+kevd_status ctx_check_arch(kevd_ctx* ctx);
+
+// Cross-unit entry points.
+kevd_status kblock_gather(kevd_ctx* ctx, const int64_t* idx_dev, int64_t count,
+                          kevd_precision prec, void* Xout, double* nrm_out);

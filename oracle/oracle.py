@@ -1,0 +1,315 @@
+"""CPU oracle for libkevd -- TEST INFRASTRUCTURE ONLY.
+
+Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, as the checker.  The product never imports it.
+
+Two parts:
+  * liboracle.so (kevd_oracle.c): FP64 Philox, kernel block, median, the
+    fixed-point sampler, the rejection Cholesky and ARPC in F-form, and
+    bandwidth sums -- restating ks_arpclm.cu / ks_bandwidth.cu.
+  * NumPy restatements: the dense bistochastic EVD of ks_reference.py:21-40
+    (``dense_reference_evd``), its F-form low-rank counterpart of
+    ks_kevd.cu / README_kevd.md:22-69 (``lowrank_evd``), dense symmetric
+    Sinkhorn, and the KS ETDRK4 integrator of ks_datagen.py:5-89
+    (``ks_numpy``).
+
+Pinning of each function is listed in tests/test_oracle.py.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "liboracle.so")
+
+_so = None
+i32, i64, u32, u64, dbl = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_double
+P = C.POINTER
+
+
+def so():
+    global _so
+    if _so is None:
+        if not os.path.exists(SO):
+            import subprocess
+            subprocess.run(["make", "-s", "-C", HERE], check=True)
+        L = C.CDLL(SO)
+        L.orc_philox4x32_10.argtypes = [P(u32), P(u32), P(u32)]
+        L.orc_draw_u64.restype = u64
+        L.orc_draw_u64.argtypes = [u64, u32, u32, u32]
+        L.orc_uniform01.restype = dbl
+        L.orc_uniform01.argtypes = [u64, u32, u32, u32]
+        L.orc_kernel_block.argtypes = [P(dbl), i64, i32, P(i64), i64, P(i64), i64, dbl, P(dbl)]
+        L.orc_median_sqdist.restype = C.c_int
+        L.orc_median_sqdist.argtypes = [P(dbl), i64, i32, u64, i64, P(dbl)]
+        L.orc_fixed_weights.restype = u64
+        L.orc_fixed_weights.argtypes = [P(dbl), i64, P(u64)]
+        L.orc_propose.restype = C.c_int
+        L.orc_propose.argtypes = [P(dbl), i64, u64, u32, i32, P(i64), P(u64)]
+        L.orc_rejchol.restype = i32
+        L.orc_rejchol.argtypes = [P(dbl), i32, u64, u32, dbl, C.c_int, P(i32), P(dbl), P(dbl)]
+        L.orc_arpc.restype = C.c_int
+        L.orc_arpc.argtypes = [P(dbl), i64, i32, dbl, i64, i32, u64, dbl, i32, P(i64), i64,
+                               P(dbl), P(i64), P(i32), P(dbl), P(i32), P(i64), P(dbl)]
+        L.orc_bandwidth.argtypes = [P(dbl), i64, i32, P(i64), i64, P(dbl), i32, P(dbl)]
+        L.orc_num_threads.restype = C.c_int
+        _so = L
+    return _so
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(P(ct))
+
+
+STREAM_PROPOSE, STREAM_REJECT, STREAM_GAUSS, STREAM_MEDIAN, STREAM_ROWS = range(5)
+
+
+def philox(ctr, key):
+    c = (u32 * 4)(*ctr)
+    k = (u32 * 2)(*key)
+    o = (u32 * 4)()
+    so().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def draw_u64(seed, stream, rnd, idx):
+    return so().orc_draw_u64(seed, stream, rnd, idx)
+
+
+def uniform01(seed, stream, rnd, idx):
+    return so().orc_uniform01(seed, stream, rnd, idx)
+
+
+def kernel_block(X, rows, cols, eps):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    out = np.empty((rows.size, cols.size))
+    so().orc_kernel_block(_p(X, dbl), X.shape[0], X.shape[1], _p(rows, i64), rows.size,
+                          _p(cols, i64), cols.size, eps, _p(out, dbl))
+    return out
+
+
+def median_sqdist(X, seed=0, nsample=0):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    out = C.c_double()
+    rc = so().orc_median_sqdist(_p(X, dbl), X.shape[0], X.shape[1], seed, nsample, C.byref(out))
+    if rc:
+        raise RuntimeError("orc_median_sqdist failed")
+    return out.value
+
+
+def fixed_weights(d):
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    pre = np.empty(d.size, dtype=np.uint64)
+    W = so().orc_fixed_weights(_p(d, dbl), d.size, _p(pre, u64))
+    return W, pre
+
+
+def propose(d, seed, rnd, b):
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    out = np.empty(b, dtype=np.int64)
+    W = C.c_uint64()
+    rc = so().orc_propose(_p(d, dbl), d.size, seed, rnd, b, _p(out, i64), C.byref(W))
+    return (out if rc == 0 else None), W.value
+
+
+def rejchol(H, seed, rnd, tol=1e-10, inject=False):
+    H = np.array(H, dtype=np.float64, order="C")
+    b = H.shape[0]
+    acc = np.zeros(b, dtype=np.int32)
+    Lc = np.zeros((b, b))
+    Li = np.zeros((b, b))
+    k = so().orc_rejchol(_p(H, dbl), b, seed, rnd, tol, int(inject), _p(acc, i32), _p(Lc, dbl),
+                         _p(Li, dbl))
+    return k, acc[:k].copy(), Lc[:k, :k].copy(), Li[:k, :k].copy()
+
+
+class ArpcRef:
+    pass
+
+
+def arpc(X, eps, rank, block=128, seed=0, tol=1e-10, max_rounds=0, inject=None):
+    """F-form ARPC in FP64 on the CPU (orc_arpc)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    n, dim = X.shape
+    mr = max_rounds if max_rounds > 0 else 4 * ((rank + block - 1) // block) + 64
+    F = np.zeros((n, rank))
+    piv = np.zeros(rank, dtype=np.int64)
+    acc = np.zeros(mr, dtype=np.int32)
+    err = np.zeros(mr)
+    rounds = C.c_int32()
+    rk = C.c_int64()
+    d = np.zeros(n)
+    injp = None
+    ninj = 0
+    if inject is not None:
+        inject = np.ascontiguousarray(inject, dtype=np.int64)
+        injp = _p(inject, i64)
+        ninj = inject.size
+    rc = so().orc_arpc(_p(X, dbl), n, dim, eps, rank, block, seed, tol, mr, injp, ninj,
+                       _p(F, dbl), _p(piv, i64), _p(acc, i32), _p(err, dbl), C.byref(rounds),
+                       C.byref(rk), _p(d, dbl))
+    if rc:
+        raise RuntimeError("orc_arpc failed: %d" % rc)
+    r = ArpcRef()
+    r.rank = rk.value
+    r.rounds = rounds.value
+    r.F = F[:, :r.rank]
+    r.pivots = piv[:r.rank]
+    r.accepted = acc[:r.rounds]
+    r.trace_err = err[:r.rounds]
+    r.d = d
+    return r
+
+
+def bandwidth(X, rows, eps):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    eps = np.ascontiguousarray(eps, dtype=np.float64)
+    S = np.zeros(eps.size)
+    so().orc_bandwidth(_p(X, dbl), X.shape[0], X.shape[1], _p(rows, i64), rows.size,
+                       _p(eps, dbl), eps.size, _p(S, dbl))
+    return S
+
+
+def num_threads():
+    return so().orc_num_threads()
+
+
+# ------------------------------------------------------------ dense EVD
+
+def dense_kernel(X, eps):
+    """compute_kernel_mat, ks_reference.py:51-58 (eps = Nq * bw): direct
+    squared differences, FP64."""
+    from scipy.spatial.distance import cdist
+    X = np.asarray(X, dtype=np.float64)
+    return np.exp(-cdist(X, X, "sqeuclidean") / eps)
+
+
+def dense_reference_evd(K):
+    """ks_reference.py:21-40: d = K1/N, q = K D^-1 1 / N, Kt = D^-1 K Q^-1/2,
+    P = Kt Kt^T / N^2; eigenpairs descending."""
+    NM = K.shape[0]
+    one = np.ones(NM)
+    dvec = (1.0 / NM) * (K @ one)
+    qvec = (1.0 / NM) * (K @ (one / dvec))
+    Kt = (K / dvec[:, None]) / np.sqrt(qvec)[None, :]
+    P = (1.0 / NM ** 2) * (Kt @ Kt.T)
+    w, V = np.linalg.eigh(P)
+    return w[::-1].copy(), V[:, ::-1].copy(), P
+
+
+def lowrank_evd(F, k_out):
+    """F-form of the streaming kEVD (README_kevd.md:22-69; ks_kevd.cu passes
+    1-4 and dense phases 1-2), all in FP64:
+      d~ = F (F^T 1); q~ = F (F^T d~^-1); A = F^T D~^-2 F; B = F^T Q~^-1 F;
+      s = eps_mach tr(A); R = chol(A + sI) (upper, A = R^T R);
+      W = R B R^T; W = V diag(lam) V^T; U = D~^-1 F R^-1 V_k.
+    Returns lam ascending (all r), U (n x k_out, columns ascending), P1 - 1."""
+    F = np.asarray(F, dtype=np.float64)
+    n, r = F.shape
+    dt = F @ (F.T @ np.ones(n))
+    qt = F @ (F.T @ (1.0 / dt))
+    Y = F / dt[:, None]
+    A = Y.T @ Y
+    B = F.T @ (F / qt[:, None])
+    s = np.finfo(np.float64).eps * np.trace(A)
+    Rl = np.linalg.cholesky(A + s * np.eye(r))  # lower, A = Rl Rl^T
+    R = Rl.T
+    W = R @ B @ R.T
+    W = 0.5 * (W + W.T)
+    lam, V = np.linalg.eigh(W)
+    Vk = V[:, r - k_out:]
+    Yk = np.linalg.solve(R, Vk)
+    U = (F @ Yk) / dt[:, None]
+    # row sums of P = Y B Y^T
+    P1 = Y @ (B @ (Y.T @ np.ones(n)))
+    return lam, U, P1 - 1.0, s
+
+
+def lowrank_sinkhorn_evd(F, k_out, tol=1e-12, max_iter=10000):
+    """Symmetric Sinkhorn on K^ = F F^T: s .* (K^ s) = 1, P = S K^ S; the
+    nonzero spectrum is that of F^T S^2 F.  Damped fixed point
+    s <- sqrt(s / (K^ s))."""
+    F = np.asarray(F, dtype=np.float64)
+    n, r = F.shape
+    s = np.ones(n) / np.sqrt(max(1e-300, np.mean(F @ (F.T @ np.ones(n)))))
+    it = 0
+    for it in range(1, max_iter + 1):
+        Ks = F @ (F.T @ s)
+        dev = np.max(np.abs(s * Ks - 1.0))
+        if dev <= tol:
+            break
+        s = np.sqrt(s / Ks)
+    G = F.T @ (F * (s * s)[:, None])
+    G = 0.5 * (G + G.T)
+    lam, V = np.linalg.eigh(G)
+    Vk = V[:, r - k_out:]
+    U = (s[:, None] * F) @ Vk / np.sqrt(np.maximum(lam[r - k_out:], 1e-300))[None, :]
+    return lam, U, s, it
+
+
+# -------------------------------------------------------------- KS data
+
+def ks_numpy(n, L=22.0, M=64, dt=0.25, n_discard=10000, stride=4, u0=None, nsteps_total=None):
+    """ETDRK4 with 3/2 dealiasing and 16-point contour coefficients,
+    ks_datagen.py:5-89, started from u0 (physical space).  Returns n
+    snapshots taken every `stride` steps after `n_discard` steps (the first
+    snapshot is the state after n_discard steps)."""
+    M2 = M // 2 + 1
+    Mext = 3 * M // 2
+    freq = 2 * np.pi / L * np.arange(M2)
+    gg = -0.5j * Mext / M * freq
+    ldiag = freq ** 2 - freq ** 4
+    e1 = np.exp(dt * ldiag)
+    e2 = np.exp(dt / 2 * ldiag)
+    J = 16
+    rvec = np.exp(1j * np.pi * np.linspace(0.5 / J, (J - 0.5) / J, num=J))
+    Z = dt * ldiag.reshape(M2, 1) + rvec.reshape(1, J)
+    f0 = dt * np.real(np.mean((np.exp(0.5 * Z) - 1) / Z, axis=-1))
+    f1 = dt * np.real(np.mean((-4 - Z + np.exp(Z) * (4 - 3 * Z + Z ** 2)) / Z ** 3, axis=-1))
+    f2 = dt * np.real(np.mean(2 * (2 + Z + np.exp(Z) * (-2 + Z)) / Z ** 3, axis=-1))
+    f3 = dt * np.real(np.mean((-4 - 3 * Z - Z ** 2 + np.exp(Z) * (4 - Z)) / Z ** 3, axis=-1))
+
+    def nl(v):
+        return gg * np.fft.rfft(np.fft.irfft(v, n=Mext) ** 2)[:M2]
+
+    v = np.fft.rfft(u0)
+
+    def step(v):
+        Nu = nl(v)
+        a = e2 * v + f0 * Nu
+        Na = nl(a)
+        b = e2 * v + f0 * Na
+        Nb = nl(b)
+        c = e2 * a + f0 * (2 * Nb - Nu)
+        Nc = nl(c)
+        return e1 * v + f1 * Nu + f2 * (Na + Nb) + f3 * Nc
+
+    for _ in range(n_discard):
+        v = step(v)
+    out = np.empty((n, M))
+    out[0] = np.fft.irfft(v, n=M)
+    for i in range(1, n):
+        for _ in range(stride):
+            v = step(v)
+        out[i] = np.fft.irfft(v, n=M)
+    return out
+
+
+def gauss_ic(M, seed, scale=0.1):
+    """0.1 N(0,1) per grid point: Box-Muller on Philox (GAUSS stream)."""
+    u = np.empty(M)
+    for p in range(0, M, 2):
+        o = philox([p // 2, 0, STREAM_GAUSS, 0], [seed & 0xffffffff, seed >> 32])
+        a = ((o[1] << 32) | o[0]) >> 11
+        b = ((o[3] << 32) | o[2]) >> 11
+        u1 = (a + 1) * 2.0 ** -53
+        u2 = b * 2.0 ** -53
+        r = np.sqrt(-2.0 * np.log(u1))
+        u[p] = scale * r * np.cos(2 * np.pi * u2)
+        if p + 1 < M:
+            u[p + 1] = scale * r * np.sin(2 * np.pi * u2)
+    return u

@@ -1,0 +1,102 @@
+"""GPU parity of the bistochastic kEVD.
+
+Oracles: oracle.lowrank_evd (the F-form of ks_kevd.cu in FP64 NumPy),
+oracle.dense_reference_evd (ks_reference.py:21-40) for the full-rank case,
+oracle.lowrank_sinkhorn_evd for the Sinkhorn mode.  Bars: FP64 path 1e-9
+relative on eigenvalues; mixed path top-32 within 1e-5 relative of FP64;
+full rank against the dense oracle 1e-8; lambda_max = 1.
+"""
+import numpy as np
+import pytest
+
+from conftest import clustered_points, gauss_points
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _factor(ctx, n=1500, dim=8, rank=200, seed=0):
+    X = clustered_points(n, dim, seed=seed)
+    ctx.set_points(X)
+    eps = orc.median_sqdist(X)
+    out = ctx.arpc(eps, rank, block=32, seed=seed, prec="fp64")
+    return X, eps, out, ctx.get_factor(out.rank, "fp64")
+
+
+def _align(U, V):
+    s = np.sign(np.sum(U * V, axis=0))
+    s[s == 0] = 1
+    return U * s
+
+
+def test_evd_fp64_matches_lowrank_oracle(gpu_ctx):
+    X, eps, out, F = _factor(gpu_ctx)
+    k = 24
+    res = gpu_ctx.evd(out.rank, k_out=k, prec="fp64")
+    lam, U, dev, shift = orc.lowrank_evd(F, k)
+    print("lam top", res.lam[-4:], "ref", lam[-4:], flush=True)
+    np.testing.assert_allclose(res.lam[-k:], lam[-k:], rtol=1e-9, atol=1e-13)
+    assert abs(res.lam[-1] - 1.0) < 1e-10
+    assert res.rowsum_dev < 1e-9
+    assert abs(res.shift - shift) <= 1e-12 * shift
+    # leading eigenvectors up to sign (skip near-degenerate ones)
+    Ua = _align(res.U.astype(np.float64), U)
+    gaps = np.abs(np.diff(lam[-k - 1:]))
+    ok = (gaps[:-1] > 1e-6) & (gaps[1:] > 1e-6) if k > 1 else np.array([True])
+    err = np.max(np.abs(Ua - U), axis=0)
+    assert np.all(err[-8:][ok[-8:]] < 1e-5), err[-8:]
+
+
+def test_evd_mixed_vs_fp64(gpu_ctx):
+    X, eps, out, F = _factor(gpu_ctx, seed=1)
+    k = 32
+    r64 = gpu_ctx.evd(out.rank, k_out=k, prec="fp64")
+    gpu_ctx.set_factor(F.astype(np.float32))
+    rmx = gpu_ctx.evd(out.rank, k_out=k, prec="mixed")
+    rel = np.abs(rmx.lam[-32:] - r64.lam[-32:]) / np.abs(r64.lam[-32:])
+    print("mixed top-32 max rel err", rel.max(), flush=True)
+    assert rel.max() < 1e-5
+    # against the FP64 oracle on the same (FP32-rounded) factor
+    lam, _, _, _ = orc.lowrank_evd(F.astype(np.float32).astype(np.float64), k)
+    assert np.max(np.abs(rmx.lam[-32:] - lam[-32:]) / np.abs(lam[-32:])) < 1e-5
+
+
+def test_full_rank_equals_dense_reference(gpu_ctx):
+    n = 240
+    X = gauss_points(n, 4, seed=5)
+    gpu_ctx.set_points(X)
+    eps = 2.0 * orc.median_sqdist(X)
+    out = gpu_ctx.arpc(eps, n, block=16, seed=3, tol=0.0, prec="fp64")
+    assert out.rank == n or out.trace_err[-1] < 1e-14
+    res = gpu_ctx.evd(out.rank, k_out=32, prec="fp64")
+    K = orc.dense_kernel(X, eps)
+    w, V, P = orc.dense_reference_evd(K)
+    top = res.lam[::-1][:32]
+    print("dense top", w[:4], "gpu", top[:4], flush=True)
+    assert np.max(np.abs(top - w[:32])) < 1e-8
+
+
+def test_sinkhorn_mode(gpu_ctx):
+    X, eps, out, F = _factor(gpu_ctx, n=1200, rank=150, seed=2)
+    k = 16
+    res = gpu_ctx.evd(out.rank, k_out=k, prec="fp64", norm="sinkhorn", sinkhorn_tol=1e-12,
+                      sinkhorn_max_iter=5000)
+    lam, U, s, it = orc.lowrank_sinkhorn_evd(F, k, tol=1e-12, max_iter=5000)
+    print("sinkhorn iters", res.sinkhorn_iters, it, "dev", res.rowsum_dev, flush=True)
+    assert res.rowsum_dev <= 1e-12
+    np.testing.assert_allclose(res.lam[-k:], lam[-k:], rtol=1e-8, atol=1e-12)
+    assert abs(res.lam[-1] - 1.0) < 1e-8
+
+
+def test_nonpositive_degree_status(gpu_ctx):
+    X = gauss_points(400, 3, seed=8)
+    gpu_ctx.set_points(X)
+    F = np.zeros((400, 5))
+    F[:, 0] = 1.0
+    F[7, 0] = -3.0  # d~_7 = F_7 . (F^T 1) < 0
+    gpu_ctx.set_factor(F)
+    from kernel_evd_amd import KevdError
+    with pytest.raises(KevdError) as e:
+        gpu_ctx.evd(5, k_out=2, prec="fp64")
+    assert e.value.status == 2
