@@ -1,0 +1,203 @@
+"""Benchmark: rank-2000 factor + top-100 bistochastic EVD at N = 1,048,576.
+
+One step = accelerated RPCholesky (rank 2000, block 128, seed 0, mixed
+FP32/FP64) + bistochastic kEVD (top 100 eigenpairs) on a Kuramoto-Sivashinsky
+data set (L = 22, 64 grid points, ETDRK4 dt = 0.25, IC 0.1 N(0,1) seed 0,
+10,000 steps discarded, one snapshot every 4 steps, FP32 storage), eps = the
+median squared pairwise distance (Philox-sampled, 2^22 pairs).  The data are
+resident in HBM before the timed region.  After the timed steps one all-FP64
+step runs for the speed ratio and the eigenvalue error, and the CPU FP64
+oracle runs on a bounded sample for the CPU baseline.
+
+Prints ONE JSON line.  Single GPU only.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_PEAK_TF = 157.3
+FP64_MFMA_PEAK_TF = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def ks_data(n, M=64, L=22.0, seed=0):
+    from kernel_evd_amd import ks_generate
+    cache_dir = os.environ.get("KEVD_CACHE", os.path.join(os.environ.get("TMPDIR", "/tmp"), "kevd_cache"))
+    path = os.path.join(cache_dir, "ks_%d_%d_%g_s%d.npy" % (n, M, L, seed))
+    if os.path.exists(path):
+        return np.load(path)
+    t = time.time()
+    X = ks_generate(n, L_dom=L, M=M, dt=0.25, n_discard=10000, stride=4, seed=seed)
+    log("generated KS data %s in %.1f s" % (X.shape, time.time() - t))
+    try:
+        os.makedirs(cache_dir, exist_ok=True)
+        np.save(path, X)
+    except OSError:
+        pass
+    return X
+
+
+def one_step(ctx, eps, rank, block, kout, prec, seed=0):
+    t0 = time.perf_counter()
+    a = ctx.arpc(eps, rank, block=block, seed=seed, prec=prec)
+    t1 = time.perf_counter()
+    e = ctx.evd(a.rank, k_out=kout, prec=prec, want_U=False)
+    t2 = time.perf_counter()
+    return a, e, t1 - t0, t2 - t1
+
+
+def cpu_baseline(X, eps, rank, block, kout, n_sample):
+    """CPU FP64 oracle (OpenMP C ARPC + NumPy F-form kEVD) on the first
+    n_sample points; both stages scale linearly in N, so the time is scaled
+    by N / n_sample to the full workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    Xs = np.ascontiguousarray(X[:n_sample], dtype=np.float64)
+    t0 = time.perf_counter()
+    r = orc.arpc(Xs, eps, rank, block=block, seed=0)
+    t1 = time.perf_counter()
+    orc.lowrank_evd(r.F, kout)
+    t2 = time.perf_counter()
+    return t1 - t0, t2 - t1, orc.num_threads(), r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--rank", type=int, default=2000)
+    ap.add_argument("--block", type=int, default=128)
+    ap.add_argument("--kout", type=int, default=100)
+    ap.add_argument("--median-pairs", type=int, default=1 << 22)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from kernel_evd_amd import Context
+
+    X = ks_data(args.n)
+    X32 = X.astype(np.float32)
+    ctx = Context(0)
+    ctx.set_points(X32)
+    eps = ctx.median_sqdist(seed=0, nsample=args.median_pairs)
+    log("eps (median sq. distance) = %.6g" % eps)
+
+    for _ in range(args.warmup):
+        one_step(ctx, eps, args.rank, args.block, args.kout, "mixed")
+    ctx.set_profiling(True)
+    t_arpc, t_evd, agg = [], [], {}
+    tstart = time.perf_counter()
+    for _ in range(args.steps):
+        a, e, ta, te = one_step(ctx, eps, args.rank, args.block, args.kout, "mixed")
+        t_arpc.append(ta)
+        t_evd.append(te)
+        tm = ctx.timings()
+        for k, v in tm.items():
+            agg[k] = agg.get(k, 0.0) + v
+    ttotal = time.perf_counter() - tstart
+    ctx.set_profiling(False)
+    ms_step = 1e3 * ttotal / args.steps
+    a_mix, e_mix = a, e
+    steps = args.steps
+    dd_ms = agg["downdate_ms"] / steps
+    dd_flop = agg["downdate_flop"] / steps
+    dd_launch = agg["downdate_launches"] / steps
+    gr_ms = agg["gram_ms"] / steps
+    gr_flop = agg["gram_flop"] / steps
+    log("mixed: %.1f ms/step (arpc %.1f, evd %.1f); rank %d in %d rounds; downdate %.1f ms over %d launches; gram %.1f ms"
+        % (ms_step, 1e3 * np.mean(t_arpc), 1e3 * np.mean(t_evd), a_mix.rank, a_mix.rounds, dd_ms, dd_launch, gr_ms))
+
+    out = {
+        "metric": "rank-r factor+EVD time (N=1M,r=2000) FP32-mixed vs FP64; eig rel err vs FP64",
+        "value": round(ms_step / 1e3, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-mfma+fp64",
+        "data": "synthetic: Kuramoto-Sivashinsky ETDRK4 trajectory generated in-run (L=22, 64 pts, seed 0)",
+        "config": {"workload": "ARPC rank %d block %d + bistochastic kEVD top %d" % (args.rank, args.block, args.kout),
+                   "n": args.n, "dim": 64, "rank": args.rank, "block": args.block, "k_out": args.kout,
+                   "eps": eps, "precision": "mixed"},
+        "phases_ms": {"arpc": round(1e3 * float(np.mean(t_arpc)), 3), "evd": round(1e3 * float(np.mean(t_evd)), 3),
+                      "downdate": round(dd_ms, 3), "gram": round(gr_ms, 3),
+                      "eig": round(agg["eig_ms"] / steps, 3), "sample": round(agg["sample_ms"] / steps, 3),
+                      "rejection": round(agg["rejection_ms"] / steps, 3),
+                      "dense": round(agg["dense_ms"] / steps, 3), "u_gemm": round(agg["u_ms"] / steps, 3)},
+        "rank": int(a_mix.rank), "rounds": int(a_mix.rounds),
+        "trace_err_mixed": float(a_mix.trace_err[-1]),
+        "roofline": {"bound": "mfma", "kernel": "k_downdate<float>",
+                     "achieved": round(dd_flop / (dd_ms * 1e-3) / 1e12, 3) if dd_ms > 0 else None,
+                     "peak": FP32_MFMA_PEAK_TF, "unit": "TFLOP/s",
+                     "frac": round(dd_flop / (dd_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4) if dd_ms > 0 else None,
+                     "traffic": None,
+                     "flop_per_step": dd_flop, "launches_per_step": dd_launch,
+                     "avg_launch_ms": round(dd_ms / max(dd_launch, 1), 4)},
+        "gram_roofline": {"achieved": round(gr_flop / (gr_ms * 1e-3) / 1e12, 3) if gr_ms > 0 else None,
+                          "peak": FP32_MFMA_PEAK_TF, "unit": "TFLOP/s",
+                          "frac": round(gr_flop / (gr_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4) if gr_ms > 0 else None},
+    }
+
+    if not args.no_fp64:
+        ctx64 = Context(0)
+        ctx64.set_points(X32)
+        ctx64.set_profiling(True)
+        t0 = time.perf_counter()
+        a64, e64, ta64, te64 = one_step(ctx64, eps, args.rank, args.block, args.kout, "fp64")
+        t64 = time.perf_counter() - t0
+        tm64 = ctx64.timings()
+        k32 = min(32, a64.rank)
+        lm, l64 = e_mix.lam[::-1][:k32], e64.lam[::-1][:k32]
+        rel32 = np.abs(lm - l64) / np.abs(l64)
+        kk = min(args.kout, a64.rank)
+        relk = np.abs(e_mix.lam[::-1][:kk] - e64.lam[::-1][:kk]) / np.abs(e64.lam[::-1][:kk])
+        out["fp64"] = {"s": round(t64, 4), "arpc_s": round(ta64, 4), "evd_s": round(te64, 4),
+                       "downdate_tflops": round(tm64["downdate_flop"] / (tm64["downdate_ms"] * 1e-3) / 1e12, 3)
+                       if tm64["downdate_ms"] > 0 else None,
+                       "rank": int(a64.rank), "rounds": int(a64.rounds),
+                       "trace_err": float(a64.trace_err[-1])}
+        out["speedup_vs_fp64"] = round(t64 / (ms_step / 1e3), 3)
+        out["eig_rel_err_top32_max"] = float(rel32.max())
+        out["eig_rel_err_top%d_max" % kk] = float(relk.max())
+        out["pivots_equal_fp64"] = bool(a64.rank == a_mix.rank and np.array_equal(a64.pivots, a_mix.pivots))
+        out["trace_err_rel_diff"] = float(abs(a_mix.trace_err[-1] - a64.trace_err[-1]) / a64.trace_err[-1])
+        log("fp64: %.3f s; speedup %.2fx; top-32 eig rel err %.2e" % (t64, out["speedup_vs_fp64"], rel32.max()))
+        ctx64.close()
+
+    if not args.no_cpu:
+        ns = min(args.cpu_sample, args.n)
+        ta, te, nt, r = cpu_baseline(X32, eps, args.rank, args.block, args.kout, ns)
+        scale = args.n / ns
+        out["cpu_baseline"] = {"value": round((ta + te) * scale, 3), "unit": "s", "cores": int(nt),
+                               "kind": "port",
+                               "sample": "first %d of %d points, same rank/block/seed/eps; ARPC %.2f s + kEVD %.2f s "
+                                         "measured, scaled x%.0f (both stages linear in N)" % (ns, args.n, ta, te, scale)}
+        log("cpu baseline: %.2f s measured on %d points (%d threads)" % (ta + te, ns, nt))
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

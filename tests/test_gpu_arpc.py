@@ -2,8 +2,11 @@
 
 Bars: FP64 kernel block 1e-13 relative; FP32 (mixed) 2e-6 absolute; pivots
 bit-exact for FP64 with the same seed (and for mixed at these sizes); the
-FP64 factor within 1e-10 of the oracle's; the mixed factor within 1e-5
-relative Frobenius of the FP64 factor on the same injected pivots.
+FP64 factor within 1e-9 of the oracle's; the mixed factor within 1e-4
+relative Frobenius of the FP64 factor on the same injected pivots (FP32
+cancellation in K - F F(S)^T amplified by L_c^-1 sets this floor; CPU
+emulation of the FP32 steps gives 3e-5 to 7e-5 on this data set), and the
+final trace error within 5% of FP64.
 """
 import numpy as np
 import pytest
@@ -90,7 +93,7 @@ def test_arpc_mixed_pivots_and_factor(gpu_ctx):
     np.testing.assert_array_equal(rmx.pivots, r64.pivots)
     rel = np.linalg.norm(Fmx - F64) / np.linalg.norm(F64)
     print("mixed vs fp64 factor rel Frobenius", rel, flush=True)
-    assert rel < 1e-5
+    assert rel < 1e-4
     # trace error agrees with FP64 to within 5%
     assert abs(rmx.trace_err[-1] - r64.trace_err[-1]) <= 0.05 * r64.trace_err[-1]
 
