@@ -157,12 +157,13 @@ __global__ void k_weights(const double* x, int64_t n, int mode, double* w)
 // G_g = F^T diag(w_g) F for NG weight vectors, lower-triangle tiles.
 // Block tile BI x BJ of the r x r output, K = rows of F.  The BI panel is
 // staged into LDS pre-multiplied by each w_g (so the inner loop is LDS
-// read + MFMA only), the BJ panel as is.  Each lane feeds one K value per
-// MFMA: A(m, k) = Fw(k, I + m), lane (li, h) reads LDS row k0 + h, column
-// li -- consecutive lanes hit consecutive words.  Every PROMOTE rows the
-// FP32/FP64 MFMA accumulators are added into this block's FP64 partial
-// tile (a private slice of `part`), which bounds the length of any single
-// low-precision accumulation.
+// read + MFMA only), the BJ panel as is, both row-major [k][col] exactly as
+// F lies in HBM.  Each lane feeds one K value per MFMA; it reads IL = 2
+// adjacent columns with one 8/16-byte LDS load, which serve two MFMA tiles
+// whose rows (columns) interleave: tile q of a pair covers panel columns
+// IL*i + q.  Every PROMOTE rows the MFMA accumulators are added into this
+// block's FP64 partial tile (a private slice of `part`), which bounds the
+// length of any single FP32 accumulation.
 template <typename T> struct GramCfg;
 template <> struct GramCfg<float> {
     static constexpr int BI = 128, BJ = 128, KC = 16, TM = 2, TN = 2, PROMOTE = 1024;
@@ -176,15 +177,19 @@ struct GramTile {
     typedef GramCfg<T> C;
     typedef Mfma<T> M;
     static constexpr int BI = C::BI, BJ = C::BJ, KC = C::KC, TM = C::TM, TN = C::TN;
+    static constexpr int IL = 2;                            // interleaved tiles per load
     static constexpr int EPV = 16 / (int)sizeof(T);
     static constexpr int PI = BI + EPV, PJ = BJ + EPV;
     static constexpr int STAGE = NG * KC * PI + KC * PJ;
     static constexpr int VI = BI / EPV, VJ = BJ / EPV;     // 16 B vectors per panel row
     static constexpr int NVI = (KC * VI + 255) / 256, NVJ = (KC * VJ + 255) / 256;
+    static constexpr int WR = TM * M::MT, WC = TN * M::MT; // wave tile
+    typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type pair_t;
+    typedef T wt_t;                                         // weight type = compute type
 
     typename M::acc_t acc[NG][TM][TN];
     uint4 ri[NVI], rj[NVJ];
-    double wr[NG][NVI];
+    wt_t wr[NG][NVI];
     int wm, wn, h, li;
 
     __device__ inline void zero()
@@ -211,7 +216,7 @@ struct GramTile {
 
     __device__ inline void gload(const T* __restrict__ F, int64_t ldf, int64_t r, int64_t ci,
                                  int64_t cj, int64_t k0, int64_t k_end,
-                                 const double* __restrict__ w1, const double* __restrict__ w2)
+                                 const wt_t* __restrict__ w1, const wt_t* __restrict__ w2)
     {
 #pragma unroll
         for (int j = 0; j < NVI; ++j) {
@@ -219,8 +224,8 @@ struct GramTile {
             int kr = lin / VI, cv = lin % VI;
             int64_t row = k0 + kr, col = ci + cv * EPV;
             ri[j] = make_uint4(0, 0, 0, 0);
-            wr[0][j] = 0.0;
-            if (NG > 1) wr[NG - 1][j] = 0.0;
+            wr[0][j] = (wt_t)0;
+            if (NG > 1) wr[NG - 1][j] = (wt_t)0;
             if (lin < KC * VI && row < k_end && col < r) {
                 ri[j] = *reinterpret_cast<const uint4*>(F + row * ldf + col);
                 mask(ri[j], col, r);
@@ -253,7 +258,7 @@ struct GramTile {
                 for (int g = 0; g < NG; ++g) {
                     T o[EPV];
 #pragma unroll
-                    for (int q = 0; q < EPV; ++q) o[q] = (T)((double)e[q] * wr[g][j]);
+                    for (int q = 0; q < EPV; ++q) o[q] = e[q] * wr[g][j];
                     *reinterpret_cast<uint4*>(st + g * KC * PI + kr * PI + cv * EPV) =
                         *reinterpret_cast<const uint4*>(o);
                 }
@@ -274,29 +279,39 @@ struct GramTile {
 #pragma unroll
         for (int s = 0; s < KC / M::NH; ++s) {
             int kr = s * M::NH + h;
-            T a[NG][TM], b[TN];
+            pair_t a[NG][TM / IL], b[TN / IL];
 #pragma unroll
             for (int g = 0; g < NG; ++g)
 #pragma unroll
-                for (int x = 0; x < TM; ++x)
-                    a[g][x] = cur[g * KC * PI + kr * PI + wm * TM * M::MT + x * M::MT + li];
+                for (int x = 0; x < TM / IL; ++x)
+                    a[g][x] = *reinterpret_cast<const pair_t*>(
+                        cur + g * KC * PI + kr * PI + wm * WR + x * M::MT * IL + IL * li);
 #pragma unroll
-            for (int y = 0; y < TN; ++y) b[y] = cur[NG * KC * PI + kr * PJ + wn * TN * M::MT + y * M::MT + li];
+            for (int y = 0; y < TN / IL; ++y)
+                b[y] = *reinterpret_cast<const pair_t*>(
+                    cur + NG * KC * PI + kr * PJ + wn * WC + y * M::MT * IL + IL * li);
 #pragma unroll
             for (int g = 0; g < NG; ++g)
 #pragma unroll
-                for (int x = 0; x < TM; ++x)
+                for (int x = 0; x < TM / IL; ++x)
 #pragma unroll
-                    for (int y = 0; y < TN; ++y) M::mma(acc[g][x][y], a[g][x], b[y]);
+                    for (int y = 0; y < TN / IL; ++y) {
+                        M::mma(acc[g][IL * x][IL * y], a[g][x].x, b[y].x);
+                        M::mma(acc[g][IL * x][IL * y + 1], a[g][x].x, b[y].y);
+                        M::mma(acc[g][IL * x + 1][IL * y], a[g][x].y, b[y].x);
+                        M::mma(acc[g][IL * x + 1][IL * y + 1], a[g][x].y, b[y].y);
+                    }
         }
     }
 
+    // tile a = IL*x + q covers panel rows wm*WR + x*MT*IL + IL*i + q (i =
+    // MFMA output row), likewise columns.
     __device__ inline void promote(double* out, bool first)
     {
         // lane base + compile-time offsets; the empty asm makes the base
         // opaque so the 64-bit addresses are not hoisted out of the K loop
         // (which would pin one register pair per accumulator value)
-        double* base = out + (int64_t)(wm * TM * M::MT + M::row_lane(h)) * BJ + wn * TN * M::MT + li;
+        double* base = out + (int64_t)(wm * WR + IL * M::row_lane(h)) * BJ + wn * WC + IL * li;
         asm volatile("" : "+v"(base));
 #pragma unroll
         for (int g = 0; g < NG; ++g)
@@ -306,8 +321,9 @@ struct GramTile {
                 for (int b = 0; b < TN; ++b)
 #pragma unroll
                     for (int v = 0; v < M::ACC; ++v) {
-                        int off = g * BI * BJ + (a * M::MT + M::row_off(v)) * BJ + b * M::MT;
-                        double* p = base + off;
+                        int row = (a / IL) * M::MT * IL + IL * M::row_off(v) + (a % IL);
+                        int col = (b / IL) * M::MT * IL + (b % IL);
+                        double* p = base + g * BI * BJ + row * BJ + col;
                         double x = (double)acc[g][a][b][v];
                         *p = first ? x : *p + x;
                     }
@@ -316,9 +332,10 @@ struct GramTile {
 };
 
 template <typename T, int NG>
-__global__ __launch_bounds__(256) void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n,
-                                              int64_t r, const double* __restrict__ w1,
-                                              const double* __restrict__ w2,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n,
+                                              int64_t r, const T* __restrict__ w1,
+                                              const T* __restrict__ w2,
                                               const int* __restrict__ tileI,
                                               const int* __restrict__ tileJ, int ntiles,
                                               int64_t rows_per_split, double* part)
@@ -362,6 +379,13 @@ __global__ __launch_bounds__(256) void k_gram(const T* __restrict__ F, int64_t l
         __syncthreads();
     }
     if (since > 0 || first) g.promote(out, first);
+}
+
+template <typename T>
+__global__ void k_cast(const double* x, int64_t n, T* out)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = (T)x[i];
 }
 
 // G_g(i, j) = sum_split part[split][tile][g](i, j) for the lower triangle,
@@ -546,6 +570,13 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
     nsplit = (int)((n + rps - 1) / rps);
     int *dTI, *dTJ;
     double* part;
+    T *wt1 = nullptr, *wt2 = nullptr;
+    KEVD_TRY(ctx_buf_t(ctx, "e_gw1", n, &wt1));
+    k_cast<T><<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(w1, n, wt1);
+    if (NG > 1) {
+        KEVD_TRY(ctx_buf_t(ctx, "e_gw2", n, &wt2));
+        k_cast<T><<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(w2, n, wt2);
+    }
     KEVD_TRY(ctx_buf_t(ctx, "e_gTI", ntiles, &dTI));
     KEVD_TRY(ctx_buf_t(ctx, "e_gTJ", ntiles, &dTJ));
     KEVD_TRY(ctx_buf_t(ctx, "e_gpart", (size_t)nsplit * ntiles * NG * C::BI * C::BJ, &part));
@@ -553,8 +584,8 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
     KEVD_HIP(ctx, hipMemcpyAsync(dTJ, TJ.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     {
         ProfScope ps(ctx, TC_GRAM);
-        k_gram<T, NG><<<dim3(ntiles, nsplit), 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, w1,
-                                                                    w2, dTI, dTJ, ntiles, rps, part);
+        k_gram<T, NG><<<dim3(ntiles, nsplit), 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, wt1,
+                                                                    wt2, dTI, dTJ, ntiles, rps, part);
     }
     k_gram_reduce<C::BI, C::BJ, NG><<<ntiles, 256, 0, ctx->stream>>>(part, nsplit, ntiles, dTI, dTJ, r,
                                                                     G1, G2);
