@@ -205,6 +205,13 @@ typedef struct {
  * Host-side generator (the trajectory is sequential in time). */
 kevd_status kevd_ks_generate(const kevd_ks_params* p, int64_t n, double* out);
 
+/* Delay embedding in the reference's point order (ks_arpclm.cu:263-268):
+ * from a series of ndata snapshots (row-major ndata x M, FP64) build
+ * N*M points of dimension nq, N = ndata - nq + 1, point g*N + t being the
+ * window series[t .. t+nq-1][g].  Host-side. */
+kevd_status kevd_delay_embed(const double* series, int64_t ndata, int32_t M, int32_t nq,
+                             double* out);
+
 /* --------------------------------------------------------------- timing */
 
 typedef struct {

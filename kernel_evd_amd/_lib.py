@@ -71,6 +71,7 @@ SIGNATURES = {
     "kevd_bandwidth_sweep": (C.c_int, [C.c_void_p, P(i64), i64, P(dbl), i32, C.c_int, P(dbl)]),
     "kevd_bandwidth_post": (C.c_int, [P(dbl), P(dbl), i32, P(dbl), P(dbl), P(dbl), P(dbl)]),
     "kevd_ks_generate": (C.c_int, [P(KsParams), i64, P(dbl)]),
+    "kevd_delay_embed": (C.c_int, [P(dbl), i64, i32, i32, P(dbl)]),
     "kevd_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "kevd_get_timings": (C.c_int, [C.c_void_p, P(Timings)]),
 }

@@ -218,3 +218,17 @@ def ks_generate(n, L_dom=22.0, M=64, dt=0.25, n_discard=10000, stride=4, seed=0,
     if s != 0:
         raise KevdError(s, "ks_generate: bad parameters")
     return out
+
+
+def delay_embed(series, nq):
+    """Delay embedding in the reference's point order: point g*N + t is the
+    window series[t:t+nq, g] (ks_arpclm.cu:263-268)."""
+    lib = L.lib()
+    series = np.ascontiguousarray(series, dtype=np.float64)
+    ndata, M = series.shape
+    N = ndata - nq + 1
+    out = np.empty((N * M, nq), dtype=np.float64)
+    s = lib.kevd_delay_embed(_p(series, C.c_double), ndata, M, nq, _p(out, C.c_double))
+    if s != 0:
+        raise KevdError(s, "delay_embed: need 1 <= nq <= ndata")
+    return out

@@ -240,3 +240,16 @@ extern "C" kevd_status kevd_ks_generate(const kevd_ks_params* p, int64_t n, doub
     }
     return KEVD_OK;
 }
+
+extern "C" kevd_status kevd_delay_embed(const double* series, int64_t ndata, int32_t M, int32_t nq,
+                                        double* out)
+{
+    if (!series || !out || M < 1 || nq < 1 || ndata < nq) return KEVD_EINVAL;
+    int64_t N = ndata - nq + 1;
+    for (int32_t g = 0; g < M; ++g)
+        for (int64_t t = 0; t < N; ++t) {
+            double* o = out + ((int64_t)g * N + t) * nq;
+            for (int32_t j = 0; j < nq; ++j) o[j] = series[(t + j) * M + g];
+        }
+    return KEVD_OK;
+}

@@ -1,0 +1,121 @@
+// kevd-arpc -- the ks_arpclm.cu stage with its constants as flags: points
+// file in; eps, rank, block size and seed as parameters; factor, pivot and
+// L^-1 files out; the reference's per-round table on stdout
+// (ks_arpclm.cu:1129-1136).
+//
+//   kevd-arpc --points X.npy (--eps E | --eps-median [--median-pairs P])
+//             --rank R [--block 128] [--seed 0] [--tol 1e-10]
+//             [--precision mixed|fp64] [--max-rounds K] [--inject-pivots S.npy]
+//             [--factor-out F.npy] [--pivots-out S.npy] [--linv-out Linv.npy]
+//             [--device 0]
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "args.h"
+#include "kevd.h"
+#include "npy.h"
+
+static int fail(kevd_ctx* c, kevd_status s, const char* what)
+{
+    std::fprintf(stderr, "%s: %s (%s)\n", what, kevd_status_string(s), c ? kevd_last_error(c) : "");
+    if (c) kevd_destroy(c);
+    return exit_code(s);
+}
+
+int main(int argc, char** argv)
+{
+    Args a(argc, argv);
+    if (a.has("help")) {
+        std::printf("kevd-arpc --points X.npy (--eps E | --eps-median) --rank R [--block 128] [--seed 0]\n"
+                    "          [--tol 1e-10] [--precision mixed|fp64] [--max-rounds K]\n"
+                    "          [--inject-pivots S.npy] [--factor-out F.npy] [--pivots-out S.npy]\n"
+                    "          [--linv-out Linv.npy] [--median-pairs 4194304] [--device 0]\n");
+        return 0;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    NpyArray X;
+    try {
+        X = npy_load(a.need("points"));
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 1;
+    }
+    if (X.shape.size() != 2) {
+        std::fprintf(stderr, "--points must be a 2-D array\n");
+        return 1;
+    }
+    kevd_ctx* c = nullptr;
+    kevd_status s = kevd_create((int)a.integer("device", 0), &c);
+    if (s != KEVD_OK) return fail(nullptr, s, "kevd_create");
+    if (X.dtype == "<f4")
+        s = kevd_set_points_f32(c, reinterpret_cast<const float*>(X.data.data()), X.rows(), (int32_t)X.cols());
+    else {
+        std::vector<double> x = npy_as_f64(X);
+        s = kevd_set_points(c, x.data(), X.rows(), (int32_t)X.cols());
+    }
+    if (s != KEVD_OK) return fail(c, s, "kevd_set_points");
+    double eps = a.num("eps", 0.0);
+    if (a.has("eps-median") || !(eps > 0.0)) {
+        int64_t pairs = a.integer("median-pairs", (int64_t)1 << 22);
+        s = kevd_median_sqdist(c, (uint64_t)a.integer("seed", 0), pairs, &eps);
+        if (s != KEVD_OK) return fail(c, s, "kevd_median_sqdist");
+    }
+    kevd_arpc_params p{};
+    p.eps = eps;
+    p.rank = a.integer("rank", 0);
+    p.block = (int32_t)a.integer("block", 128);
+    p.seed = (uint64_t)a.integer("seed", 0);
+    p.tol = a.num("tol", 1e-10);
+    p.prec = a.str("precision", "mixed") == "fp64" ? KEVD_FP64 : KEVD_MIXED;
+    p.max_rounds = (int32_t)a.integer("max-rounds", 0);
+    std::vector<int64_t> inj;
+    if (a.has("inject-pivots")) {
+        inj = npy_as_i64(npy_load(a.str("inject-pivots")));
+        p.inject_pivots = inj.data();
+        p.n_inject = (int64_t)inj.size();
+    }
+    int32_t mr = p.max_rounds > 0 ? p.max_rounds : (int32_t)(4 * ((p.rank + p.block - 1) / (p.block > 0 ? p.block : 1)) + 64);
+    std::vector<int64_t> piv((size_t)(p.rank > 0 ? p.rank : 1));
+    std::vector<int32_t> acc((size_t)mr);
+    std::vector<double> err((size_t)mr);
+    kevd_arpc_result r{};
+    r.pivots = piv.data();
+    r.accepted = acc.data();
+    r.trace_err = err.data();
+    r.rounds_cap = mr;
+    std::printf("Points: %lld x %lld  eps %.6g  rank %lld  block %d  seed %llu  precision %s\n\n",
+                (long long)X.rows(), (long long)X.cols(), eps, (long long)p.rank, p.block,
+                (unsigned long long)p.seed, p.prec == KEVD_FP64 ? "fp64" : "mixed");
+    s = kevd_arpc(c, &p, &r);
+    if (s != KEVD_OK) return fail(c, s, "kevd_arpc");
+    std::printf("Accepted rank: %lld in %d rounds\n", (long long)r.rank, r.rounds);
+    std::printf("# round accepted cumrank relerr\n");
+    long long cum = 0;
+    for (int i = 0; i < r.rounds; ++i) {
+        cum += acc[i];
+        std::printf("%d %d %lld %.5e\n", i, acc[i], cum, err[i]);
+    }
+    if (a.has("factor-out")) {
+        bool f32 = p.prec == KEVD_MIXED;
+        std::vector<char> F((size_t)X.rows() * r.rank * (f32 ? 4 : 8));
+        s = kevd_get_factor(c, F.data(), r.rank, p.prec);
+        if (s != KEVD_OK) return fail(c, s, "kevd_get_factor");
+        npy_save(a.str("factor-out"), f32 ? "<f4" : "<f8", {X.rows(), r.rank}, F.data());
+    }
+    if (a.has("pivots-out")) npy_save(a.str("pivots-out"), "<i8", {r.rank}, piv.data());
+    if (a.has("linv-out")) {
+        std::vector<double> L((size_t)r.rank * r.rank);
+        s = kevd_get_linv(c, L.data());
+        if (s != KEVD_OK) return fail(c, s, "kevd_get_linv");
+        // column-major r x r, flattened (the reference's /linv layout)
+        npy_save(a.str("linv-out"), "<f8", {r.rank * r.rank}, L.data());
+    }
+    kevd_timings tm;
+    kevd_get_timings(c, &tm);
+    kevd_destroy(c);
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("ARPC time: %.4f s  Total time: %.2f s\n", tm.arpc_ms / 1e3, secs);
+    return 0;
+}

@@ -1,0 +1,83 @@
+"""The C++ drivers (the reference's program-level interface: points file
+in, parameters as flags, factor / pivot / eigenpair / bw0.out files out)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from kernel_evd_amd import delay_embed, ks_generate
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "kernel_evd_amd", "bin")
+
+
+def run(*args, check=True):
+    r = subprocess.run([os.path.join(BIN, args[0])] + [str(a) for a in args[1:]],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    if check and r.returncode != 0:
+        raise AssertionError("%s failed (%d):\n%s" % (args[0], r.returncode, r.stdout))
+    return r
+
+
+@pytest.mark.parametrize("exe", ["kevd-gen", "kevd-arpc", "kevd-evd", "kevd-bandwidth"])
+def test_help(exe):
+    assert run(exe, "--help").returncode == 0
+
+
+def test_gen_matches_library(tmp_path):
+    out = tmp_path / "X.npy"
+    run("kevd-gen", "--n", 300, "--discard", 200, "--seed", 4, "--out", out)
+    X = np.load(out)
+    assert X.shape == (300, 64)
+    assert np.array_equal(X, ks_generate(300, n_discard=200, seed=4))
+
+
+def test_delay_embedding_index_map(tmp_path):
+    out = tmp_path / "Xd.npy"
+    run("kevd-gen", "--n", 40, "--discard", 50, "--M", 32, "--delay", 8, "--out", out)
+    Xd = np.load(out)
+    U = ks_generate(47, M=32, n_discard=50)
+    assert Xd.shape == (40 * 32, 8)
+    # point g*N + t is the window U[t:t+8, g] (ks_arpclm.cu:263-268)
+    for g, t in [(0, 0), (5, 17), (31, 39)]:
+        assert np.array_equal(Xd[g * 40 + t], U[t:t + 8, g])
+    assert np.array_equal(Xd, delay_embed(U, 8))
+
+
+def test_missing_file_exit_code(tmp_path):
+    r = run("kevd-arpc", "--points", tmp_path / "nope.npy", "--rank", 4, check=False)
+    assert r.returncode == 1
+
+
+@pytest.mark.gpu
+def test_pipeline_end_to_end(tmp_path):
+    X = tmp_path / "X.npy"
+    run("kevd-gen", "--n", 3000, "--discard", 2000, "--seed", 0, "--out", X)
+    F, S, Li = tmp_path / "F.npy", tmp_path / "S.npy", tmp_path / "Linv.npy"
+    r = run("kevd-arpc", "--points", X, "--eps-median", "--rank", 200, "--block", 32,
+            "--precision", "fp64", "--factor-out", F, "--pivots-out", S, "--linv-out", Li)
+    print(r.stdout)
+    assert "# round accepted cumrank relerr" in r.stdout
+    Fa, Sa = np.load(F), np.load(S)
+    assert Fa.shape == (3000, 200) and Sa.shape == (200,)
+    pts = np.load(X)
+    eps = orc.median_sqdist(pts, seed=0, nsample=3000 * 2999 // 2)  # exact
+    lam_f = tmp_path / "lam.npy"
+    U_f = tmp_path / "U.npy"
+    r = run("kevd-evd", "--points", X, "--factor", F, "--kout", 16, "--precision", "fp64",
+            "--lam-out", lam_f, "--U-out", U_f)
+    lam = np.load(lam_f)
+    ref_lam, _, _, _ = orc.lowrank_evd(Fa, 16)
+    assert np.max(np.abs(lam[-16:] - ref_lam[-16:])) < 1e-9
+    assert np.load(U_f).shape == (3000, 16)
+    # rebuild from the pivot list (the reference's spiv-driven kEVD)
+    lam2_f = tmp_path / "lam2.npy"
+    run("kevd-evd", "--points", X, "--pivots", S, "--eps", repr(eps), "--kout", 16,
+        "--precision", "fp64", "--lam-out", lam2_f)
+    assert np.max(np.abs(np.load(lam2_f)[-16:] - lam[-16:])) < 1e-8
+    bw = tmp_path / "bw0.out"
+    run("kevd-bandwidth", "--points", X, "--rows", 500, "--grid", "ref36", "--out", bw)
+    lines = open(bw).read().strip().split("\n")
+    assert len(lines) == 34 + 1 + 2 and lines[34] == ""
