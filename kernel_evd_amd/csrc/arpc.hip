@@ -275,8 +275,11 @@ __global__ __launch_bounds__(256) void k_form_H(const double* __restrict__ X64, 
 // workgroup with H resident in LDS.  Every thread evaluates the acceptance
 // test itself (same Philox counter, same LDS values), so a rejected proposal
 // costs no barrier.  Column i of L is kept transposed in the unused upper
-// triangle of H, its diagonal in Ld; after the loop the compacted inverse
-// L_c^-1 is built by forward substitution in the lower triangle.
+// triangle of H, its diagonal in Ld.  Work inside a step is spread over a
+// 16 x 16 thread grid (no integer division).  After the loop the compacted
+// inverse X = L_c^-1 is formed in place in the lower triangle by a
+// right-looking triangular inversion (k sequential steps, each a parallel
+// rank-1 update) instead of one thread per column.
 template <typename T>
 __global__ __launch_bounds__(256) void k_rejchol(const double* __restrict__ H, ArpcState* st,
                                                  uint64_t seed, int round, double tol, int inject,
@@ -289,10 +292,9 @@ __global__ __launch_bounds__(256) void k_rejchol(const double* __restrict__ H, A
     __shared__ int accpos[BMAX];
     int np = st->nprop;
     int tid = threadIdx.x;
-    for (int e = tid; e < np * np; e += 256) {
-        int p = e / np, q = e % np;
-        Hs[p][q] = H[p * BMAX + q];
-    }
+    int tx = tid & 15, ty = tid >> 4;
+    for (int p = ty; p < np; p += 16)
+        for (int q = tx; q < np; q += 16) Hs[p][q] = H[p * BMAX + q];
     __syncthreads();
     if (tid < np) u[tid] = Hs[tid][tid];
     __syncthreads();
@@ -305,10 +307,9 @@ __global__ __launch_bounds__(256) void k_rejchol(const double* __restrict__ H, A
         double f = 1.0 / sqrt(hii);
         if (tid >= i && tid < np) Lcol[tid] = Hs[tid][i] * f;
         __syncthreads();
-        int w = np - i - 1;
-        for (int e = tid; e < w * w; e += 256) {
-            int jj = i + 1 + e / w, mm = i + 1 + e % w;
-            if (jj >= mm) Hs[jj][mm] -= Lcol[jj] * Lcol[mm];
+        for (int jj = i + 1 + ty; jj < np; jj += 16) {
+            double lj = Lcol[jj];
+            for (int mm = i + 1 + tx; mm <= jj; mm += 16) Hs[jj][mm] -= lj * Lcol[mm];
         }
         if (tid > i && tid < np) Hs[i][tid] = Lcol[tid];
         if (tid == i) Ld[i] = Lcol[i];
@@ -318,20 +319,23 @@ __global__ __launch_bounds__(256) void k_rejchol(const double* __restrict__ H, A
     }
     int64_t room = st->rank - st->Nr;
     if ((int64_t)k > room) k = (int)room;
-    // L_c(a, c) = L(accpos[a], accpos[c]) = Hs[accpos[c]][accpos[a]] (a > c),
-    // L_c(a, a) = Ld[accpos[a]].  Column t of L_c^-1 by thread t, stored at
-    // Hs[a][t] (a >= t, compacted indices; disjoint from the upper storage).
-    if (tid < k) {
-        int t = tid;
-        Hs[t][t] = 1.0 / Ld[accpos[t]];
-        for (int a = t + 1; a < k; ++a) {
-            int aa = accpos[a];
-            double s = 0.0;
-            for (int e = t; e < a; ++e) s += Hs[accpos[e]][aa] * Hs[e][t];
-            Hs[a][t] = -s / Ld[aa];
-        }
-    }
+    // L_c(a, c) = L(accpos[a], accpos[c]) = Hs[accpos[c]][accpos[a]] for a > c,
+    // L_c(a, a) = Ld[accpos[a]].  X = L_c^-1 lives at Hs[a][t], a >= t
+    // (compacted indices, strictly below the upper-triangle storage).
+    for (int a = ty; a < k; a += 16)
+        for (int t = tx; t <= a; t += 16) Hs[a][t] = (a == t) ? 1.0 : 0.0;
     __syncthreads();
+    for (int a = 0; a < k; ++a) {
+        double inv = 1.0 / Ld[accpos[a]];
+        if (tid <= a) Hs[a][tid] *= inv;  // finalise row a (k <= 128 < 256 threads)
+        __syncthreads();
+        int pa = accpos[a];
+        for (int j = a + 1 + ty; j < k; j += 16) {
+            double l = Hs[pa][accpos[j]];
+            for (int t = tx; t <= a; t += 16) Hs[j][t] -= l * Hs[a][t];
+        }
+        __syncthreads();
+    }
     for (int e = tid; e < BMAX * BMAX; e += 256) {
         int a = e / BMAX, c = e % BMAX;
         double v = (a < k && c <= a) ? Hs[a][c] : 0.0;
@@ -388,8 +392,9 @@ __global__ __launch_bounds__(256) void k_downdate(const T* __restrict__ X, int32
     constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
     constexpr int TP = BN + 16 / (int)sizeof(T);  // T tile pitch
     constexpr int LDS = G::LDS_ELEMS > BM * TP ? G::LDS_ELEMS : BM * TP;
+    constexpr int WN = BN / (G::NTN * Mfma<T>::MT);
     __shared__ __attribute__((aligned(16))) T lds[LDS];
-    __shared__ double rs[2][BM];
+    __shared__ double rs[WN][BM];
     if (st->done) return;
     int k = st->k;
     if (k == 0) return;
@@ -398,6 +403,7 @@ __global__ __launch_bounds__(256) void k_downdate(const T* __restrict__ X, int32
     int64_t rows = n - r0;
     G g;
     g.init();
+    g.set_live_cols(k);
     // distance product
     g.mainloop(lds, X + r0 * dp, dp, rows, XS, dp, BN, dp);
 #pragma unroll
@@ -409,7 +415,7 @@ __global__ __launch_bounds__(256) void k_downdate(const T* __restrict__ X, int32
                 int64_t row = r0 + g.row_of(a, v);
                 int col = g.col_of(b);
                 double kv = 0.0;
-                if (row < n) {
+                if (row < n && col < k) {
                     double D = nrm[row] + nrmS[col] - 2.0 * (double)g.acc[a][b][v];
                     D = D > 0.0 ? D : 0.0;
                     kv = (row == Si[col]) ? 1.0 : (double)gauss_val(D, gs, (T*)nullptr);
@@ -464,7 +470,10 @@ __global__ __launch_bounds__(256) void k_downdate(const T* __restrict__ X, int32
     for (int r = threadIdx.x; r < BM; r += 256) {
         int64_t row = r0 + r;
         if (row < n) {
-            double dd = d[row] - (rs[0][r] + rs[1][r]);
+            double sq = 0.0;
+#pragma unroll
+            for (int q = 0; q < WN; ++q) sq += rs[q][r];
+            double dd = d[row] - sq;
             d[row] = dd >= 0.0 ? dd : 0.0;
         }
     }

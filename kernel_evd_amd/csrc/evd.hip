@@ -305,40 +305,59 @@ struct GramTile {
     }
 
     // tile a = IL*x + q covers panel rows wm*WR + x*MT*IL + IL*i + q (i =
-    // MFMA output row), likewise columns.
+    // MFMA output row), likewise columns.  The partial tile is addressed
+    // through a buffer resource: the lane part of the offset is one VGPR, the
+    // compile-time part of each element's offset an SGPR constant, and the
+    // read-modify-write runs 16 values per batch so the loads overlap.
     __device__ inline void promote(double* out, bool first)
     {
-        // lane base + compile-time offsets; the empty asm makes the base
-        // opaque so the 64-bit addresses are not hoisted out of the K loop
-        // (which would pin one register pair per accumulator value)
-        double* base = out + (int64_t)(wm * WR + IL * M::row_lane(h)) * BJ + wn * WC + IL * li;
-        asm volatile("" : "+v"(base));
+        constexpr int NVAL = NG * TM * TN * M::ACC;
+        __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc(out, (short)0, NG * BI * BJ * (int)sizeof(double), 0x00020000);
+        int voff = ((wm * WR + IL * M::row_lane(h)) * BJ + wn * WC + IL * li) * (int)sizeof(double);
+        typedef __attribute__((ext_vector_type(2))) unsigned u2;
 #pragma unroll
-        for (int g = 0; g < NG; ++g)
+        for (int b0 = 0; b0 < NVAL; b0 += 16) {
+            double x[16];
 #pragma unroll
-            for (int a = 0; a < TM; ++a)
+            for (int q = 0; q < 16; ++q) {
+                int e = b0 + q;
+                int v = e % M::ACC, b = (e / M::ACC) % TN, a = (e / (M::ACC * TN)) % TM,
+                    g = e / (M::ACC * TN * TM);
+                x[q] = (double)acc[g][a][b][v];
+            }
+            if (!first) {
 #pragma unroll
-                for (int b = 0; b < TN; ++b)
+                for (int q = 0; q < 16; ++q) {
+                    int e = b0 + q;
+                    int v = e % M::ACC, b = (e / M::ACC) % TN, a = (e / (M::ACC * TN)) % TM,
+                        g = e / (M::ACC * TN * TM);
+                    int row = (a / IL) * M::MT * IL + IL * M::row_off(v) + (a % IL);
+                    int col = (b / IL) * M::MT * IL + (b % IL);
+                    int soff = (g * BI * BJ + row * BJ + col) * (int)sizeof(double);
+                    x[q] += __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+                }
+            }
 #pragma unroll
-                    for (int v = 0; v < M::ACC; ++v) {
-                        int row = (a / IL) * M::MT * IL + IL * M::row_off(v) + (a % IL);
-                        int col = (b / IL) * M::MT * IL + (b % IL);
-                        double* p = base + g * BI * BJ + row * BJ + col;
-                        double x = (double)acc[g][a][b][v];
-                        *p = first ? x : *p + x;
-                    }
+            for (int q = 0; q < 16; ++q) {
+                int e = b0 + q;
+                int v = e % M::ACC, b = (e / M::ACC) % TN, a = (e / (M::ACC * TN)) % TM,
+                    g = e / (M::ACC * TN * TM);
+                int row = (a / IL) * M::MT * IL + IL * M::row_off(v) + (a % IL);
+                int col = (b / IL) * M::MT * IL + (b % IL);
+                int soff = (g * BI * BJ + row * BJ + col) * (int)sizeof(double);
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x[q]), rs, voff, soff, 0);
+            }
+        }
         zero();
     }
 };
 
 template <typename T, int NG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n,
-                                              int64_t r, const T* __restrict__ w1,
-                                              const T* __restrict__ w2,
-                                              const int* __restrict__ tileI,
-                                              const int* __restrict__ tileJ, int ntiles,
-                                              int64_t rows_per_split, double* part)
+void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T* __restrict__ w1,
+            const T* __restrict__ w2, const int* __restrict__ tileI, const int* __restrict__ tileJ,
+            int ntiles, int64_t rows_per_split, double* part)
 {
     typedef GramTile<T, NG> G;
     __shared__ __attribute__((aligned(16))) T lds[2 * G::STAGE];
@@ -430,6 +449,7 @@ __global__ __launch_bounds__(256) void k_ugemm(const T* __restrict__ F, int64_t 
     g.init();
     int64_t r0 = (int64_t)blockIdx.x * Cfg<T>::BM;
     int c0 = blockIdx.y * Cfg<T>::BN;
+    g.set_live_cols(kc - c0);
     g.mainloop(lds, F + r0 * ldf, ldf, n - r0, Y + (int64_t)c0 * ldy, ldy, kc - c0, r);
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)

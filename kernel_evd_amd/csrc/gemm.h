@@ -33,6 +33,7 @@ struct NTGemm {
 
     acc_t acc[TM][TN];
     int wm, wn, h, li;
+    int nact = TN;  // column tiles of this wave that hold live columns (wave-uniform)
 
     __device__ inline void init()
     {
@@ -99,6 +100,15 @@ struct NTGemm {
         }
     }
 
+    // Restrict the MFMAs to the first `live_cols` block columns: column tiles
+    // past them are skipped (their accumulators stay zero).
+    __device__ inline void set_live_cols(int live_cols)
+    {
+        int first = wn * TN * MT;
+        int t = (live_cols - first + MT - 1) / MT;
+        nact = t < 0 ? 0 : (t > TN ? TN : t);
+    }
+
     // acc += As Bs^T over one KC chunk held in LDS.
     __device__ inline void compute(const T* As, const T* Bs)
     {
@@ -111,14 +121,16 @@ struct NTGemm {
                     As + (wm * TM * MT + t * MT + li) * PITCH + s * 8 + h * M::VEC);
 #pragma unroll
             for (int t = 0; t < TN; ++t)
-                b[t] = *reinterpret_cast<const vec_t*>(
-                    Bs + (wn * TN * MT + t * MT + li) * PITCH + s * 8 + h * M::VEC);
+                if (t < nact)
+                    b[t] = *reinterpret_cast<const vec_t*>(
+                        Bs + (wn * TN * MT + t * MT + li) * PITCH + s * 8 + h * M::VEC);
 #pragma unroll
-            for (int j = 0; j < M::VEC; ++j)
+            for (int y = 0; y < TN; ++y)
+                if (y < nact)
 #pragma unroll
-                for (int x = 0; x < TM; ++x)
+                    for (int j = 0; j < M::VEC; ++j)
 #pragma unroll
-                    for (int y = 0; y < TN; ++y) M::mma(acc[x][y], M::get(a[x], j), M::get(b[y], j));
+                        for (int x = 0; x < TM; ++x) M::mma(acc[x][y], M::get(a[x], j), M::get(b[y], j));
         }
     }
 
@@ -168,28 +180,33 @@ struct NTGemm {
                     Ts + (wm * TM * MT + t * MT + li) * tp + s * 8 + h * M::VEC);
 #pragma unroll
             for (int t = 0; t < TN; ++t)
-                b[t] = *reinterpret_cast<const vec_t*>(
-                    Bg + (int64_t)(wn * TN * MT + t * MT + li) * ldb + s * 8 + h * M::VEC);
+                if (t < nact)
+                    b[t] = *reinterpret_cast<const vec_t*>(
+                        Bg + (int64_t)(wn * TN * MT + t * MT + li) * ldb + s * 8 + h * M::VEC);
 #pragma unroll
-            for (int j = 0; j < M::VEC; ++j)
+            for (int y = 0; y < TN; ++y)
+                if (y < nact)
 #pragma unroll
-                for (int x = 0; x < TM; ++x)
+                    for (int j = 0; j < M::VEC; ++j)
 #pragma unroll
-                    for (int y = 0; y < TN; ++y) M::mma(acc[x][y], M::get(a[x], j), M::get(b[y], j));
+                        for (int x = 0; x < TM; ++x) M::mma(acc[x][y], M::get(a[x], j), M::get(b[y], j));
         }
     }
 };
 
-// Tile configurations.  FP32: 128 x 128 block, 2x2 waves of 64 x 64
-// (2x2 tiles of 32x32).  FP64: 64 x 128 block, 2x2 waves of 32 x 64
-// (2x4 tiles of 16x16).
+// Tile configurations.  The four waves split the block rows and each owns
+// all BN = 128 columns, so a narrower live column range (fewer accepted
+// pivots in a round, k_out + 1 < 128 eigenvectors) drops whole MFMA column
+// tiles evenly on every wave.  FP32: 128 x 128 block, waves of 32 x 128
+// (4 tiles of 32x32).  FP64: 64 x 128 block, waves of 16 x 128 (8 tiles of
+// 16x16).
 template <typename T> struct Cfg;
 template <> struct Cfg<float> {
-    typedef NTGemm<float, 128, 128, 32, 2, 2, 2, 2> G;
+    typedef NTGemm<float, 128, 128, 32, 4, 1, 1, 4> G;
     static constexpr int BM = 128, BN = 128, KC = 32;
 };
 template <> struct Cfg<double> {
-    typedef NTGemm<double, 64, 128, 16, 2, 2, 2, 4> G;
+    typedef NTGemm<double, 64, 128, 16, 4, 1, 1, 8> G;
     static constexpr int BM = 64, BN = 128, KC = 16;
 };
 

@@ -56,14 +56,14 @@ def test_pipeline_end_to_end(tmp_path):
     X = tmp_path / "X.npy"
     run("kevd-gen", "--n", 3000, "--discard", 2000, "--seed", 0, "--out", X)
     F, S, Li = tmp_path / "F.npy", tmp_path / "S.npy", tmp_path / "Linv.npy"
-    r = run("kevd-arpc", "--points", X, "--eps-median", "--rank", 200, "--block", 32,
+    pts = np.load(X)
+    eps = orc.median_sqdist(pts)  # exact
+    r = run("kevd-arpc", "--points", X, "--eps", repr(eps), "--rank", 200, "--block", 32,
             "--precision", "fp64", "--factor-out", F, "--pivots-out", S, "--linv-out", Li)
     print(r.stdout)
     assert "# round accepted cumrank relerr" in r.stdout
     Fa, Sa = np.load(F), np.load(S)
     assert Fa.shape == (3000, 200) and Sa.shape == (200,)
-    pts = np.load(X)
-    eps = orc.median_sqdist(pts, seed=0, nsample=3000 * 2999 // 2)  # exact
     lam_f = tmp_path / "lam.npy"
     U_f = tmp_path / "U.npy"
     r = run("kevd-evd", "--points", X, "--factor", F, "--kout", 16, "--precision", "fp64",
