@@ -147,6 +147,10 @@ typedef struct {
     double sinkhorn_tol;      /* max row-sum deviation target (Sinkhorn)    */
     int32_t sinkhorn_max_iter;
     kevd_precision prec;
+    int32_t all_eigenvalues;  /* 1: all r eigenvalues (full symmetric
+                                 eigensolve, the reference's /lam);
+                                 0: the leading ones only, by block subspace
+                                 iteration; lam entries not computed are NaN */
 } kevd_evd_params;
 
 typedef struct {

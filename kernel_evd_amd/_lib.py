@@ -31,7 +31,7 @@ class ArpcResult(C.Structure):
 
 class EvdParams(C.Structure):
     _fields_ = [("k_out", i32), ("norm", i32), ("sinkhorn_tol", dbl),
-                ("sinkhorn_max_iter", i32), ("prec", C.c_int)]
+                ("sinkhorn_max_iter", i32), ("prec", C.c_int), ("all_eigenvalues", i32)]
 
 
 class EvdResult(C.Structure):

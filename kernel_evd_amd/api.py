@@ -148,13 +148,14 @@ class Context:
 
     # -------------------------------------------------------------- kEVD
     def evd(self, rank, k_out=100, norm="ref", prec="mixed", sinkhorn_tol=1e-12,
-            sinkhorn_max_iter=1000, want_U=True):
+            sinkhorn_max_iter=1000, want_U=True, all_eigenvalues=False):
         p = L.EvdParams()
         p.k_out = k_out
         p.norm = KEVD_NORM_SINKHORN if norm == "sinkhorn" else KEVD_NORM_REF
         p.sinkhorn_tol = sinkhorn_tol
         p.sinkhorn_max_iter = sinkhorn_max_iter
         p.prec = PREC[prec]
+        p.all_eigenvalues = 1 if all_eigenvalues else 0
         lam = np.zeros(rank, dtype=np.float64)
         r = L.EvdResult()
         r.lam = _p(lam, C.c_double)

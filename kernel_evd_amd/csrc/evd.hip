@@ -15,6 +15,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -155,21 +156,22 @@ __global__ void k_weights(const double* x, int64_t n, int mode, double* w)
 // --------------------------------------------------------- weighted Gram
 
 // G_g = F^T diag(w_g) F for NG weight vectors, lower-triangle tiles.
-// Block tile BI x BJ of the r x r output, K = rows of F.  The BI panel is
-// staged into LDS pre-multiplied by each w_g (so the inner loop is LDS
-// read + MFMA only), the BJ panel as is, both row-major [k][col] exactly as
-// F lies in HBM.  Each lane feeds one K value per MFMA; it reads IL = 2
-// adjacent columns with one 8/16-byte LDS load, which serve two MFMA tiles
-// whose rows (columns) interleave: tile q of a pair covers panel columns
-// IL*i + q.  Every PROMOTE rows the MFMA accumulators are added into this
-// block's FP64 partial tile (a private slice of `part`), which bounds the
-// length of any single FP32 accumulation.
+// Block tile BI x BJ of the r x r output, K = rows of F.  Both panels are
+// staged into LDS unweighted, row-major [k][col] exactly as F lies in HBM,
+// with the chunk's row weights beside them; a lane scales its A operand by
+// w_g(k) in registers, so one staged copy serves all NG Grams.  Each lane
+// feeds one K value per MFMA; it reads IL = 2 adjacent columns with one
+// 8/16-byte LDS load, which serve two MFMA tiles whose rows (columns)
+// interleave: tile q of a pair covers panel columns IL*i + q.  Every
+// PROMOTE rows the MFMA accumulators are added into this block's FP64
+// partial tile (a private slice of `part`), which bounds the length of any
+// single FP32 accumulation.
 template <typename T> struct GramCfg;
 template <> struct GramCfg<float> {
-    static constexpr int BI = 128, BJ = 128, KC = 16, TM = 2, TN = 2, PROMOTE = 1024;
+    static constexpr int BI = 128, BJ = 128, KC = 32, TM = 2, TN = 2, PROMOTE = 1024;
 };
 template <> struct GramCfg<double> {
-    static constexpr int BI = 128, BJ = 64, KC = 8, TM = 4, TN = 2, PROMOTE = 1 << 30;
+    static constexpr int BI = 128, BJ = 64, KC = 16, TM = 4, TN = 2, PROMOTE = 1 << 30;
 };
 
 template <typename T, int NG>
@@ -180,16 +182,16 @@ struct GramTile {
     static constexpr int IL = 2;                            // interleaved tiles per load
     static constexpr int EPV = 16 / (int)sizeof(T);
     static constexpr int PI = BI + EPV, PJ = BJ + EPV;
-    static constexpr int STAGE = NG * KC * PI + KC * PJ;
+    static constexpr int WOFF = KC * PI + KC * PJ;          // weights after the panels
+    static constexpr int STAGE = WOFF + NG * KC;
     static constexpr int VI = BI / EPV, VJ = BJ / EPV;     // 16 B vectors per panel row
     static constexpr int NVI = (KC * VI + 255) / 256, NVJ = (KC * VJ + 255) / 256;
     static constexpr int WR = TM * M::MT, WC = TN * M::MT; // wave tile
     typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type pair_t;
-    typedef T wt_t;                                         // weight type = compute type
 
     typename M::acc_t acc[NG][TM][TN];
     uint4 ri[NVI], rj[NVJ];
-    wt_t wr[NG][NVI];
+    T wv;                                                   // this thread's staged weight
     int wm, wn, h, li;
 
     __device__ inline void zero()
@@ -216,7 +218,7 @@ struct GramTile {
 
     __device__ inline void gload(const T* __restrict__ F, int64_t ldf, int64_t r, int64_t ci,
                                  int64_t cj, int64_t k0, int64_t k_end,
-                                 const wt_t* __restrict__ w1, const wt_t* __restrict__ w2)
+                                 const T* __restrict__ w1, const T* __restrict__ w2)
     {
 #pragma unroll
         for (int j = 0; j < NVI; ++j) {
@@ -224,13 +226,9 @@ struct GramTile {
             int kr = lin / VI, cv = lin % VI;
             int64_t row = k0 + kr, col = ci + cv * EPV;
             ri[j] = make_uint4(0, 0, 0, 0);
-            wr[0][j] = (wt_t)0;
-            if (NG > 1) wr[NG - 1][j] = (wt_t)0;
             if (lin < KC * VI && row < k_end && col < r) {
                 ri[j] = *reinterpret_cast<const uint4*>(F + row * ldf + col);
                 mask(ri[j], col, r);
-                wr[0][j] = w1[row];
-                if (NG > 1) wr[NG - 1][j] = w2[row];
             }
         }
 #pragma unroll
@@ -244,6 +242,13 @@ struct GramTile {
                 mask(rj[j], col, r);
             }
         }
+        // threads [0, NG*KC) stage the chunk's row weights
+        wv = (T)0;
+        if ((int)threadIdx.x < NG * KC) {
+            int g = threadIdx.x / KC, kr = threadIdx.x % KC;
+            int64_t row = k0 + kr;
+            if (row < k_end) wv = g == 0 ? w1[row] : w2[row];
+        }
     }
 
     __device__ inline void sstore(T* st)
@@ -253,15 +258,7 @@ struct GramTile {
             int lin = threadIdx.x + 256 * j;
             if (lin < KC * VI) {
                 int kr = lin / VI, cv = lin % VI;
-                const T* e = reinterpret_cast<const T*>(&ri[j]);
-#pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    T o[EPV];
-#pragma unroll
-                    for (int q = 0; q < EPV; ++q) o[q] = e[q] * wr[g][j];
-                    *reinterpret_cast<uint4*>(st + g * KC * PI + kr * PI + cv * EPV) =
-                        *reinterpret_cast<const uint4*>(o);
-                }
+                *reinterpret_cast<uint4*>(st + kr * PI + cv * EPV) = ri[j];
             }
         }
 #pragma unroll
@@ -269,9 +266,10 @@ struct GramTile {
             int lin = threadIdx.x + 256 * j;
             if (lin < KC * VJ) {
                 int kr = lin / VJ, cv = lin % VJ;
-                *reinterpret_cast<uint4*>(st + NG * KC * PI + kr * PJ + cv * EPV) = rj[j];
+                *reinterpret_cast<uint4*>(st + KC * PI + kr * PJ + cv * EPV) = rj[j];
             }
         }
+        if ((int)threadIdx.x < NG * KC) st[WOFF + threadIdx.x] = wv;
     }
 
     __device__ inline void compute(const T* cur)
@@ -279,28 +277,30 @@ struct GramTile {
 #pragma unroll
         for (int s = 0; s < KC / M::NH; ++s) {
             int kr = s * M::NH + h;
-            pair_t a[NG][TM / IL], b[TN / IL];
+            pair_t araw[TM / IL], b[TN / IL];
+            T wk[NG];
 #pragma unroll
-            for (int g = 0; g < NG; ++g)
+            for (int g = 0; g < NG; ++g) wk[g] = cur[WOFF + g * KC + kr];
 #pragma unroll
-                for (int x = 0; x < TM / IL; ++x)
-                    a[g][x] = *reinterpret_cast<const pair_t*>(
-                        cur + g * KC * PI + kr * PI + wm * WR + x * M::MT * IL + IL * li);
+            for (int x = 0; x < TM / IL; ++x)
+                araw[x] = *reinterpret_cast<const pair_t*>(cur + kr * PI + wm * WR + x * M::MT * IL + IL * li);
 #pragma unroll
             for (int y = 0; y < TN / IL; ++y)
                 b[y] = *reinterpret_cast<const pair_t*>(
-                    cur + NG * KC * PI + kr * PJ + wn * WC + y * M::MT * IL + IL * li);
+                    cur + KC * PI + kr * PJ + wn * WC + y * M::MT * IL + IL * li);
 #pragma unroll
             for (int g = 0; g < NG; ++g)
 #pragma unroll
-                for (int x = 0; x < TM / IL; ++x)
+                for (int x = 0; x < TM / IL; ++x) {
+                    T a0 = araw[x].x * wk[g], a1 = araw[x].y * wk[g];
 #pragma unroll
                     for (int y = 0; y < TN / IL; ++y) {
-                        M::mma(acc[g][IL * x][IL * y], a[g][x].x, b[y].x);
-                        M::mma(acc[g][IL * x][IL * y + 1], a[g][x].x, b[y].y);
-                        M::mma(acc[g][IL * x + 1][IL * y], a[g][x].y, b[y].x);
-                        M::mma(acc[g][IL * x + 1][IL * y + 1], a[g][x].y, b[y].y);
+                        M::mma(acc[g][IL * x][IL * y], a0, b[y].x);
+                        M::mma(acc[g][IL * x][IL * y + 1], a0, b[y].y);
+                        M::mma(acc[g][IL * x + 1][IL * y], a1, b[y].x);
+                        M::mma(acc[g][IL * x + 1][IL * y + 1], a1, b[y].y);
                     }
+                }
         }
     }
 
@@ -664,6 +664,142 @@ kevd_status eigh(kevd_ctx* ctx, double* W, int64_t r, double* lam)
     return KEVD_OK;
 }
 
+// ------------------------------------------------ leading eigenpairs
+
+__global__ void k_randn(double* Q, int64_t count, uint64_t seed)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    uint32_t o[4];
+    kevd_philox((uint32_t)i, 0u, 7u, 0u, (uint32_t)seed, (uint32_t)(seed >> 32), o);
+    double u1 = kevd_u64_to_unit(((uint64_t)o[1] << 32) | o[0]);
+    double u2 = (double)((((uint64_t)o[3] << 32) | o[2]) >> 11) * 0x1.0p-53;
+    Q[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+__global__ void k_symmetrize(double* H, int l)
+{
+    int i = blockIdx.x * 16 + (threadIdx.x >> 4), j = blockIdx.y * 16 + (threadIdx.x & 15);
+    if (i < l && j < l && i < j) {
+        double a = 0.5 * (H[i + (int64_t)j * l] + H[j + (int64_t)i * l]);
+        H[i + (int64_t)j * l] = a;
+        H[j + (int64_t)i * l] = a;
+    }
+}
+
+// res2[c] = s_i^T (G s_i) - theta_i^2 for the leading k Ritz pairs, where
+// G = Y^T Y and Y = W Q: the squared residual ||W x_i - theta_i x_i||^2 of
+// the Ritz vector x_i = Q s_i (Q orthonormal, ||s_i|| = 1).
+__global__ void k_ritz_res2(const double* S, const double* GS, const double* th, int l, int k,
+                            double* res2)
+{
+    int c = blockIdx.x;
+    int i = l - k + c;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < l; j += 256) acc += S[j + (int64_t)i * l] * GS[j + (int64_t)i * l];
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) res2[c] = red[0] - th[i] * th[i];
+}
+
+__global__ void k_lam_tail(const double* th, int l, int64_t r, double* lam)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= r) return;
+    lam[i] = i >= r - l ? th[i - (r - l)] : NAN;
+}
+
+// Leading k eigenpairs of the symmetric r x r FP64 matrix W (column-major).
+// lam (device, r entries) gets eigenvalues ascending, *Vk an r x k block of
+// eigenvectors in the same order (the last column is the leading one).
+// all = true: the full divide-and-conquer eigensolve (every eigenvalue,
+// as the reference's syevd).  Otherwise block subspace iteration with
+// Rayleigh-Ritz: Y = W Q, H = Q^T Y = S diag(theta) S^T, and, until the
+// leading k residuals ||W x - theta x|| <= 1e-6 theta + 1e-14 theta_max,
+// Q = qr(Y).  The error of a Ritz value is of the order of the squared
+// relative residual, so the stopping rule gives ~1e-12 relative
+// eigenvalues; each step reduces the unwanted components by
+// lambda_{l+1} / lambda_i with block size l >= k + 64.
+kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, double* lam,
+                        double** Vk, int* iters)
+{
+    int l = (int)KEVD_ROUND_UP(std::max(k + 64, 256), 32);
+    *iters = 0;
+    if (all || r < 2 * (int64_t)l) {
+        KEVD_TRY(eigh(ctx, W, r, lam));
+        *Vk = W + (r - k) * r;
+        return KEVD_OK;
+    }
+    rocblas_handle h = blas_handle(ctx);
+    double *Q, *Y, *H, *G, *GS, *th, *tau, *res2, *E, *X;
+    int* info;
+    KEVD_TRY(ctx_buf_t(ctx, "q_Q", (size_t)r * l, &Q));
+    KEVD_TRY(ctx_buf_t(ctx, "q_Y", (size_t)r * l, &Y));
+    KEVD_TRY(ctx_buf_t(ctx, "q_H", (size_t)l * l, &H));
+    KEVD_TRY(ctx_buf_t(ctx, "q_G", (size_t)l * l, &G));
+    KEVD_TRY(ctx_buf_t(ctx, "q_GS", (size_t)l * l, &GS));
+    KEVD_TRY(ctx_buf_t(ctx, "q_th", l, &th));
+    KEVD_TRY(ctx_buf_t(ctx, "q_tau", l, &tau));
+    KEVD_TRY(ctx_buf_t(ctx, "q_res2", k, &res2));
+    KEVD_TRY(ctx_buf_t(ctx, "q_E", l, &E));
+    KEVD_TRY(ctx_buf_t(ctx, "q_X", (size_t)r * k, &X));
+    KEVD_TRY(ctx_buf_t(ctx, "q_info", 1, &info));
+    ProfScope ps(ctx, TC_EIG);
+    const rocblas_int R = (rocblas_int)r, Lb = (rocblas_int)l;
+    double one = 1.0, zero = 0.0;
+    k_randn<<<(unsigned)((r * l + 255) / 256), 256, 0, ctx->stream>>>(Q, r * l, 0x6b657664ull);
+    KEVD_HIP(ctx, hipGetLastError());
+    BLAS(rocsolver_dgeqrf(h, R, Lb, Q, R, tau));
+    BLAS(rocsolver_dorgqr(h, R, Lb, Lb, Q, R, tau));
+    std::vector<double> hres((size_t)k), hth((size_t)l);
+    const int maxit = 12;
+    int it = 0;
+    for (it = 1; it <= maxit; ++it) {
+        BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, Lb, R, &one, W, R, Q, R,
+                           &zero, Y, R));
+        BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Q, R,
+                           Y, R, &zero, H, Lb));
+        BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Y, R,
+                           Y, R, &zero, G, Lb));
+        k_symmetrize<<<dim3((l + 15) / 16, (l + 15) / 16), 256, 0, ctx->stream>>>(H, l);
+        BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, Lb, H, Lb, th, E, info));
+        BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, Lb, Lb, Lb, &one, G, Lb, H, Lb,
+                           &zero, GS, Lb));
+        k_ritz_res2<<<k, 256, 0, ctx->stream>>>(H, GS, th, l, k, res2);
+        KEVD_HIP(ctx, hipGetLastError());
+        int hinfo = 0;
+        KEVD_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        KEVD_HIP(ctx, hipMemcpyAsync(hres.data(), res2, k * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KEVD_HIP(ctx, hipMemcpyAsync(hth.data(), th, l * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (hinfo != 0) return kevd_fail(ctx, KEVD_EEIG, "Rayleigh-Ritz eigensolve failed, info " + std::to_string(hinfo));
+        double thmax = std::fabs(hth[l - 1]);
+        bool conv = true;
+        for (int c = 0; c < k && conv; ++c) {
+            double t = hth[l - k + c];
+            double res = std::sqrt(std::max(hres[c], 0.0));
+            if (res > 1e-6 * std::fabs(t) + 1e-14 * thmax) conv = false;
+        }
+        if (conv || it == maxit) break;
+        BLAS(rocsolver_dgeqrf(h, R, Lb, Y, R, tau));
+        BLAS(rocsolver_dorgqr(h, R, Lb, Lb, Y, R, tau));
+        std::swap(Q, Y);
+    }
+    *iters = it;
+    // Ritz vectors of the leading k: X = Q S(:, l-k:l)
+    BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, (rocblas_int)k, Lb, &one, Q, R,
+                       H + (int64_t)(l - k) * l, Lb, &zero, X, R));
+    k_lam_tail<<<(unsigned)((r + 255) / 256), 256, 0, ctx->stream>>>(th, l, r, lam);
+    KEVD_HIP(ctx, hipGetLastError());
+    *Vk = X;
+    return KEVD_OK;
+}
+
 template <typename T>
 kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res)
 {
@@ -734,9 +870,10 @@ kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* re
                            rocblas_diagonal_non_unit, (rocblas_int)r, (rocblas_int)r, &one, A,
                            (rocblas_int)r, W, (rocblas_int)r, B, (rocblas_int)r));  // B = L^T W
     }
-    KEVD_TRY(eigh(ctx, B, r, lam));  // V in B
-    // Y = L^-T V_k  (the trailing k columns of V, ascending)
-    double* Vk = B + (r - k) * r;
+    double* Vk = nullptr;
+    int eit = 0;
+    KEVD_TRY(eig_leading(ctx, B, r, k, p->all_eigenvalues != 0, lam, &Vk, &eit));
+    // Y = L^-T V_k  (the trailing k eigenvectors, ascending)
     {
         ProfScope ps(ctx, TC_DENSE);
         BLAS(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
@@ -814,9 +951,11 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     k_weights<<<gn, 256, 0, ctx->stream>>>(s, n, 3, w1);  // s^2
     KEVD_HIP(ctx, hipGetLastError());
     KEVD_TRY((gram<T, 1>(ctx, w1, nullptr, G, nullptr)));
-    KEVD_TRY(eigh(ctx, G, r, lam));
+    double* Vk = nullptr;
+    int eit = 0;
+    KEVD_TRY(eig_leading(ctx, G, r, k, p->all_eigenvalues != 0, lam, &Vk, &eit));
     k_inv_sqrt_tail<<<(k + 255) / 256, 256, 0, ctx->stream>>>(lam, r, k, isq);
-    KEVD_TRY(ugemm<T>(ctx, G + (r - k) * r, k, isq, nullptr, s, ctx->U, nullptr));
+    KEVD_TRY(ugemm<T>(ctx, Vk, k, isq, nullptr, s, ctx->U, nullptr));
     KEVD_HIP(ctx, hipMemcpyAsync(res->lam, lam, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> hs((size_t)n);
     KEVD_HIP(ctx, hipMemcpy(hs.data(), s, n * sizeof(double), hipMemcpyDeviceToHost));

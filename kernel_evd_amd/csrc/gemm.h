@@ -169,20 +169,30 @@ struct NTGemm {
 
     // acc += Ts[0:BM, 0:K] Bg[0:BN, 0:K]^T with Ts in LDS (row pitch tp) and
     // Bg read straight from global memory (a small operand every block
-    // shares, served from L2).  K multiple of 8.
+    // shares, served from L2).  K multiple of 8.  The B operand of sub-step
+    // s + 1 is loaded while sub-step s multiplies, hiding the L2 latency.
     __device__ inline void lds_global(const T* Ts, int tp, const T* Bg, int64_t ldb, int K)
     {
-        for (int s = 0; s < K / 8; ++s) {
-            vec_t a[TM], b[TN];
+        int ns = K / 8;
+        if (ns <= 0) return;
+        vec_t b[TN], bn[TN];
+#pragma unroll
+        for (int t = 0; t < TN; ++t)
+            if (t < nact)
+                b[t] = *reinterpret_cast<const vec_t*>(Bg + (int64_t)(wn * TN * MT + t * MT + li) * ldb + h * M::VEC);
+        for (int s = 0; s < ns; ++s) {
+            if (s + 1 < ns) {
+#pragma unroll
+                for (int t = 0; t < TN; ++t)
+                    if (t < nact)
+                        bn[t] = *reinterpret_cast<const vec_t*>(
+                            Bg + (int64_t)(wn * TN * MT + t * MT + li) * ldb + (s + 1) * 8 + h * M::VEC);
+            }
+            vec_t a[TM];
 #pragma unroll
             for (int t = 0; t < TM; ++t)
                 a[t] = *reinterpret_cast<const vec_t*>(
                     Ts + (wm * TM * MT + t * MT + li) * tp + s * 8 + h * M::VEC);
-#pragma unroll
-            for (int t = 0; t < TN; ++t)
-                if (t < nact)
-                    b[t] = *reinterpret_cast<const vec_t*>(
-                        Bg + (int64_t)(wn * TN * MT + t * MT + li) * ldb + s * 8 + h * M::VEC);
 #pragma unroll
             for (int y = 0; y < TN; ++y)
                 if (y < nact)
@@ -190,6 +200,8 @@ struct NTGemm {
                     for (int j = 0; j < M::VEC; ++j)
 #pragma unroll
                         for (int x = 0; x < TM; ++x) M::mma(acc[x][y], M::get(a[x], j), M::get(b[y], j));
+#pragma unroll
+            for (int t = 0; t < TN; ++t) b[t] = bn[t];
         }
     }
 };

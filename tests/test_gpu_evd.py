@@ -69,7 +69,7 @@ def test_full_rank_equals_dense_reference(gpu_ctx):
     eps = 2.0 * orc.median_sqdist(X)
     out = gpu_ctx.arpc(eps, n, block=16, seed=3, tol=0.0, prec="fp64")
     assert out.rank == n or out.trace_err[-1] < 1e-14
-    res = gpu_ctx.evd(out.rank, k_out=32, prec="fp64")
+    res = gpu_ctx.evd(out.rank, k_out=32, prec="fp64", all_eigenvalues=True)
     K = orc.dense_kernel(X, eps)
     w, V, P = orc.dense_reference_evd(K)
     top = res.lam[::-1][:32]
@@ -100,3 +100,20 @@ def test_nonpositive_degree_status(gpu_ctx):
     with pytest.raises(KevdError) as e:
         gpu_ctx.evd(5, k_out=2, prec="fp64")
     assert e.value.status == 2
+
+
+def test_leading_solver_matches_full_eigensolve(gpu_ctx):
+    X, eps, out, F = _factor(gpu_ctx, n=4000, dim=8, rank=700, seed=6)
+    k = 100
+    full = gpu_ctx.evd(out.rank, k_out=k, prec="fp64", all_eigenvalues=True)
+    lead = gpu_ctx.evd(out.rank, k_out=k, prec="fp64", all_eigenvalues=False)
+    assert np.all(np.isfinite(lead.lam[-k:]))
+    # the full divide-and-conquer solve is accurate to ~1e-16 lambda_max in
+    # absolute terms, so tiny eigenvalues are compared with that floor
+    diff = np.abs(lead.lam[-k:] - full.lam[-k:])
+    print("leading vs full: max abs", diff.max(), "lam_k", full.lam[-k], flush=True)
+    assert np.all(diff <= 1e-9 * np.abs(full.lam[-k:]) + 1e-14)
+    lam, U, _, _ = orc.lowrank_evd(F, k)
+    assert np.all(np.abs(lead.lam[-k:] - lam[-k:]) <= 1e-8 * np.abs(lam[-k:]) + 1e-14)
+    Ua = _align(lead.U.astype(np.float64), U)
+    assert np.max(np.abs(Ua[:, -16:] - U[:, -16:])) < 1e-5
