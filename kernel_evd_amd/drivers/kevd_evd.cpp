@@ -6,7 +6,10 @@
 //
 //   kevd-evd --points X.npy (--factor F.npy | --pivots S.npy (--eps E | --eps-median))
 //            [--kout 100] [--norm ref|sinkhorn] [--sinkhorn-tol 1e-12]
-//            [--precision mixed|fp64] [--lam-out lam.npy] [--U-out U.npy] [--device 0]
+//            [--precision mixed|fp64] [--all-eigenvalues] [--lam-out lam.npy]
+//            [--U-out U.npy] [--device 0]
+// Without --all-eigenvalues only the leading eigenvalues are computed and
+// the rest of lam is NaN.
 // With --pivots, F = K(:,S) L^-T is rebuilt by the ARPC kernels with the
 // pivot list injected (K(S,S) = L L^T), which is what the reference's kEVD
 // regenerates panel by panel from /spiv and /linv.
@@ -31,7 +34,7 @@ int main(int argc, char** argv)
     if (a.has("help")) {
         std::printf("kevd-evd --points X.npy (--factor F.npy | --pivots S.npy (--eps E | --eps-median))\n"
                     "         [--kout 100] [--norm ref|sinkhorn] [--sinkhorn-tol 1e-12]\n"
-                    "         [--precision mixed|fp64] [--lam-out lam.npy] [--U-out U.npy]\n");
+                    "         [--precision mixed|fp64] [--all-eigenvalues] [--lam-out lam.npy] [--U-out U.npy]\n");
         return 0;
     }
     auto t0 = std::chrono::steady_clock::now();
@@ -111,6 +114,7 @@ int main(int argc, char** argv)
     ep.sinkhorn_tol = a.num("sinkhorn-tol", 1e-12);
     ep.sinkhorn_max_iter = (int32_t)a.integer("sinkhorn-max-iter", 1000);
     ep.prec = prec;
+    ep.all_eigenvalues = a.has("all-eigenvalues") ? 1 : 0;
     std::vector<double> lam((size_t)r);
     std::vector<float> U;
     kevd_evd_result er{};
