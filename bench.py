@@ -48,6 +48,24 @@ def ks_data(n, M=64, L=22.0, seed=0):
     return X
 
 
+def copy_calibration(mib):
+    """One elementwise copy (y = -x) of `mib` MiB of FP32 after a 1 GiB cache
+    flush, so that the FETCH_SIZE / WRITE_SIZE counters of a profiled run can
+    be scaled by what they read for a copy of known size (mib MiB read,
+    mib MiB written)."""
+    import torch
+    n = (mib << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    flush = torch.zeros((1 << 30) // 4, dtype=torch.float32, device="cuda")
+    flush.add_(1.0)
+    torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    del x, y, flush
+    torch.cuda.empty_cache()
+    log("copy calibration: %d MiB read, %d MiB written" % (mib, mib))
+
+
 def one_step(ctx, eps, rank, block, kout, prec, seed=0):
     t0 = time.perf_counter()
     a = ctx.arpc(eps, rank, block=block, seed=seed, prec=prec)
@@ -85,6 +103,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run one cold FP32 copy of MIB MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -93,6 +113,8 @@ def main():
 
     from kernel_evd_amd import Context
 
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
     X = ks_data(args.n)
     X32 = X.astype(np.float32)
     ctx = Context(0)

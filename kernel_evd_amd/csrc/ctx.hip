@@ -125,6 +125,7 @@ kevd_status ctx_check_arch(kevd_ctx* ctx)
         return kevd_fail(ctx, KEVD_EINVAL,
                          std::string("libkevd is built for gfx950 only; device is ") +
                              prop.gcnArchName);
+    ctx->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     return KEVD_OK;
 }
 

@@ -357,11 +357,17 @@ template <typename T, int NG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T* __restrict__ w1,
             const T* __restrict__ w2, const int* __restrict__ tileI, const int* __restrict__ tileJ,
-            int ntiles, int64_t rows_per_split, double* part)
+            int ntiles, int nitems, int64_t rows_per_split, double* part)
 {
     typedef GramTile<T, NG> G;
     __shared__ __attribute__((aligned(16))) T lds[2 * G::STAGE];
-    int tile = blockIdx.x, split = blockIdx.y;
+    // Workgroups go to the 8 XCDs round-robin by id.  XCD x takes a
+    // contiguous range of the split-major items (item = split * ntiles +
+    // tile), so the workgroups resident on one XCD read the same rows of F
+    // and its L2 serves the column panels they share.
+    int id = blockIdx.x, xcd = id & 7, per = nitems >> 3, rem = nitems & 7;
+    int item = xcd * per + (xcd < rem ? xcd : rem) + (id >> 3);
+    int tile = item % ntiles, split = item / ntiles;
     int64_t ci = (int64_t)tileI[tile] * G::BI, cj = (int64_t)tileJ[tile] * G::BJ;
     int64_t k_beg = (int64_t)split * rows_per_split;
     int64_t k_end = k_beg + rows_per_split < n ? k_beg + rows_per_split : n;
@@ -583,11 +589,25 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
                 TJ.push_back(J);
             }
     int ntiles = (int)TI.size();
-    // enough blocks to fill 256 CUs a few times over
-    int nsplit = (int)((2048 + ntiles - 1) / ntiles);
-    int64_t rps = KEVD_ROUND_UP((n + nsplit - 1) / nsplit, C::KC);
-    if (rps < 2048) rps = 2048;
-    nsplit = (int)((n + rps - 1) / rps);
+    // At least ~2048 workgroups, with the split count chosen so that the
+    // grid fills whole waves of the device's resident slots (2 per CU).
+    const int64_t slots = 2 * (int64_t)ctx->num_cu;
+    int ns0 = (int)std::max<int64_t>(1, (2048 + ntiles - 1) / ntiles);
+    int nsplit = 1;
+    int64_t rps = n;
+    double best = -1.0;
+    for (int ns = ns0; ns <= 2 * ns0; ++ns) {
+        int64_t rp = KEVD_ROUND_UP((n + ns - 1) / ns, C::KC);
+        if (rp < 2048) rp = 2048;
+        int64_t nse = (n + rp - 1) / rp, w = nse * ntiles;
+        double eff = (double)w / (double)(slots * ((w + slots - 1) / slots));
+        if (eff > best + 1e-9) {
+            best = eff;
+            nsplit = (int)nse;
+            rps = rp;
+        }
+    }
+    int nitems = ntiles * nsplit;
     int *dTI, *dTJ;
     double* part;
     T *wt1 = nullptr, *wt2 = nullptr;
@@ -604,8 +624,8 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
     KEVD_HIP(ctx, hipMemcpyAsync(dTJ, TJ.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     {
         ProfScope ps(ctx, TC_GRAM);
-        k_gram<T, NG><<<dim3(ntiles, nsplit), 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, wt1,
-                                                                    wt2, dTI, dTJ, ntiles, rps, part);
+        k_gram<T, NG><<<nitems, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, wt1, wt2, dTI, dTJ,
+                                                      ntiles, nitems, rps, part);
     }
     k_gram_reduce<C::BI, C::BJ, NG><<<ntiles, 256, 0, ctx->stream>>>(part, nsplit, ntiles, dTI, dTJ, r,
                                                                     G1, G2);
@@ -714,17 +734,68 @@ __global__ void k_lam_tail(const double* th, int l, int64_t r, double* lam)
     lam[i] = i >= r - l ? th[i - (r - l)] : NAN;
 }
 
+// G += c tr(G) I for the l x l column-major Gram matrix of shifted
+// Cholesky QR (one workgroup).
+__global__ __launch_bounds__(256) void k_chol_shift(double* G, int l, double c)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < l; i += 256) acc += G[i + (int64_t)i * l];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    double s = c * red[0];
+    for (int i = threadIdx.x; i < l; i += 256) G[i + (int64_t)i * l] += s;
+}
+
+// Orthonormal basis of the columns of the r x l block Y, in place.
+// Householder: rocSOLVER geqrf + orgqr.  Otherwise shifted Cholesky QR 3
+// (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput.
+// 42 (2020) A477): R = chol(Y^T Y + s I), Y <- Y R^-1 with
+// s = 11 (r l + l (l + 1)) u ||Y||_F^2, then two plain Cholesky QR passes.
+// It is stable for cond(Y) up to O(1/u) and costs three small GEMM +
+// potrf + trsm sweeps instead of l sequential Householder steps.  The
+// potrf statuses go to info[0..2]; a nonzero one means the basis is not
+// to be trusted and the caller falls back to Householder.
+kevd_status orth_basis(kevd_ctx* ctx, double* Y, int64_t r, int l, bool householder, double* G,
+                       double* tau, int* info)
+{
+    rocblas_handle h = blas_handle(ctx);
+    const rocblas_int R = (rocblas_int)r, Lb = (rocblas_int)l;
+    if (householder) {
+        BLAS(rocsolver_dgeqrf(h, R, Lb, Y, R, tau));
+        BLAS(rocsolver_dorgqr(h, R, Lb, Lb, Y, R, tau));
+        return KEVD_OK;
+    }
+    double one = 1.0, zero = 0.0;
+    const double c = 11.0 * ((double)r * l + (double)l * (l + 1)) * 0x1.0p-53;
+    for (int pass = 0; pass < 3; ++pass) {
+        BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Y, R, Y,
+                           R, &zero, G, Lb));
+        if (pass == 0) k_chol_shift<<<1, 256, 0, ctx->stream>>>(G, l, c);
+        BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, Lb, G, Lb, info + pass));
+        BLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                           rocblas_diagonal_non_unit, R, Lb, &one, G, Lb, Y, R));
+    }
+    KEVD_HIP(ctx, hipGetLastError());
+    return KEVD_OK;
+}
+
 // Leading k eigenpairs of the symmetric r x r FP64 matrix W (column-major).
 // lam (device, r entries) gets eigenvalues ascending, *Vk an r x k block of
 // eigenvectors in the same order (the last column is the leading one).
 // all = true: the full divide-and-conquer eigensolve (every eigenvalue,
-// as the reference's syevd).  Otherwise block subspace iteration with
-// Rayleigh-Ritz: Y = W Q, H = Q^T Y = S diag(theta) S^T, and, until the
-// leading k residuals ||W x - theta x|| <= 1e-6 theta + 1e-14 theta_max,
-// Q = qr(Y).  The error of a Ritz value is of the order of the squared
-// relative residual, so the stopping rule gives ~1e-12 relative
-// eigenvalues; each step reduces the unwanted components by
-// lambda_{l+1} / lambda_i with block size l >= k + 64.
+// as the reference's syevd).  Otherwise block subspace iteration on
+// l >= k + 64 columns: Q <- orth(W Q), each step reducing the unwanted
+// components of eigenvector i by lambda_{l+1} / lambda_i.  Rayleigh-Ritz
+// (H = Q^T W Q = S diag(theta) S^T) runs after the first few steps and
+// then after every step, until the leading k residuals
+// ||W x - theta x|| <= 1e-6 theta + 1e-14 theta_max.  The error of a Ritz
+// value is of the order of the squared relative residual, so the stopping
+// rule gives ~1e-12 relative eigenvalues.
 kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, double* lam,
                         double** Vk, int* iters)
 {
@@ -748,56 +819,75 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
     KEVD_TRY(ctx_buf_t(ctx, "q_res2", k, &res2));
     KEVD_TRY(ctx_buf_t(ctx, "q_E", l, &E));
     KEVD_TRY(ctx_buf_t(ctx, "q_X", (size_t)r * k, &X));
-    KEVD_TRY(ctx_buf_t(ctx, "q_info", 1, &info));
+    // info[0]: Rayleigh-Ritz syevd; info[1 + 3 j + pass]: Cholesky QR
+    // passes of the j-th orthonormalisation since the last check
+    const int max_orth = 8;
+    KEVD_TRY(ctx_buf_t(ctx, "q_info", 1 + 3 * max_orth, &info));
     ProfScope ps(ctx, TC_EIG);
     const rocblas_int R = (rocblas_int)r, Lb = (rocblas_int)l;
     double one = 1.0, zero = 0.0;
-    k_randn<<<(unsigned)((r * l + 255) / 256), 256, 0, ctx->stream>>>(Q, r * l, 0x6b657664ull);
-    KEVD_HIP(ctx, hipGetLastError());
-    BLAS(rocsolver_dgeqrf(h, R, Lb, Q, R, tau));
-    BLAS(rocsolver_dorgqr(h, R, Lb, Lb, Q, R, tau));
     std::vector<double> hres((size_t)k), hth((size_t)l);
-    const int maxit = 12;
-    int it = 0;
-    for (it = 1; it <= maxit; ++it) {
-        BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, Lb, R, &one, W, R, Q, R,
-                           &zero, Y, R));
-        BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Q, R,
-                           Y, R, &zero, H, Lb));
-        BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Y, R,
-                           Y, R, &zero, G, Lb));
-        k_symmetrize<<<dim3((l + 15) / 16, (l + 15) / 16), 256, 0, ctx->stream>>>(H, l);
-        BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, Lb, H, Lb, th, E, info));
-        BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, Lb, Lb, Lb, &one, G, Lb, H, Lb,
-                           &zero, GS, Lb));
-        k_ritz_res2<<<k, 256, 0, ctx->stream>>>(H, GS, th, l, k, res2);
+    std::vector<int> hinfo(1 + 3 * max_orth);
+    const int maxit = 12, first_check = 4;
+    for (int householder = 0; householder < 2; ++householder) {
+        KEVD_HIP(ctx, hipMemsetAsync(info, 0, (1 + 3 * max_orth) * sizeof(int), ctx->stream));
+        k_randn<<<(unsigned)((r * l + 255) / 256), 256, 0, ctx->stream>>>(Q, r * l, 0x6b657664ull);
         KEVD_HIP(ctx, hipGetLastError());
-        int hinfo = 0;
-        KEVD_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        KEVD_HIP(ctx, hipMemcpyAsync(hres.data(), res2, k * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        KEVD_HIP(ctx, hipMemcpyAsync(hth.data(), th, l * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (hinfo != 0) return kevd_fail(ctx, KEVD_EEIG, "Rayleigh-Ritz eigensolve failed, info " + std::to_string(hinfo));
-        double thmax = std::fabs(hth[l - 1]);
-        bool conv = true;
-        for (int c = 0; c < k && conv; ++c) {
-            double t = hth[l - k + c];
-            double res = std::sqrt(std::max(hres[c], 0.0));
-            if (res > 1e-6 * std::fabs(t) + 1e-14 * thmax) conv = false;
+        int north = 0;
+        KEVD_TRY(orth_basis(ctx, Q, r, l, householder, G, tau, info + 1 + 3 * north++));
+        bool conv = false, broken = false;
+        int it = 0;
+        for (it = 1; it <= maxit; ++it) {
+            BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, Lb, R, &one, W, R, Q, R,
+                               &zero, Y, R));
+            if (it < first_check) {
+                KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++));
+                std::swap(Q, Y);
+                continue;
+            }
+            BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Q, R,
+                               Y, R, &zero, H, Lb));
+            BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Y, R,
+                               Y, R, &zero, G, Lb));
+            k_symmetrize<<<dim3((l + 15) / 16, (l + 15) / 16), 256, 0, ctx->stream>>>(H, l);
+            BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, Lb, H, Lb, th, E, info));
+            BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, Lb, Lb, Lb, &one, G, Lb, H,
+                               Lb, &zero, GS, Lb));
+            k_ritz_res2<<<k, 256, 0, ctx->stream>>>(H, GS, th, l, k, res2);
+            KEVD_HIP(ctx, hipGetLastError());
+            KEVD_HIP(ctx, hipMemcpyAsync(hinfo.data(), info, hinfo.size() * sizeof(int), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+            KEVD_HIP(ctx, hipMemcpyAsync(hres.data(), res2, k * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KEVD_HIP(ctx, hipMemcpyAsync(hth.data(), th, l * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            for (int j = 1; j < 1 + 3 * north; ++j) broken |= hinfo[j] != 0;
+            if (broken) break;
+            if (hinfo[0] != 0)
+                return kevd_fail(ctx, KEVD_EEIG, "Rayleigh-Ritz eigensolve failed, info " + std::to_string(hinfo[0]));
+            double thmax = std::fabs(hth[l - 1]);
+            conv = true;
+            for (int c = 0; c < k && conv; ++c) {
+                double t = hth[l - k + c];
+                double res = std::sqrt(std::max(hres[c], 0.0));
+                if (res > 1e-6 * std::fabs(t) + 1e-14 * thmax) conv = false;
+            }
+            if (conv || it == maxit) break;
+            KEVD_HIP(ctx, hipMemsetAsync(info, 0, (1 + 3 * max_orth) * sizeof(int), ctx->stream));
+            north = 0;
+            KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++));
+            std::swap(Q, Y);
         }
-        if (conv || it == maxit) break;
-        BLAS(rocsolver_dgeqrf(h, R, Lb, Y, R, tau));
-        BLAS(rocsolver_dorgqr(h, R, Lb, Lb, Y, R, tau));
-        std::swap(Q, Y);
+        *iters = it;
+        if (broken) continue;  // a Cholesky QR pass lost definiteness: redo with Householder
+        // Ritz vectors of the leading k: X = Q S(:, l-k:l)
+        BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, (rocblas_int)k, Lb, &one, Q, R,
+                           H + (int64_t)(l - k) * l, Lb, &zero, X, R));
+        k_lam_tail<<<(unsigned)((r + 255) / 256), 256, 0, ctx->stream>>>(th, l, r, lam);
+        KEVD_HIP(ctx, hipGetLastError());
+        *Vk = X;
+        return KEVD_OK;
     }
-    *iters = it;
-    // Ritz vectors of the leading k: X = Q S(:, l-k:l)
-    BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, (rocblas_int)k, Lb, &one, Q, R,
-                       H + (int64_t)(l - k) * l, Lb, &zero, X, R));
-    k_lam_tail<<<(unsigned)((r + 255) / 256), 256, 0, ctx->stream>>>(th, l, r, lam);
-    KEVD_HIP(ctx, hipGetLastError());
-    *Vk = X;
-    return KEVD_OK;
+    return kevd_fail(ctx, KEVD_EEIG, "subspace iteration: orthonormalisation failed");
 }
 
 template <typename T>

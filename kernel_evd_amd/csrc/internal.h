@@ -26,6 +26,7 @@ enum TimeCat {
 
 struct kevd_ctx {
     int device = 0;
+    int num_cu = 256;  // compute units of the device
     hipStream_t stream = nullptr;
     std::string err;
 
