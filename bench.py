@@ -66,6 +66,18 @@ def copy_calibration(mib):
     log("copy calibration: %d MiB read, %d MiB written" % (mib, mib))
 
 
+def downdate_bytes(accepted, n, dp, elt=4):
+    """Algorithmic HBM bytes of all downdate launches of one ARPC run: per
+    round with Nr columns already in F and k accepted, each of the n rows
+    reads its point (dp values), its squared norm, F(row, :Nr), reads and
+    writes d (FP64), and writes the k new factor entries."""
+    nr, total = 0, 0.0
+    for k in accepted:
+        total += n * (elt * dp + elt + elt * nr + elt * int(k) + 16)
+        nr += int(k)
+    return total
+
+
 def one_step(ctx, eps, rank, block, kout, prec, seed=0):
     t0 = time.perf_counter()
     a = ctx.arpc(eps, rank, block=block, seed=seed, prec=prec)
@@ -169,6 +181,7 @@ def main():
                       "rejection": round(agg["rejection_ms"] / steps, 3),
                       "dense": round(agg["dense_ms"] / steps, 3), "u_gemm": round(agg["u_ms"] / steps, 3)},
         "rank": int(a_mix.rank), "rounds": int(a_mix.rounds),
+        "accepted_per_round": [int(v) for v in a_mix.accepted],
         "trace_err_mixed": float(a_mix.trace_err[-1]),
         "roofline": {"bound": "mfma", "kernel": "k_downdate<float>",
                      "achieved": round(dd_flop / (dd_ms * 1e-3) / 1e12, 3) if dd_ms > 0 else None,
@@ -176,6 +189,7 @@ def main():
                      "frac": round(dd_flop / (dd_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4) if dd_ms > 0 else None,
                      "traffic": None,
                      "flop_per_step": dd_flop, "launches_per_step": dd_launch,
+                     "algorithmic_bytes_per_launch": downdate_bytes(a_mix.accepted, args.n, 64) / max(dd_launch, 1),
                      "avg_launch_ms": round(dd_ms / max(dd_launch, 1), 4)},
         "gram_roofline": {"achieved": round(gr_flop / (gr_ms * 1e-3) / 1e12, 3) if gr_ms > 0 else None,
                           "peak": FP32_MFMA_PEAK_TF, "unit": "TFLOP/s",

@@ -62,47 +62,97 @@ __global__ __launch_bounds__(256) void k_colsum(const T* __restrict__ F, int64_t
     }
 }
 
-// c[j] = sum_b part[b][j] in block order (deterministic).
-__global__ __launch_bounds__(256) void k_colsum_reduce(const double* part, int nb, int64_t r,
-                                                       int64_t rpad, double* c)
+// c[j] = sum_b part[b][j], deterministic: a block owns 64 columns; its 16
+// row groups sum the blocks b = g (mod 16) in order, then group 0 adds the
+// 16 group sums in order.
+__global__ __launch_bounds__(1024) void k_colsum_reduce(const double* part, int nb, int64_t r,
+                                                        int64_t rpad, double* c)
 {
-    int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j >= r) return;
+    __shared__ double red[16][64];
+    int col = threadIdx.x & 63, g = threadIdx.x >> 6;
+    int64_t j = (int64_t)blockIdx.x * 64 + col;
     double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += part[(int64_t)b * rpad + j];
-    c[j] = s;
+    if (j < r) {
+        int b = g;
+        for (; b + 48 < nb; b += 64) {
+            double v0 = part[(int64_t)b * rpad + j], v1 = part[(int64_t)(b + 16) * rpad + j];
+            double v2 = part[(int64_t)(b + 32) * rpad + j], v3 = part[(int64_t)(b + 48) * rpad + j];
+            s += v0;
+            s += v1;
+            s += v2;
+            s += v3;
+        }
+        for (; b < nb; b += 16) s += part[(int64_t)b * rpad + j];
+    }
+    red[g][col] = s;
+    __syncthreads();
+    if (g == 0 && j < r) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) t += red[q][col];
+        c[j] = t;
+    }
 }
 
-// y_i = F_i . c (FP64), one wave per row.  mode 0: plain; mode 1 (Sinkhorn
-// step): with s the current scaling, also dev_i = |s_i y_i - 1| and
-// s_new_i = sqrt(s_i / y_i).  Per-block min(y) and max(dev) go to bmin/bdev.
-template <typename T>
+constexpr int ROWDOT_LDS_MAX = 8192;  // c staged in LDS up to this many columns
+
+// y_i = F_i . c (FP64).  A block walks groups of 16 rows (grid-stride);
+// each wave takes 4 rows of a group at once, so a lane has 4 independent
+// 16-byte loads in flight per column step, with c read from LDS (STAGED)
+// or from global memory.  mode 0: plain; mode 1 (Sinkhorn step): with s
+// the current scaling, also dev_i = |s_i y_i - 1| and s_new_i =
+// sqrt(s_i / y_i).  Per-block min(y) and max(dev) go to bmin/bdev.
+template <typename T, bool STAGED>
 __global__ __launch_bounds__(256) void k_rowdot(const T* __restrict__ F, int64_t ldf, int64_t n,
                                                 int64_t r, const double* __restrict__ c,
                                                 double* y, double* bmin, int mode,
                                                 const double* s, double* s_new, double* bdev)
 {
-    constexpr int V = 16 / (int)sizeof(T);
+    constexpr int V = 16 / (int)sizeof(T), RW = 4;
     typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type vec_t;
+    extern __shared__ double csh[];
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (STAGED) {
+        for (int64_t j = threadIdx.x; j < r; j += 256) csh[j] = c[j];
+        __syncthreads();
+    }
     double mn = INFINITY, dv = 0.0;
-    for (int64_t row = (int64_t)blockIdx.x * 16 + wv; row < n && row < (int64_t)blockIdx.x * 16 + 16; row += 4) {
-        double acc = 0.0;
-        for (int64_t cg = (int64_t)lane * V; cg < r; cg += 64 * V) {
-            vec_t v = *reinterpret_cast<const vec_t*>(F + row * ldf + cg);
-            const T* e = reinterpret_cast<const T*>(&v);
+    int64_t ngroups = (n + 15) / 16;
+    for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        int64_t row0 = grp * 16 + wv * RW;
+        double acc[RW];
 #pragma unroll
-            for (int q = 0; q < V; ++q)
-                if (cg + q < r) acc += (double)e[q] * c[cg + q];
+        for (int i = 0; i < RW; ++i) acc[i] = 0.0;
+        for (int64_t cg = (int64_t)lane * V; cg < r; cg += 64 * V) {
+            double cv[V];
+#pragma unroll
+            for (int q = 0; q < V; ++q) cv[q] = cg + q < r ? (STAGED ? csh[cg + q] : c[cg + q]) : 0.0;
+            vec_t v[RW];
+#pragma unroll
+            for (int i = 0; i < RW; ++i)
+                if (row0 + i < n) v[i] = *reinterpret_cast<const vec_t*>(F + (row0 + i) * ldf + cg);
+#pragma unroll
+            for (int i = 0; i < RW; ++i) {
+                if (row0 + i >= n) continue;
+                const T* e = reinterpret_cast<const T*>(&v[i]);
+#pragma unroll
+                for (int q = 0; q < V; ++q)
+                    if (cg + q < r) acc[i] += (double)e[q] * cv[q];
+            }
         }
-        acc = wave_sum(acc);
-        mn = fmin(mn, acc);
-        if (lane == 0) {
-            y[row] = acc;
-            if (mode == 1) {
-                double si = s[row];
-                dv = fmax(dv, fabs(si * acc - 1.0));
-                s_new[row] = acc > 0.0 ? sqrt(si / acc) : si;
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            int64_t row = row0 + i;
+            double a = wave_sum(acc[i]);
+            if (row >= n) continue;
+            mn = fmin(mn, a);
+            if (lane == 0) {
+                y[row] = a;
+                if (mode == 1) {
+                    double si = s[row];
+                    dv = fmax(dv, fabs(si * a - 1.0));
+                    s_new[row] = a > 0.0 ? sqrt(si / a) : si;
+                }
             }
         }
     }
@@ -167,8 +217,11 @@ __global__ void k_weights(const double* x, int64_t n, int mode, double* w)
 // partial tile (a private slice of `part`), which bounds the length of any
 // single FP32 accumulation.
 template <typename T> struct GramCfg;
+#ifndef KEVD_GRAM_PROMOTE
+#define KEVD_GRAM_PROMOTE 1024
+#endif
 template <> struct GramCfg<float> {
-    static constexpr int BI = 128, BJ = 128, KC = 32, TM = 2, TN = 2, PROMOTE = 1024;
+    static constexpr int BI = 128, BJ = 128, KC = 32, TM = 2, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
 };
 template <> struct GramCfg<double> {
     static constexpr int BI = 128, BJ = 64, KC = 16, TM = 4, TN = 2, PROMOTE = 1 << 30;
@@ -550,7 +603,7 @@ kevd_status colsum(kevd_ctx* ctx, const double* w, double* c)
     double* part = nullptr;
     KEVD_TRY(ctx_buf_t(ctx, "e_cpart", (size_t)nb * rpad, &part));
     k_colsum<T><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, w, part, rpad);
-    k_colsum_reduce<<<(unsigned)((r + 255) / 256), 256, 0, ctx->stream>>>(part, nb, r, rpad, c);
+    k_colsum_reduce<<<(unsigned)((r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, r, rpad, c);
     KEVD_HIP(ctx, hipGetLastError());
     return KEVD_OK;
 }
@@ -561,13 +614,17 @@ kevd_status rowdot(kevd_ctx* ctx, const double* c, double* y, int mode, const do
                    double* s_new, double host_mm[2])
 {
     int64_t n = ctx->n, r = ctx->rank;
-    int nb = (int)((n + 15) / 16);
+    int nb = (int)std::min<int64_t>((n + 15) / 16, 8 * (int64_t)ctx->num_cu);
     double *bmin, *bdev, *mm;
     KEVD_TRY(ctx_buf_t(ctx, "e_bmin", nb, &bmin));
     KEVD_TRY(ctx_buf_t(ctx, "e_bdev", nb, &bdev));
     KEVD_TRY(ctx_buf_t(ctx, "e_mm", 2, &mm));
-    k_rowdot<T><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, c, y, bmin, mode, s,
-                                            s_new, bdev);
+    if (r <= ROWDOT_LDS_MAX)
+        k_rowdot<T, true><<<nb, 256, r * sizeof(double), ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, c, y,
+                                                                       bmin, mode, s, s_new, bdev);
+    else
+        k_rowdot<T, false><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, c, y, bmin, mode,
+                                                       s, s_new, bdev);
     k_block_minmax<<<1, 256, 0, ctx->stream>>>(bmin, mode == 1 ? bdev : nullptr, nb, mm);
     KEVD_HIP(ctx, hipGetLastError());
     KEVD_HIP(ctx, hipMemcpyAsync(host_mm, mm, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
