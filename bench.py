@@ -1,13 +1,19 @@
 """Benchmark: rank-2000 factor + top-100 bistochastic EVD at N = 1,048,576.
 
 One step = accelerated RPCholesky (rank 2000, block 128, seed 0, mixed
-FP32/FP64) + bistochastic kEVD (top 100 eigenpairs) on a Kuramoto-Sivashinsky
+FP32/FP64) + bistochastic kEVD (top 100 eigenpairs; symmetric Sinkhorn
+scaling to a maximum row-sum deviation of 1e-12 by default, or the
+reference's two-step normalisation with --norm ref) on a Kuramoto-Sivashinsky
 data set (L = 22, 64 grid points, ETDRK4 dt = 0.25, IC 0.1 N(0,1) seed 0,
 10,000 steps discarded, one snapshot every 4 steps, FP32 storage), eps = the
 median squared pairwise distance (Philox-sampled, 2^22 pairs).  The data are
-resident in HBM before the timed region.  After the timed steps one all-FP64
-step runs for the speed ratio and the eigenvalue error, and the CPU FP64
-oracle runs on a bounded sample for the CPU baseline.
+resident in HBM before the timed region.
+
+After the timed steps (untimed): one all-FP64 step for the speed ratio, the
+eigenvalue errors and the leading-eigenvector subspace angles; the EVD in
+the other normalisation on both factors; the mixed ARPC with the FP64
+pivots injected, for the factor's relative Frobenius difference; and the
+CPU FP64 oracle on a bounded sample for the CPU baseline.
 
 Prints ONE JSON line.  Single GPU only.
 """
@@ -78,26 +84,57 @@ def downdate_bytes(accepted, n, dp, elt=4):
     return total
 
 
-def one_step(ctx, eps, rank, block, kout, prec, seed=0):
+SINKHORN_TOL = 1e-12
+
+
+def run_evd(ctx, rank, kout, prec, norm):
+    return ctx.evd(rank, k_out=kout, prec=prec, norm=norm, sinkhorn_tol=SINKHORN_TOL,
+                   sinkhorn_max_iter=1000, want_U=False)
+
+
+def one_step(ctx, eps, rank, block, kout, prec, norm, seed=0):
     t0 = time.perf_counter()
     a = ctx.arpc(eps, rank, block=block, seed=seed, prec=prec)
     t1 = time.perf_counter()
-    e = ctx.evd(a.rank, k_out=kout, prec=prec, want_U=False)
+    e = run_evd(ctx, a.rank, kout, prec, norm)
     t2 = time.perf_counter()
     return a, e, t1 - t0, t2 - t1
 
 
-def cpu_baseline(X, eps, rank, block, kout, n_sample):
+def max_principal_angle(U1, U2):
+    """Largest principal angle (radians) between the column spans."""
+    Q1, _ = np.linalg.qr(np.asarray(U1, dtype=np.float64))
+    Q2, _ = np.linalg.qr(np.asarray(U2, dtype=np.float64))
+    sv = np.linalg.svd(Q1.T @ Q2, compute_uv=False)
+    return float(np.arccos(np.clip(sv.min(), -1.0, 1.0)))
+
+
+def rel_frobenius(Fa, Fb, chunk=1 << 16):
+    """||Fa - Fb||_F / ||Fb||_F, accumulated over row chunks in FP64."""
+    num = den = 0.0
+    for i in range(0, Fb.shape[0], chunk):
+        b = np.asarray(Fb[i:i + chunk], dtype=np.float64)
+        d = np.asarray(Fa[i:i + chunk], dtype=np.float64) - b
+        num += float(np.sum(d * d))
+        den += float(np.sum(b * b))
+    return float(np.sqrt(num / den))
+
+
+def cpu_baseline(X, eps, rank, block, kout, n_sample, norm):
     """CPU FP64 oracle (OpenMP C ARPC + NumPy F-form kEVD) on the first
-    n_sample points; both stages scale linearly in N, so the time is scaled
-    by N / n_sample to the full workload."""
+    n_sample points; both stages scale linearly in N (the Sinkhorn iteration
+    count aside, which the sample shares with the full problem), so the
+    time is scaled by N / n_sample to the full workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     Xs = np.ascontiguousarray(X[:n_sample], dtype=np.float64)
     t0 = time.perf_counter()
     r = orc.arpc(Xs, eps, rank, block=block, seed=0)
     t1 = time.perf_counter()
-    orc.lowrank_evd(r.F, kout)
+    if norm == "sinkhorn":
+        orc.lowrank_sinkhorn_evd(r.F, kout, tol=SINKHORN_TOL, max_iter=1000)
+    else:
+        orc.lowrank_evd(r.F, kout)
     t2 = time.perf_counter()
     return t1 - t0, t2 - t1, orc.num_threads(), r
 
@@ -112,7 +149,11 @@ def main():
     ap.add_argument("--block", type=int, default=128)
     ap.add_argument("--kout", type=int, default=100)
     ap.add_argument("--median-pairs", type=int, default=1 << 22)
+    ap.add_argument("--norm", choices=["sinkhorn", "ref"], default="sinkhorn",
+                    help="kEVD normalisation of the timed step (the other one is reported beside it)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the subspace angles, the other normalisation and the factor difference")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -134,13 +175,14 @@ def main():
     eps = ctx.median_sqdist(seed=0, nsample=args.median_pairs)
     log("eps (median sq. distance) = %.6g" % eps)
 
+    norm, other = args.norm, ("ref" if args.norm == "sinkhorn" else "sinkhorn")
     for _ in range(args.warmup):
-        one_step(ctx, eps, args.rank, args.block, args.kout, "mixed")
+        one_step(ctx, eps, args.rank, args.block, args.kout, "mixed", norm)
     ctx.set_profiling(True)
     t_arpc, t_evd, agg = [], [], {}
     tstart = time.perf_counter()
     for _ in range(args.steps):
-        a, e, ta, te = one_step(ctx, eps, args.rank, args.block, args.kout, "mixed")
+        a, e, ta, te = one_step(ctx, eps, args.rank, args.block, args.kout, "mixed", norm)
         t_arpc.append(ta)
         t_evd.append(te)
         tm = ctx.timings()
@@ -159,6 +201,8 @@ def main():
     log("mixed: %.1f ms/step (arpc %.1f, evd %.1f); rank %d in %d rounds; downdate %.1f ms over %d launches; gram %.1f ms"
         % (ms_step, 1e3 * np.mean(t_arpc), 1e3 * np.mean(t_evd), a_mix.rank, a_mix.rounds, dd_ms, dd_launch, gr_ms))
 
+    evd_desc = ("symmetric Sinkhorn to max row-sum deviation %g" % SINKHORN_TOL if norm == "sinkhorn"
+                else "two-step degree normalisation (ks_kevd)")
     out = {
         "metric": "rank-r factor+EVD time (N=1M,r=2000) FP32-mixed vs FP64; eig rel err vs FP64",
         "value": round(ms_step / 1e3, 6),
@@ -172,17 +216,22 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32-mfma+fp64",
         "data": "synthetic: Kuramoto-Sivashinsky ETDRK4 trajectory generated in-run (L=22, 64 pts, seed 0)",
-        "config": {"workload": "ARPC rank %d block %d + bistochastic kEVD top %d" % (args.rank, args.block, args.kout),
+        "config": {"workload": "ARPC rank %d block %d + bistochastic kEVD top %d, %s"
+                               % (args.rank, args.block, args.kout, evd_desc),
                    "n": args.n, "dim": 64, "rank": args.rank, "block": args.block, "k_out": args.kout,
-                   "eps": eps, "precision": "mixed"},
+                   "eps": eps, "precision": "mixed", "norm": norm},
         "phases_ms": {"arpc": round(1e3 * float(np.mean(t_arpc)), 3), "evd": round(1e3 * float(np.mean(t_evd)), 3),
                       "downdate": round(dd_ms, 3), "gram": round(gr_ms, 3),
                       "eig": round(agg["eig_ms"] / steps, 3), "sample": round(agg["sample_ms"] / steps, 3),
                       "rejection": round(agg["rejection_ms"] / steps, 3),
+                      "matvec": round(agg["matvec_ms"] / steps, 3),
                       "dense": round(agg["dense_ms"] / steps, 3), "u_gemm": round(agg["u_ms"] / steps, 3)},
+        "eig_iters": int(tm["eig_iters"]), "eig_householder_fallback": bool(tm["eig_householder"]),
         "rank": int(a_mix.rank), "rounds": int(a_mix.rounds),
         "accepted_per_round": [int(v) for v in a_mix.accepted],
         "trace_err_mixed": float(a_mix.trace_err[-1]),
+        "sinkhorn_iters": int(e_mix.sinkhorn_iters) if norm == "sinkhorn" else None,
+        "rowsum_dev": float(e_mix.rowsum_dev),
         "roofline": {"bound": "mfma", "kernel": "k_downdate<float>",
                      "achieved": round(dd_flop / (dd_ms * 1e-3) / 1e12, 3) if dd_ms > 0 else None,
                      "peak": FP32_MFMA_PEAK_TF, "unit": "TFLOP/s",
@@ -191,19 +240,35 @@ def main():
                      "flop_per_step": dd_flop, "launches_per_step": dd_launch,
                      "algorithmic_bytes_per_launch": downdate_bytes(a_mix.accepted, args.n, 64) / max(dd_launch, 1),
                      "avg_launch_ms": round(dd_ms / max(dd_launch, 1), 4)},
-        "gram_roofline": {"achieved": round(gr_flop / (gr_ms * 1e-3) / 1e12, 3) if gr_ms > 0 else None,
+        "gram_roofline": {"kernel": "k_gram<float,%d>" % (1 if norm == "sinkhorn" else 2),
+                          "achieved": round(gr_flop / (gr_ms * 1e-3) / 1e12, 3) if gr_ms > 0 else None,
                           "peak": FP32_MFMA_PEAK_TF, "unit": "TFLOP/s",
                           "frac": round(gr_flop / (gr_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4) if gr_ms > 0 else None},
     }
+    if norm == "sinkhorn":
+        # F^T 1 once, then one fused pass over F per Sinkhorn iteration
+        mv_bytes = float(args.n) * args.rank * 4 * (1 + max(1, e_mix.sinkhorn_iters))
+        mv_ms = agg["matvec_ms"] / steps
+        if mv_ms > 0:
+            out["matvec_roofline"] = {"bound": "hbm", "achieved": round(mv_bytes / (mv_ms * 1e-3) / 1e9, 1),
+                                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": round(mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    extras = not args.no_extras
+    if extras:
+        U_mix = ctx.get_u()
+        t0 = time.perf_counter()
+        e_other = run_evd(ctx, a_mix.rank, args.kout, "mixed", other)
+        out["other_norm"] = {"norm": other, "evd_ms": round(1e3 * (time.perf_counter() - t0), 3)}
 
     if not args.no_fp64:
         ctx64 = Context(0)
         ctx64.set_points(X32)
         for _ in range(max(1, args.warmup)):
-            one_step(ctx64, eps, args.rank, args.block, args.kout, "fp64")
+            one_step(ctx64, eps, args.rank, args.block, args.kout, "fp64", norm)
         ctx64.set_profiling(True)
         t0 = time.perf_counter()
-        a64, e64, ta64, te64 = one_step(ctx64, eps, args.rank, args.block, args.kout, "fp64")
+        a64, e64, ta64, te64 = one_step(ctx64, eps, args.rank, args.block, args.kout, "fp64", norm)
         t64 = time.perf_counter() - t0
         tm64 = ctx64.timings()
         k32 = min(32, a64.rank)
@@ -214,24 +279,47 @@ def main():
         out["fp64"] = {"s": round(t64, 4), "arpc_s": round(ta64, 4), "evd_s": round(te64, 4),
                        "downdate_tflops": round(tm64["downdate_flop"] / (tm64["downdate_ms"] * 1e-3) / 1e12, 3)
                        if tm64["downdate_ms"] > 0 else None,
+                       "gram_tflops": round(tm64["gram_flop"] / (tm64["gram_ms"] * 1e-3) / 1e12, 3)
+                       if tm64["gram_ms"] > 0 else None,
                        "rank": int(a64.rank), "rounds": int(a64.rounds),
-                       "trace_err": float(a64.trace_err[-1])}
+                       "trace_err": float(a64.trace_err[-1]),
+                       "sinkhorn_iters": int(e64.sinkhorn_iters) if norm == "sinkhorn" else None}
         out["speedup_vs_fp64"] = round(t64 / (ms_step / 1e3), 3)
         out["eig_rel_err_top32_max"] = float(rel32.max())
         out["eig_rel_err_top%d_max" % kk] = float(relk.max())
         out["pivots_equal_fp64"] = bool(a64.rank == a_mix.rank and np.array_equal(a64.pivots, a_mix.pivots))
         out["trace_err_rel_diff"] = float(abs(a_mix.trace_err[-1] - a64.trace_err[-1]) / a64.trace_err[-1])
         log("fp64: %.3f s; speedup %.2fx; top-32 eig rel err %.2e" % (t64, out["speedup_vs_fp64"], rel32.max()))
+        if extras:
+            U64 = ctx64.get_u()
+            k32u = min(32, kk)
+            out["eigvec_max_angle_top32_rad"] = max_principal_angle(U_mix[:, -k32u:], U64[:, -k32u:])
+            out["eigvec_max_angle_top%d_rad" % kk] = max_principal_angle(U_mix[:, -kk:], U64[:, -kk:])
+            del U_mix, U64
+            t0 = time.perf_counter()
+            e64o = run_evd(ctx64, a64.rank, args.kout, "fp64", other)
+            out["other_norm"]["evd_ms_fp64"] = round(1e3 * (time.perf_counter() - t0), 3)
+            relo = (np.abs(e_other.lam[::-1][:k32] - e64o.lam[::-1][:k32]) / np.abs(e64o.lam[::-1][:k32]))
+            out["other_norm"]["eig_rel_err_top32_max"] = float(relo.max())
+            # factor difference with the pivot list fixed: mixed ARPC on the FP64 pivots
+            ctx.arpc(eps, a64.rank, block=args.block, seed=0, prec="mixed", tol=1e-10,
+                     inject_pivots=a64.pivots)
+            F32 = ctx.get_factor(a64.rank, "mixed")
+            F64 = ctx64.get_factor(a64.rank, "fp64")
+            out["factor_rel_frobenius_vs_fp64_injected"] = rel_frobenius(F32, F64)
+            del F32, F64
+            log("factor rel Frobenius (injected pivots) %.3e" % out["factor_rel_frobenius_vs_fp64_injected"])
         ctx64.close()
 
     if not args.no_cpu:
         ns = min(args.cpu_sample, args.n)
-        ta, te, nt, r = cpu_baseline(X32, eps, args.rank, args.block, args.kout, ns)
+        ta, te, nt, r = cpu_baseline(X32, eps, args.rank, args.block, args.kout, ns, norm)
         scale = args.n / ns
         out["cpu_baseline"] = {"value": round((ta + te) * scale, 3), "unit": "s", "cores": int(nt),
                                "kind": "port",
-                               "sample": "first %d of %d points, same rank/block/seed/eps; ARPC %.2f s + kEVD %.2f s "
-                                         "measured, scaled x%.0f (both stages linear in N)" % (ns, args.n, ta, te, scale)}
+                               "sample": "first %d of %d points, same rank/block/seed/eps/normalisation; ARPC %.2f s "
+                                         "+ kEVD %.2f s measured, scaled x%.0f (both stages linear in N)"
+                                         % (ns, args.n, ta, te, scale)}
         log("cpu baseline: %.2f s measured on %d points (%d threads)" % (ta + te, ns, nt))
     ctx.close()
     print(json.dumps(out), flush=True)

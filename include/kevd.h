@@ -233,6 +233,9 @@ typedef struct {
     double matvec_ms;         /* streaming matvec passes                   */
     double dense_ms;          /* r x r dense phase other than eigensolve   */
     double u_ms;              /* U GEMM                                    */
+    int32_t eig_iters;        /* subspace iterations of the leading-eigenpair
+                                 solve (0: full divide-and-conquer)         */
+    int32_t eig_householder;  /* 1 if that solve fell back to Householder QR */
 } kevd_timings;
 
 /* Enable per-kernel HIP event timing on the context's stream (adds one

@@ -50,6 +50,7 @@ class Context:
         self._h = h
         self.n = 0
         self.dim = 0
+        self._k_out = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -164,7 +165,15 @@ class Context:
             U = np.zeros((self.n, k_out), dtype=np.float32)
             r.U = _p(U, C.c_float)
         self._check(self._lib.kevd_evd(self._h, C.byref(p), C.byref(r)))
+        self._k_out = k_out
         return EvdOutput(lam, U, r.rowsum_dev, r.shift, r.min_degree, r.sinkhorn_iters)
+
+    def get_u(self):
+        """Eigenvectors of the last evd call (n x k_out FP32, columns ascending)."""
+        k = self._k_out
+        U = np.empty((self.n, k), dtype=np.float32)
+        self._check(self._lib.kevd_get_u(self._h, _p(U, C.c_float)))
+        return U
 
     # --------------------------------------------------------- bandwidth
     def bandwidth_sweep(self, rows, eps, prec="mixed"):

@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void k_gather_pivots(const T* __restrict__ Xsr
 //   acc  = T L_c^-T                         (MFMA, K = k)
 //   F(R, Nr:Nr+k) = acc;  d(R) = max(d(R) - rowsq(acc), 0)
 template <typename T>
-__global__ __launch_bounds__(256) void k_downdate(const T* __restrict__ X, int32_t dp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_downdate(const T* __restrict__ X, int32_t dp,
                                                   const double* __restrict__ nrm,
                                                   T* __restrict__ F, int64_t ldf,
                                                   double* __restrict__ d, int64_t n,
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(256) void k_downdate(const T* __restrict__ X, int32
                 g.acc[a][b][v] = (T)(-kv);
             }
     // overlap with the current factor
-    if (Nr > 0) g.mainloop(lds, F + r0 * ldf, ldf, rows, FS, ldf, BN, Nr);
+    if (Nr > 0) g.mainloop_live(lds, F + r0 * ldf, ldf, rows, FS, ldf, BN, Nr);
     // T = K(R,S_i) - F_R F_{S_i}^T into LDS
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(256) void k_downdate(const T* __restrict__ X, int32
                 lds[g.row_of(a, v) * TP + g.col_of(b)] = -g.acc[a][b][v];
     __syncthreads();
     g.zero();
-    g.lds_global(lds, TP, LinvT, BMAX, (k + 7) & ~7);
+    g.lds_global_tri_live(lds, TP, LinvT, BMAX, (k + 7) & ~7);  // L_c^-T is triangular
     // write the new columns and the squared row norms
     double part[G::NTM][Mfma<T>::ACC];
 #pragma unroll

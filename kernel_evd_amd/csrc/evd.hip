@@ -168,6 +168,168 @@ __global__ __launch_bounds__(256) void k_rowdot(const T* __restrict__ F, int64_t
     }
 }
 
+// One fused Sinkhorn pass over F (symmetric scaling of K^ = F F^T).
+// With the current iterate s = bscale * b and u = uscale * uraw = F^T s,
+// each row i gives y_i = F_i . u = (K^ s)_i, the scaled row sum
+// rs_i = s_i y_i and the next iterate
+//   undamped (Sinkhorn-Knopp):  b'_i = 1 / y_i
+//   damped:                     b'_i = sqrt(s_i / y_i),
+// and the same loaded row adds b'_i F_i to this block's partial of
+// F^T b' -- so one iteration streams F once instead of twice.  A block
+// owns SK_ROWS rows and walks them SKR at a time; each thread keeps VPT
+// 16-byte column vectors of u and of its F^T b' partial in registers.
+// Per block: bstat = [min y, max |rs - 1|, sum rs].
+constexpr int SK_ROWS = 512;
+
+template <typename T> struct SkRows;
+template <> struct SkRows<float> { static constexpr int R = 4; };
+template <> struct SkRows<double> { static constexpr int R = 2; };
+
+template <typename T, int VPT>
+__global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, int64_t ldf, int64_t n,
+                                                       int64_t r, const double* __restrict__ uraw,
+                                                       double uscale, const double* __restrict__ b,
+                                                       double bscale, int damped, double* b_next,
+                                                       double* part, int64_t rpad, double* bstat)
+{
+    constexpr int V = 16 / (int)sizeof(T), R = SkRows<T>::R;
+    typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type vec_t;
+    __shared__ double red[2][4][R];
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int64_t r0 = (int64_t)blockIdx.x * SK_ROWS;
+    int64_t r1 = r0 + SK_ROWS < n ? r0 + SK_ROWS : n;
+    double uv[VPT][V], acc[VPT][V];
+#pragma unroll
+    for (int q = 0; q < VPT; ++q)
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            int64_t c = ((int64_t)threadIdx.x + 256 * q) * V + e;
+            uv[q][e] = c < r ? uscale * uraw[c] : 0.0;
+            acc[q][e] = 0.0;
+        }
+    // two register sets, each indexed only by compile-time constants
+    vec_t f0[R][VPT], f1[R][VPT];
+    auto load = [&](vec_t (&f)[R][VPT], int64_t i0) {
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr)
+#pragma unroll
+            for (int q = 0; q < VPT; ++q) {
+                int64_t c = ((int64_t)threadIdx.x + 256 * q) * V;
+                if (i0 + rr < r1 && c < r)
+                    f[rr][q] = *reinterpret_cast<const vec_t*>(F + (i0 + rr) * ldf + c);
+                else
+                    f[rr][q] = vec_t{};
+            }
+    };
+    double mn = INFINITY, dv = 0.0, srs = 0.0;
+    auto group = [&](const vec_t (&f)[R][VPT], int64_t i0, double (&rd)[4][R]) {
+        double fv[R][VPT][V];
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr)
+#pragma unroll
+            for (int q = 0; q < VPT; ++q) {
+                const T* e = reinterpret_cast<const T*>(&f[rr][q]);
+                int64_t c = ((int64_t)threadIdx.x + 256 * q) * V;
+#pragma unroll
+                for (int k = 0; k < V; ++k) fv[rr][q][k] = c + k < r ? (double)e[k] : 0.0;
+            }
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            double d = 0.0;
+#pragma unroll
+            for (int q = 0; q < VPT; ++q)
+#pragma unroll
+                for (int k = 0; k < V; ++k) d += fv[rr][q][k] * uv[q][k];
+            d = wave_sum(d);
+            if (lane == 0) rd[wv][rr] = d;
+        }
+        // the iterate's entries for these rows, read before the barrier
+        double sv[R];
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) sv[rr] = i0 + rr < r1 ? bscale * b[i0 + rr] : 1.0;
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+            int64_t i = i0 + rr;
+            if (i >= r1) break;
+            double y = rd[0][rr] + rd[1][rr] + rd[2][rr] + rd[3][rr];
+            double si = sv[rr];
+            double rs = si * y;
+            double bn = damped ? sqrt(si / y) : 1.0 / y;
+            if (threadIdx.x == 0) {
+                b_next[i] = bn;
+                mn = fmin(mn, y);
+                dv = fmax(dv, fabs(rs - 1.0));
+                srs += rs;
+            }
+#pragma unroll
+            for (int q = 0; q < VPT; ++q)
+#pragma unroll
+                for (int k = 0; k < V; ++k) acc[q][k] += bn * fv[rr][q][k];
+        }
+    };
+    // red[0] / red[1] alternate between groups: a group's partials are
+    // written before its barrier and read after it, and the next write to
+    // the same slot comes after the following barrier.
+    int ng = (int)((r1 - r0 + R - 1) / R);
+    if (ng > 0) load(f0, r0);
+    for (int g = 0; g < ng; g += 2) {
+        int64_t i0 = r0 + (int64_t)g * R;
+        if (g + 1 < ng) load(f1, i0 + R);
+        group(f0, i0, red[0]);
+        if (g + 1 >= ng) break;
+        if (g + 2 < ng) load(f0, i0 + 2 * R);
+        group(f1, i0 + R, red[1]);
+    }
+#pragma unroll
+    for (int q = 0; q < VPT; ++q)
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            int64_t c = ((int64_t)threadIdx.x + 256 * q) * V + e;
+            if (c < r) part[(int64_t)blockIdx.x * rpad + c] = acc[q][e];
+        }
+    if (threadIdx.x == 0) {
+        bstat[3 * blockIdx.x + 0] = mn;
+        bstat[3 * blockIdx.x + 1] = dv;
+        bstat[3 * blockIdx.x + 2] = srs;
+    }
+}
+
+// [min, max, sum] over the nb per-block triples, in block order.
+__global__ __launch_bounds__(256) void k_stat_reduce(const double* bstat, int nb, double* out)
+{
+    __shared__ double smn[256], sdv[256], ssm[256];
+    double mn = INFINITY, dv = 0.0, sm = 0.0;
+    for (int j = threadIdx.x; j < nb; j += 256) {
+        mn = fmin(mn, bstat[3 * j]);
+        dv = fmax(dv, bstat[3 * j + 1]);
+        sm += bstat[3 * j + 2];
+    }
+    smn[threadIdx.x] = mn;
+    sdv[threadIdx.x] = dv;
+    ssm[threadIdx.x] = sm;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + o]);
+            sdv[threadIdx.x] = fmax(sdv[threadIdx.x], sdv[threadIdx.x + o]);
+            ssm[threadIdx.x] += ssm[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = smn[0];
+        out[1] = sdv[0];
+        out[2] = ssm[0];
+    }
+}
+
+__global__ void k_scale_d(double* p, int64_t n, double a)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] *= a;
+}
+
 __global__ void k_block_minmax(const double* bmin, const double* bdev, int nb, double* out)
 {
     __shared__ double smn[256], sdv[256];
@@ -218,7 +380,7 @@ __global__ void k_weights(const double* x, int64_t n, int mode, double* w)
 // single FP32 accumulation.
 template <typename T> struct GramCfg;
 #ifndef KEVD_GRAM_PROMOTE
-#define KEVD_GRAM_PROMOTE 1024
+#define KEVD_GRAM_PROMOTE 2048
 #endif
 template <> struct GramCfg<float> {
     static constexpr int BI = 128, BJ = 128, KC = 32, TM = 2, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
@@ -497,7 +659,7 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const double* part, int nsp
 // column c == kout (if extra) goes to P1_i in FP64.  Y rows are the B
 // operand (kc x r, leading dimension ldy).
 template <typename T>
-__global__ __launch_bounds__(256) void k_ugemm(const T* __restrict__ F, int64_t ldf, int64_t n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ugemm(const T* __restrict__ F, int64_t ldf, int64_t n,
                                                int64_t r, const T* __restrict__ Y, int64_t ldy,
                                                int kc, int kout, const double* __restrict__ scale,
                                                float* U, double* P1)
@@ -509,7 +671,7 @@ __global__ __launch_bounds__(256) void k_ugemm(const T* __restrict__ F, int64_t 
     int64_t r0 = (int64_t)blockIdx.x * Cfg<T>::BM;
     int c0 = blockIdx.y * Cfg<T>::BN;
     g.set_live_cols(kc - c0);
-    g.mainloop(lds, F + r0 * ldf, ldf, n - r0, Y + (int64_t)c0 * ldy, ldy, kc - c0, r);
+    g.mainloop_live(lds, F + r0 * ldf, ldf, n - r0, Y + (int64_t)c0 * ldy, ldy, kc - c0, r);
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
 #pragma unroll
@@ -579,6 +741,12 @@ __global__ void k_rowsum_dev(const double* P1, int64_t n, double* bdev)
     if (threadIdx.x == 0) bdev[blockIdx.x] = s[0];
 }
 
+__global__ void k_fill_d(double* p, int64_t n, double v)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void k_inv_sqrt_tail(const double* lam, int64_t r, int k, double* out)
 {
     int c = blockIdx.x * 256 + threadIdx.x;
@@ -602,6 +770,7 @@ kevd_status colsum(kevd_ctx* ctx, const double* w, double* c)
     int64_t rpad = KEVD_ROUND_UP(r, 4);
     double* part = nullptr;
     KEVD_TRY(ctx_buf_t(ctx, "e_cpart", (size_t)nb * rpad, &part));
+    ProfScope ps(ctx, TC_MATVEC);
     k_colsum<T><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, w, part, rpad);
     k_colsum_reduce<<<(unsigned)((r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, r, rpad, c);
     KEVD_HIP(ctx, hipGetLastError());
@@ -619,13 +788,16 @@ kevd_status rowdot(kevd_ctx* ctx, const double* c, double* y, int mode, const do
     KEVD_TRY(ctx_buf_t(ctx, "e_bmin", nb, &bmin));
     KEVD_TRY(ctx_buf_t(ctx, "e_bdev", nb, &bdev));
     KEVD_TRY(ctx_buf_t(ctx, "e_mm", 2, &mm));
-    if (r <= ROWDOT_LDS_MAX)
-        k_rowdot<T, true><<<nb, 256, r * sizeof(double), ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, c, y,
-                                                                       bmin, mode, s, s_new, bdev);
-    else
-        k_rowdot<T, false><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, c, y, bmin, mode,
-                                                       s, s_new, bdev);
-    k_block_minmax<<<1, 256, 0, ctx->stream>>>(bmin, mode == 1 ? bdev : nullptr, nb, mm);
+    {
+        ProfScope ps(ctx, TC_MATVEC);
+        if (r <= ROWDOT_LDS_MAX)
+            k_rowdot<T, true><<<nb, 256, r * sizeof(double), ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, c,
+                                                                           y, bmin, mode, s, s_new, bdev);
+        else
+            k_rowdot<T, false><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, c, y, bmin,
+                                                           mode, s, s_new, bdev);
+        k_block_minmax<<<1, 256, 0, ctx->stream>>>(bmin, mode == 1 ? bdev : nullptr, nb, mm);
+    }
     KEVD_HIP(ctx, hipGetLastError());
     KEVD_HIP(ctx, hipMemcpyAsync(host_mm, mm, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -935,6 +1107,8 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
             std::swap(Q, Y);
         }
         *iters = it;
+        ctx->tm.eig_iters = it;
+        ctx->tm.eig_householder = householder;
         if (broken) continue;  // a Cholesky QR pass lost definiteness: redo with Householder
         // Ritz vectors of the leading k: X = Q S(:, l-k:l)
         BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, (rocblas_int)k, Lb, &one, Q, R,
@@ -972,10 +1146,7 @@ kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* re
     KEVD_TRY(ctx_buf_t(ctx, "e_pinfo", 1, &info));
     unsigned gn = (unsigned)((n + 255) / 256);
     double mm[2];
-    {
-        ProfScope ps(ctx, TC_MATVEC);
-        KEVD_TRY(colsum<T>(ctx, nullptr, t));  // t = F^T 1
-    }
+    KEVD_TRY(colsum<T>(ctx, nullptr, t));  // t = F^T 1
     KEVD_TRY(rowdot<T>(ctx, t, dt, 0, nullptr, nullptr, mm));  // d~ = F t
     res->min_degree = mm[0];
     if (!(mm[0] > 0.0))
@@ -1062,34 +1233,97 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     KEVD_TRY(ctx_buf_t(ctx, "e_lam", r, &lam));
     KEVD_TRY(ctx_buf_t(ctx, "e_isq", k, &isq));
     unsigned gn = (unsigned)((n + 255) / 256);
-    // start from s = 1 / sqrt(mean(K^ 1)) so that s K^ s is O(1)
+    // start from s = 1 / sqrt(mean(K^ 1)) so that s K^ s is O(1); with
+    // u = F^T 1, mean(K^ 1) = 1^T F F^T 1 / n = |u|^2 / n.
     double mm[2];
     KEVD_TRY(colsum<T>(ctx, nullptr, u));
-    KEVD_TRY(rowdot<T>(ctx, u, Ks, 0, nullptr, nullptr, mm));
-    if (!(mm[0] > 0.0))
-        return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
-    std::vector<double> hk((size_t)n);
-    KEVD_HIP(ctx, hipMemcpy(hk.data(), Ks, n * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> hu((size_t)r);
+    KEVD_HIP(ctx, hipMemcpyAsync(hu.data(), u, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     double mean = 0.0;
-    for (double v : hk) mean += v;
+    for (double v : hu) mean += v * v;
     mean /= (double)n;
-    std::fill(hk.begin(), hk.end(), 1.0 / std::sqrt(mean));
-    KEVD_HIP(ctx, hipMemcpy(s, hk.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    if (!(mean > 0.0)) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
     int it = 0;
     double dev = INFINITY;
     int maxit = p->sinkhorn_max_iter > 0 ? p->sinkhorn_max_iter : 1000;
     double tol = p->sinkhorn_tol > 0.0 ? p->sinkhorn_tol : 1e-12;
-    for (it = 1; it <= maxit; ++it) {
-        {
-            ProfScope ps(ctx, TC_MATVEC);
-            KEVD_TRY(colsum<T>(ctx, s, u));
+    constexpr int V = 16 / (int)sizeof(T), VPT_MAX = sizeof(T) == 4 ? 4 : 8;
+    int vpt = (int)(((r + V - 1) / V + 255) / 256);
+    if (vpt <= VPT_MAX) {
+        // One fused pass per iteration.  Undamped Sinkhorn-Knopp
+        // s <- 1 / (K^ s): near the fixed point the error in log s
+        // contracts by P = S K^ S, whose eigenvalue 1 (the common scale of
+        // s) does not decay but alternates; it is removed by rescaling
+        // with a = sqrt(mean(s .* K^ s)), so the rate is the second
+        // eigenvalue of P.  Three rises of the deviation in a row switch to
+        // the damped update s <- sqrt(s / K^ s) (rate <= 1/2).
+        int nb = (int)((n + SK_ROWS - 1) / SK_ROWS);
+        int64_t rpad = KEVD_ROUND_UP(r, 4);
+        double *part, *bstat, *st3;
+        KEVD_TRY(ctx_buf_t(ctx, "e_skpart", (size_t)nb * rpad, &part));
+        KEVD_TRY(ctx_buf_t(ctx, "e_skstat", (size_t)3 * nb, &bstat));
+        KEVD_TRY(ctx_buf_t(ctx, "e_skst3", 3, &st3));
+        double* u2;
+        KEVD_TRY(ctx_buf_t(ctx, "e_u2", r, &u2));
+        // s_0 = c0 * 1 and u_0 = F^T s_0 = c0 * (F^T 1), which u holds
+        double c0 = 1.0 / std::sqrt(mean), bscale = c0, uscale = c0;
+        k_fill_d<<<gn, 256, 0, ctx->stream>>>(s, n, 1.0);
+        bool damped = false;
+        int rises = 0;
+        double prev = INFINITY, st[3];
+        for (it = 1; it <= maxit; ++it) {
+            {
+                ProfScope ps(ctx, TC_MATVEC);
+#define KEVD_SK_PASS(VP)                                                                                     \
+    k_sinkhorn_pass<T, VP><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, u, uscale, s,     \
+                                                       bscale, damped ? 1 : 0, s2, part, rpad, bstat)
+                if (vpt <= 1)
+                    KEVD_SK_PASS(1);
+                else if (vpt <= 2)
+                    KEVD_SK_PASS(2);
+                else if (vpt <= 4)
+                    KEVD_SK_PASS(4);
+                else
+                    KEVD_SK_PASS(VPT_MAX);
+#undef KEVD_SK_PASS
+                k_colsum_reduce<<<(unsigned)((r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, r, rpad, u2);
+                k_stat_reduce<<<1, 256, 0, ctx->stream>>>(bstat, nb, st3);
+            }
+            KEVD_HIP(ctx, hipGetLastError());
+            KEVD_HIP(ctx, hipMemcpyAsync(st, st3, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (!(st[0] > 0.0)) {
+                // the first pass sees K^ 1 up to a positive constant: a degree
+                if (it == 1) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
+                return kevd_fail(ctx, KEVD_ENONPOS_Q, "Sinkhorn: non-positive K^ s");
+            }
+            dev = st[1];
+            if (dev <= tol || it == maxit) break;  // s = bscale * b is the iterate measured
+            bool was_damped = damped;
+            rises = dev > prev ? rises + 1 : 0;
+            prev = dev;
+            if (!damped && rises >= 3) damped = true;
+            bscale = was_damped ? 1.0 : std::sqrt(st[2] / (double)n);
+            uscale = bscale;  // u2 = F^T b_next
+            std::swap(s, s2);
+            std::swap(u, u2);
         }
-        KEVD_TRY(rowdot<T>(ctx, u, Ks, 1, s, s2, mm));
-        if (!(mm[0] > 0.0))
-            return kevd_fail(ctx, KEVD_ENONPOS_Q, "Sinkhorn: non-positive K^ s");
-        dev = mm[1];
-        if (dev <= tol) break;
-        std::swap(s, s2);
+        if (bscale != 1.0) k_scale_d<<<gn, 256, 0, ctx->stream>>>(s, n, bscale);
+    } else {
+        // rank beyond the register tiling: two passes per damped iteration
+        k_fill_d<<<gn, 256, 0, ctx->stream>>>(s, n, 1.0 / std::sqrt(mean));
+        for (it = 1; it <= maxit; ++it) {
+            KEVD_TRY(colsum<T>(ctx, s, u));
+            KEVD_TRY(rowdot<T>(ctx, u, Ks, 1, s, s2, mm));
+            if (!(mm[0] > 0.0)) {
+                if (it == 1) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
+                return kevd_fail(ctx, KEVD_ENONPOS_Q, "Sinkhorn: non-positive K^ s");
+            }
+            dev = mm[1];
+            if (dev <= tol || it == maxit) break;
+            std::swap(s, s2);
+        }
     }
     res->sinkhorn_iters = it > maxit ? maxit : it;
     res->rowsum_dev = dev;
@@ -1104,11 +1338,13 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     k_inv_sqrt_tail<<<(k + 255) / 256, 256, 0, ctx->stream>>>(lam, r, k, isq);
     KEVD_TRY(ugemm<T>(ctx, Vk, k, isq, nullptr, s, ctx->U, nullptr));
     KEVD_HIP(ctx, hipMemcpyAsync(res->lam, lam, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<double> hs((size_t)n);
-    KEVD_HIP(ctx, hipMemcpy(hs.data(), s, n * sizeof(double), hipMemcpyDeviceToHost));
-    double smin = INFINITY;
-    for (double v : hs) smin = std::min(smin, v);
-    res->min_degree = smin;
+    double* smm;
+    KEVD_TRY(ctx_buf_t(ctx, "e_smm", 2, &smm));
+    k_block_minmax<<<1, 256, 0, ctx->stream>>>(s, nullptr, (int)n, smm);
+    KEVD_HIP(ctx, hipGetLastError());
+    KEVD_HIP(ctx, hipMemcpyAsync(mm, smm, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    res->min_degree = mm[0];
     return KEVD_OK;
 }
 
@@ -1136,6 +1372,8 @@ extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_ev
     ctx->tm.matvec_ms = 0;
     ctx->tm.dense_ms = 0;
     ctx->tm.u_ms = 0;
+    ctx->tm.eig_iters = 0;
+    ctx->tm.eig_householder = 0;
     prof_reset(ctx);
     size_t ubytes = (size_t)ctx->n * p->k_out * sizeof(float);
     if (ctx->U_bytes < ubytes) {

@@ -167,6 +167,164 @@ struct NTGemm {
         }
     }
 
+    // Like gload, for chunks that lie wholly below kvalid: only the row
+    // bound is checked, so no load waits on a masking step.
+    template <int NV>
+    __device__ static inline void gload_full(uint4 (&r)[NV], const T* src, int64_t ld,
+                                             int64_t nvalid, int64_t k0)
+    {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            int lin = threadIdx.x + 256 * j;
+            int row = lin / VPR, cv = lin % VPR;
+            r[j] = row < nvalid ? *reinterpret_cast<const uint4*>(src + (int64_t)row * ld + k0 + cv * EPV)
+                                : make_uint4(0, 0, 0, 0);
+        }
+    }
+
+    // acc += As Bs^T over one KC chunk for NACT live column tiles (a
+    // compile-time count, so no branch splits the MFMA stream).  The operand
+    // fragments of sub-step s + 1 are read from LDS while sub-step s
+    // multiplies, and the MFMAs cycle over the accumulators so consecutive
+    // ones are independent.
+    template <int NACT>
+    __device__ inline void compute_n(const T* As, const T* Bs)
+    {
+        constexpr int NS = KC / 8;
+        vec_t a[2][TM], b[2][NACT];
+        auto frag = [&](int q, int s) {
+#pragma unroll
+            for (int t = 0; t < TM; ++t)
+                a[q][t] = *reinterpret_cast<const vec_t*>(
+                    As + (wm * TM * MT + t * MT + li) * PITCH + s * 8 + h * M::VEC);
+#pragma unroll
+            for (int t = 0; t < NACT; ++t)
+                b[q][t] = *reinterpret_cast<const vec_t*>(
+                    Bs + (wn * TN * MT + t * MT + li) * PITCH + s * 8 + h * M::VEC);
+        };
+        frag(0, 0);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            if (s + 1 < NS) frag((s + 1) & 1, s + 1);
+#pragma unroll
+            for (int j = 0; j < M::VEC; ++j)
+#pragma unroll
+                for (int y = 0; y < NACT; ++y)
+#pragma unroll
+                    for (int x = 0; x < TM; ++x)
+                        M::mma(acc[x][y], M::get(a[s & 1][x], j), M::get(b[s & 1][y], j));
+        }
+    }
+
+    // mainloop for NACT live column tiles.  Global loads run two chunks
+    // ahead: while chunk c is multiplied from LDS stage c&1, chunk c+1 waits
+    // in one register set (stored to the other stage after the multiply)
+    // and chunk c+2 is loading into the other, so a load has two chunks of
+    // MFMA work to arrive.  Chunks past the last full one mask K.
+    template <int NACT>
+    __device__ inline void mainloop_n(T* lds, const T* A, int64_t lda, int64_t arows,
+                                      const T* B, int64_t ldb, int64_t brows, int64_t K)
+    {
+        int64_t nk = (K + KC - 1) / KC, nfull = K / KC;
+        if (nk <= 0) return;
+        uint4 ra0[NVA], rb0[NVB], ra1[NVA], rb1[NVB];
+        auto load = [&](uint4 (&ra)[NVA], uint4 (&rb)[NVB], int64_t c) {
+            if (c < nfull) {
+                gload_full<NVA>(ra, A, lda, arows, c * KC);
+                gload_full<NVB>(rb, B, ldb, brows, c * KC);
+            } else {
+                gload<NVA>(ra, A, lda, arows, c * KC, K);
+                gload<NVB>(rb, B, ldb, brows, c * KC, K);
+            }
+        };
+        load(ra0, rb0, 0);
+        sstore<NVA>(ra0, lds);
+        sstore<NVB>(rb0, lds + BM * PITCH);
+        if (nk > 1) load(ra1, rb1, 1);
+        __syncthreads();
+        for (int64_t c = 0; c < nk; c += 2) {
+            // even chunk c: LDS stage 0, c+1 in set 1, c+2 loads into set 0
+            if (c + 2 < nk) load(ra0, rb0, c + 2);
+            compute_n<NACT>(lds, lds + BM * PITCH);
+            if (c + 1 < nk) {
+                sstore<NVA>(ra1, lds + STAGE);
+                sstore<NVB>(rb1, lds + STAGE + BM * PITCH);
+            }
+            __syncthreads();
+            if (c + 1 >= nk) break;
+            // odd chunk c+1: LDS stage 1, c+2 in set 0, c+3 loads into set 1
+            if (c + 3 < nk) load(ra1, rb1, c + 3);
+            compute_n<NACT>(lds + STAGE, lds + STAGE + BM * PITCH);
+            if (c + 2 < nk) {
+                sstore<NVA>(ra0, lds);
+                sstore<NVB>(rb0, lds + BM * PITCH);
+            }
+            __syncthreads();
+        }
+    }
+
+    // mainloop_n dispatched on the wave's live column tiles (nact).
+    template <int N = TN>
+    __device__ inline void mainloop_live(T* lds, const T* A, int64_t lda, int64_t arows,
+                                         const T* B, int64_t ldb, int64_t brows, int64_t K)
+    {
+        static_assert(WN == 1, "nact must be block-uniform: every wave owns all block columns");
+        if (nact == N)
+            mainloop_n<N>(lds, A, lda, arows, B, ldb, brows, K);
+        else if constexpr (N > 1)
+            mainloop_live<N - 1>(lds, A, lda, arows, B, ldb, brows, K);
+    }
+
+    // acc += Ts Bg^T as lds_global, for a B operand that is lower triangular
+    // (Bg[j][m] = 0 for m > j, row j = output column j), NACT live column
+    // tiles and K <= NACT * MT.  Column tile y only needs K < MT (y + 1), so
+    // sub-step s (8 K values) runs the tiles y >= s / (MT / 8): about half
+    // the MFMAs of the full product.  B fragments are read from global
+    // memory one sub-step ahead.
+    template <int NACT>
+    __device__ inline void lds_global_tri(const T* Ts, int tp, const T* Bg, int64_t ldb, int K)
+    {
+        constexpr int SPT = MT / 8, NSM = NACT * SPT;
+        const int ns = K / 8;
+        vec_t b[2][NACT];
+        auto bload = [&](int q, int s) {
+#pragma unroll
+            for (int t = 0; t < NACT; ++t)
+                if (t >= s / SPT)
+                    b[q][t] = *reinterpret_cast<const vec_t*>(
+                        Bg + (int64_t)(wn * TN * MT + t * MT + li) * ldb + s * 8 + h * M::VEC);
+        };
+        if (ns > 0) bload(0, 0);
+#pragma unroll
+        for (int s = 0; s < NSM; ++s) {
+            if (s < ns) {
+                if (s + 1 < ns) bload((s + 1) & 1, s + 1);
+                vec_t a[TM];
+#pragma unroll
+                for (int t = 0; t < TM; ++t)
+                    a[t] = *reinterpret_cast<const vec_t*>(
+                        Ts + (wm * TM * MT + t * MT + li) * tp + s * 8 + h * M::VEC);
+#pragma unroll
+                for (int j = 0; j < M::VEC; ++j)
+#pragma unroll
+                    for (int y = s / SPT; y < NACT; ++y)
+#pragma unroll
+                        for (int x = 0; x < TM; ++x)
+                            M::mma(acc[x][y], M::get(a[x], j), M::get(b[s & 1][y], j));
+            }
+        }
+    }
+
+    template <int N = TN>
+    __device__ inline void lds_global_tri_live(const T* Ts, int tp, const T* Bg, int64_t ldb, int K)
+    {
+        static_assert(WN == 1, "nact must be block-uniform: every wave owns all block columns");
+        if (nact == N)
+            lds_global_tri<N>(Ts, tp, Bg, ldb, K);
+        else if constexpr (N > 1)
+            lds_global_tri_live<N - 1>(Ts, tp, Bg, ldb, K);
+    }
+
     // acc += Ts[0:BM, 0:K] Bg[0:BN, 0:K]^T with Ts in LDS (row pitch tp) and
     // Bg read straight from global memory (a small operand every block
     // shares, served from L2).  K multiple of 8.  The B operand of sub-step
