@@ -243,6 +243,38 @@ typedef struct {
 kevd_status kevd_set_profiling(kevd_ctx* ctx, int enable);
 kevd_status kevd_get_timings(const kevd_ctx* ctx, kevd_timings* out);
 
+/* ------------------------------------------------------------ ensembles
+ * Many independent problems on one GPU.  Each problem is an ordinary
+ * context (points and eps set through the calls above); all contexts must be
+ * on the same device.  The reference runs one problem per program launch
+ * (ks_arpclm.cu main, ks_kevd.cu main); these calls batch that loop.
+ *
+ * kevd_ensemble_arpc: every problem's ARPC (as kevd_arpc, params[i] and
+ * results[i] per problem, one precision for all, no injected pivots) runs
+ * as batched rounds: one launch per round stage for all problems, the
+ * downdate as a persistent work-queue kernel, the round captured once as a
+ * hipGraph and replayed.  Pivots and factors equal kevd_arpc's bit for bit.
+ * Each context then holds its factor, as after kevd_arpc.  *used_graph
+ * (optional) is 1 when the rounds ran as graph launches.  Errors are
+ * reported on ctxs[0]. */
+kevd_status kevd_ensemble_arpc(kevd_ctx** ctxs, int32_t nprob, const kevd_arpc_params* params,
+                               kevd_arpc_result* results, int32_t* used_graph);
+/* kevd_ensemble_evd: kevd_evd on every context with the same parameters,
+ * on `workers` host threads (<= 0: 8), each problem on its own context
+ * stream.  results[i] as kevd_evd's; errors are reported on ctxs[0]. */
+kevd_status kevd_ensemble_evd(kevd_ctx** ctxs, int32_t nprob, const kevd_evd_params* p,
+                              kevd_evd_result* results, int32_t workers);
+/* kevd_autotune_block: the block size among candidates[0..ncand) with the
+ * highest factor-columns-per-second of kevd_arpc on this context's points
+ * (base supplies eps, rank, seed, tol, precision; two timed runs per
+ * candidate, the faster one counts).  Without cols_per_s the choice is
+ * cached per (device, n, padded dim, rank, precision, tol) for the process
+ * and a cached answer is returned without measuring; with it, every
+ * candidate is measured and its rate written.  The run leaves the context
+ * holding the last candidate's factor. */
+kevd_status kevd_autotune_block(kevd_ctx* ctx, const kevd_arpc_params* base, const int32_t* candidates,
+                                int32_t ncand, int32_t* best_block, double* cols_per_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,9 @@ SIGNATURES = {
     "kevd_delay_embed": (C.c_int, [P(dbl), i64, i32, i32, P(dbl)]),
     "kevd_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "kevd_get_timings": (C.c_int, [C.c_void_p, P(Timings)]),
+    "kevd_ensemble_arpc": (C.c_int, [P(C.c_void_p), i32, P(ArpcParams), P(ArpcResult), P(i32)]),
+    "kevd_ensemble_evd": (C.c_int, [P(C.c_void_p), i32, P(EvdParams), P(EvdResult), i32]),
+    "kevd_autotune_block": (C.c_int, [C.c_void_p, P(ArpcParams), P(i32), i32, P(i32), P(dbl)]),
 }
 
 _lib = None

@@ -185,6 +185,25 @@ class Context:
                                                    _p(S, C.c_double)))
         return S
 
+    def autotune_block(self, eps, rank, candidates=(32, 64, 128), seed=0, tol=1e-10, prec="mixed",
+                       measure=False):
+        """Block size with the best factor columns/s for this point set
+        (kevd_autotune_block).  measure=True times every candidate and
+        returns the rates too; otherwise a cached choice may be returned."""
+        p = L.ArpcParams()
+        p.eps = eps
+        p.rank = rank
+        p.block = 1
+        p.seed = seed
+        p.tol = tol
+        p.prec = PREC[prec]
+        cand = np.ascontiguousarray(candidates, dtype=np.int32)
+        best = C.c_int32()
+        rates = np.zeros(cand.size) if measure else None
+        self._check(self._lib.kevd_autotune_block(self._h, C.byref(p), _p(cand, C.c_int32), cand.size,
+                                                  C.byref(best), _p(rates, C.c_double) if measure else None))
+        return (best.value, rates) if measure else best.value
+
     # ------------------------------------------------------------ timing
     def set_profiling(self, on=True):
         self._check(self._lib.kevd_set_profiling(self._h, 1 if on else 0))
@@ -193,6 +212,79 @@ class Context:
         t = L.Timings()
         self._check(self._lib.kevd_get_timings(self._h, C.byref(t)))
         return {name: getattr(t, name) for name, _ in L.Timings._fields_}
+
+
+def ensemble_arpc(contexts, eps, ranks, blocks, seeds=None, tol=1e-10, prec="mixed"):
+    """Batched ARPC over several contexts (kevd_ensemble_arpc).  eps, ranks,
+    blocks (and seeds) are per-problem sequences.  Returns (list of
+    ArpcOutput, used_graph)."""
+    lib = L.lib()
+    P = len(contexts)
+    seeds = [0] * P if seeds is None else list(seeds)
+    ps = (L.ArpcParams * P)()
+    rs = (L.ArpcResult * P)()
+    keep = []
+    for i, ctx in enumerate(contexts):
+        ps[i].eps = eps[i]
+        ps[i].rank = ranks[i]
+        ps[i].block = blocks[i]
+        ps[i].seed = seeds[i]
+        ps[i].tol = tol
+        ps[i].prec = PREC[prec]
+        mr = 4 * ((ranks[i] + blocks[i] - 1) // blocks[i]) + 64
+        ps[i].max_rounds = mr
+        piv = np.zeros(ranks[i], dtype=np.int64)
+        acc = np.zeros(mr, dtype=np.int32)
+        err = np.zeros(mr, dtype=np.float64)
+        keep.append((piv, acc, err))
+        rs[i].pivots = _p(piv, C.c_int64)
+        rs[i].accepted = _p(acc, C.c_int32)
+        rs[i].trace_err = _p(err, C.c_double)
+        rs[i].rounds_cap = mr
+    hs = (C.c_void_p * P)(*[c._h.value for c in contexts])
+    used = C.c_int32()
+    st = lib.kevd_ensemble_arpc(hs, P, ps, rs, C.byref(used))
+    if st != 0:
+        raise KevdError(st, lib.kevd_last_error(contexts[0]._h).decode())
+    outs = []
+    for i in range(P):
+        piv, acc, err = keep[i]
+        nr = rs[i].rounds
+        outs.append(ArpcOutput(rs[i].rank, nr, piv[:rs[i].rank].copy(), acc[:nr].copy(), err[:nr].copy()))
+    return outs, bool(used.value)
+
+
+def ensemble_evd(contexts, ranks, k_out=32, norm="sinkhorn", prec="mixed", sinkhorn_tol=1e-12,
+                 sinkhorn_max_iter=1000, want_U=False, workers=8):
+    """kevd_evd on every context concurrently (kevd_ensemble_evd)."""
+    lib = L.lib()
+    P = len(contexts)
+    p = L.EvdParams()
+    p.k_out = k_out
+    p.norm = KEVD_NORM_SINKHORN if norm == "sinkhorn" else KEVD_NORM_REF
+    p.sinkhorn_tol = sinkhorn_tol
+    p.sinkhorn_max_iter = sinkhorn_max_iter
+    p.prec = PREC[prec]
+    p.all_eigenvalues = 0
+    rs = (L.EvdResult * P)()
+    lams, Us = [], []
+    for i, ctx in enumerate(contexts):
+        lam = np.zeros(ranks[i])
+        lams.append(lam)
+        rs[i].lam = _p(lam, C.c_double)
+        U = None
+        if want_U:
+            U = np.zeros((ctx.n, k_out), dtype=np.float32)
+            rs[i].U = _p(U, C.c_float)
+        Us.append(U)
+    hs = (C.c_void_p * P)(*[c._h.value for c in contexts])
+    st = lib.kevd_ensemble_evd(hs, P, C.byref(p), rs, workers)
+    if st != 0:
+        raise KevdError(st, lib.kevd_last_error(contexts[0]._h).decode())
+    for ctx in contexts:
+        ctx._k_out = k_out
+    return [EvdOutput(lams[i], Us[i], rs[i].rowsum_dev, rs[i].shift, rs[i].min_degree, rs[i].sinkhorn_iters)
+            for i in range(P)]
 
 
 def bandwidth_post(eps, S):

@@ -24,7 +24,7 @@ LIB = os.path.join(PKG, "libkevd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["ctx.hip", "arpc.hip", "factor.hip", "evd.hip", "bandwidth.hip"]
+HIP_SOURCES = ["ctx.hip", "arpc.hip", "factor.hip", "evd.hip", "bandwidth.hip", "ensemble.hip"]
 CPP_SOURCES = ["ksgen.cpp"]
 DRIVERS = ["kevd_gen.cpp", "kevd_arpc.cpp", "kevd_evd.cpp", "kevd_bandwidth.cpp"]
 

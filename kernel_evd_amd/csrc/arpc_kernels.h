@@ -1,0 +1,545 @@
+// arpc_kernels.h -- device bodies of one accelerated-RPCholesky round,
+// shared by the single-problem kernels (arpc.hip) and the batched ensemble
+// kernels (ensemble.hip).  Each body works on one problem given by plain
+// pointers and sizes; the block index it would take from blockIdx is an
+// argument, so a batched kernel can map its blocks onto (problem, block)
+// pairs.  The round index comes from the device state, which makes a round
+// free of host-side per-round arguments (capturable as a graph).
+//
+// One round (README_arpclm.md:23-39; ks_arpclm.cu:793-1046):
+//   reduce_d + weights_scan + draw   S' ~ d / sum d      (propose_block 193-224)
+//   form_H                           H = K(S',S') - F(S',:) F(S',:)^T
+//                                                         (extract_H + Dsyrk 820-874)
+//   rejchol                          accept / Cholesky / L_c^-1  (rejection_sample 300-408)
+//   gather_pivots                    X(S_i,:), F(S_i,:), L_c^-1 in the GEMM precision
+//   downdate                         G = (K(:,S_i) - F F(S_i,:)^T) L_c^-T,
+//                                    F(:,Nr:Nr+k) = G, d = max(d - rowsq(G), 0)
+//                                                         (929-995, 455-478)
+//   commit                           S = S u S_i, Nr += k   (1017-1019)
+#pragma once
+
+#include "gemm.h"
+#include "internal.h"
+
+namespace arpck {
+
+constexpr int SCAN_CHUNK = 4096;       // residual entries per scan block
+constexpr int MAX_SCAN_BLOCKS = 8192;  // => n <= 33,554,432
+constexpr int BMAX = 128;              // largest block size b
+
+struct ArpcState {
+    int32_t done;
+    int32_t stop_reason;  // 1 rank reached, 2 zero residual mass, 4 pivot list exhausted
+    int32_t k;            // accepted this round, after truncation to the target rank
+    int32_t nprop;        // proposals this round
+    int64_t Nr;           // rank so far
+    int64_t rank;         // target
+    int64_t inj_pos;      // next injected pivot
+    int32_t round;        // rounds completed
+    int32_t pad;
+    uint64_t W;           // fixed-point residual mass of the last draw
+};
+
+// ---------------------------------------------------------------- sampling
+
+// Per-block max and FP64 sum of the residual diagonal (fixed reduction shape,
+// so the sums are bit-reproducible).
+__device__ inline void reduce_d_body(const double* __restrict__ d, int64_t n, double* pmax, double* psum,
+                                     const ArpcState* st, int force, int blk)
+{
+    if (!force && st->done) return;
+    int64_t base = (int64_t)blk * SCAN_CHUNK;
+    double mx = 0.0, sm = 0.0;
+#pragma unroll 4
+    for (int j = 0; j < SCAN_CHUNK / 256; ++j) {
+        int64_t i = base + threadIdx.x + 256 * j;
+        if (i < n) {
+            double v = d[i];
+            mx = fmax(mx, v);
+            sm += v;
+        }
+    }
+    __shared__ double smx[4], ssm[4];
+    mx = wave_max(mx);
+    sm = wave_sum(sm);
+    if ((threadIdx.x & 63) == 0) {
+        smx[threadIdx.x >> 6] = mx;
+        ssm[threadIdx.x >> 6] = sm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        pmax[blk] = fmax(fmax(smx[0], smx[1]), fmax(smx[2], smx[3]));
+        psum[blk] = (ssm[0] + ssm[1]) + (ssm[2] + ssm[3]);
+    }
+}
+
+// Fixed-point weights w_i = floor(d_i 2^(shift_base - e)), d_max = m 2^e,
+// then a block-local inclusive scan (integer, hence order independent).
+__device__ inline void weights_scan_body(const double* __restrict__ d, int64_t n,
+                                         const double* __restrict__ pmax, int nb, int shift_base,
+                                         uint64_t* prefix, uint64_t* btot, const ArpcState* st, int blk)
+{
+    if (st->done) return;
+    __shared__ uint64_t w[SCAN_CHUNK];
+    __shared__ uint64_t wtot[4];
+    __shared__ double gmx_s[4];
+    double mx = 0.0;
+    for (int j = threadIdx.x; j < nb; j += 256) mx = fmax(mx, pmax[j]);
+    mx = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) gmx_s[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    double gmax = fmax(fmax(gmx_s[0], gmx_s[1]), fmax(gmx_s[2], gmx_s[3]));
+    int e = 0;
+    frexp(gmax, &e);
+    int shift = shift_base - e;
+    bool live = gmax > 0.0;
+    int64_t base = (int64_t)blk * SCAN_CHUNK;
+    for (int j = 0; j < SCAN_CHUNK / 256; ++j) {
+        int idx = threadIdx.x + 256 * j;
+        int64_t i = base + idx;
+        uint64_t wi = 0;
+        if (live && i < n) {
+            double v = d[i];
+            v = v > 0.0 ? v : 0.0;
+            wi = (uint64_t)ldexp(v, shift);
+        }
+        w[idx] = wi;
+    }
+    __syncthreads();
+    // each thread scans 16 consecutive weights
+    constexpr int PER = SCAN_CHUNK / 256;
+    uint64_t loc[PER];
+    uint64_t run = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        run += w[threadIdx.x * PER + q];
+        loc[q] = run;
+    }
+    // exclusive scan of the 256 thread totals
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t incl = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint64_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wtot[wv] = incl;
+    __syncthreads();
+    uint64_t off = incl - run;
+    for (int q = 0; q < wv; ++q) off += wtot[q];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) w[threadIdx.x * PER + q] = loc[q] + off;
+    __syncthreads();
+    for (int j = 0; j < SCAN_CHUNK / 256; ++j) {
+        int idx = threadIdx.x + 256 * j;
+        int64_t i = base + idx;
+        if (i < n) prefix[i] = w[idx];
+    }
+    if (threadIdx.x == 0) btot[blk] = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+}
+
+// Record the previous round's trace error, stop on zero mass, then draw the
+// b proposals by integer inverse CDF (or take them from the injected list).
+// The round index is the device state's, so a captured round replays as is.
+__device__ inline void draw_body(const double* __restrict__ psum, const uint64_t* __restrict__ btot, int nb,
+                                 const uint64_t* __restrict__ prefix, int64_t n, double trK, uint64_t seed,
+                                 int b, int64_t* Sprime, ArpcState* st, double* err,
+                                 const int64_t* inject, int64_t n_inject)
+{
+    if (st->done) return;
+    const int round = st->round;
+    __shared__ uint64_t bpre[MAX_SCAN_BLOCKS];
+    __shared__ int nprop_s;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        uint64_t run = 0;
+        for (int j = 0; j < nb; ++j) {
+            s += psum[j];
+            run += btot[j];
+            bpre[j] = run;
+        }
+        if (round > 0) err[round - 1] = s / trK;
+        st->W = run;
+        int np = b;
+        if (inject) {
+            int64_t left = n_inject - st->inj_pos;
+            np = (int)(left < (int64_t)b ? left : (int64_t)b);
+        }
+        if (run == 0) {
+            st->done = 1;
+            st->stop_reason = 2;
+            np = 0;
+        } else if (np <= 0) {
+            st->done = 1;
+            st->stop_reason = 4;
+            np = 0;
+        }
+        st->nprop = np;
+        nprop_s = np;
+    }
+    __syncthreads();
+    int np = nprop_s;
+    if (np == 0) return;
+    int t = threadIdx.x;
+    if (inject) {
+        if (t < np) Sprime[t] = inject[st->inj_pos + t];
+        __syncthreads();
+        if (t == 0) st->inj_pos += np;
+        return;
+    }
+    if (t >= np) return;
+    uint64_t W = bpre[nb - 1];
+    uint64_t T = kevd_mulhi64(kevd_draw_u64(seed, KEVD_STREAM_PROPOSE, (uint32_t)round, (uint32_t)t), W);
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (bpre[mid] > T) hi = mid; else lo = mid + 1;
+    }
+    uint64_t Tl = T - (lo > 0 ? bpre[lo - 1] : 0);
+    int64_t b0 = (int64_t)lo * SCAN_CHUNK;
+    int64_t len = n - b0 < SCAN_CHUNK ? n - b0 : SCAN_CHUNK;
+    int64_t l2 = 0, h2 = len - 1;
+    while (l2 < h2) {
+        int64_t mid = (l2 + h2) >> 1;
+        if (prefix[b0 + mid] > Tl) h2 = mid; else l2 = mid + 1;
+    }
+    Sprime[t] = b0 + l2;
+}
+
+// ------------------------------------------------------------ H = residual
+
+// H(p,q) = K(S'_p, S'_q) - sum_{m<Nr} F(S'_p, m) F(S'_q, m) in FP64, one
+// 16 x 16 tile per block.  K from the FP64 centred points, direct
+// differences.  Rows/columns past the proposal count are zero.
+template <typename T>
+__device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, const T* __restrict__ F,
+                                   int64_t ldf, const int64_t* __restrict__ Sprime, const ArpcState* st,
+                                   double neg_inv_eps, double* H, int bx, int by)
+{
+    if (st->done) return;
+    int np = st->nprop;
+    int64_t Nr = st->Nr;
+    int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    int p = by * 16 + ty, q = bx * 16 + tx;
+    if (by * 16 >= np || bx * 16 >= np) {
+        if (p < BMAX && q < BMAX) H[p * BMAX + q] = 0.0;
+        return;
+    }
+    __shared__ double Fp[16][65], Fq[16][65];
+    int64_t sp = Sprime[p < np ? p : 0], sq = Sprime[q < np ? q : 0];
+    double acc = 0.0;
+    for (int64_t m0 = 0; m0 < Nr; m0 += 64) {
+        // stage 16 rows x 64 columns of F(S'_p) and F(S'_q)
+        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+            int rr = e >> 6, cc = e & 63;
+            int pp = by * 16 + rr, qq = bx * 16 + rr;
+            int64_t m = m0 + cc;
+            double vp = 0.0, vq = 0.0;
+            if (m < Nr) {
+                if (pp < np) vp = (double)F[Sprime[pp] * ldf + m];
+                if (qq < np) vq = (double)F[Sprime[qq] * ldf + m];
+            }
+            Fp[rr][cc] = vp;
+            Fq[rr][cc] = vq;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int c = 0; c < 64; ++c) acc += Fp[ty][c] * Fq[tx][c];
+        __syncthreads();
+    }
+    double kv;
+    if (sp == sq) {
+        kv = 1.0;
+    } else {
+        const double* a = X64 + sp * dp;
+        const double* bq = X64 + sq * dp;
+        double s = 0.0;
+        for (int j = 0; j < dp; ++j) {
+            double dd = a[j] - bq[j];
+            s += dd * dd;
+        }
+        kv = exp(s * neg_inv_eps);
+    }
+    if (p < np && q < np) H[p * BMAX + q] = kv - acc;
+    else H[p * BMAX + q] = 0.0;
+}
+
+// ------------------------------------------------------- rejection block
+
+// Algorithm 2.1 on H (rejection_sample, ks_arpclm.cu:300-408) in one
+// workgroup with H resident in LDS.  Every thread evaluates the acceptance
+// test itself (same Philox counter, same LDS values), so a rejected proposal
+// costs no barrier.  Column i of L is kept transposed in the unused upper
+// triangle of H, its diagonal in Ld.  Work inside a step is spread over a
+// 16 x 16 thread grid (no integer division).  After the loop the compacted
+// inverse X = L_c^-1 is formed in place in the lower triangle by a
+// right-looking triangular inversion (k sequential steps, each a parallel
+// rank-1 update) instead of one thread per column.
+template <typename T>
+__device__ inline void rejchol_body(const double* __restrict__ H, ArpcState* st, uint64_t seed, double tol,
+                                    int inject, const int64_t* __restrict__ Sprime, int64_t* Si, T* LinvT,
+                                    double* Linv64)
+{
+    if (st->done) return;
+    const int round = st->round;
+    __shared__ double Hs[BMAX][BMAX + 1];
+    __shared__ double u[BMAX], Lcol[BMAX], Ld[BMAX];
+    __shared__ int accpos[BMAX];
+    int np = st->nprop;
+    int tid = threadIdx.x;
+    int tx = tid & 15, ty = tid >> 4;
+    for (int p = ty; p < np; p += 16)
+        for (int q = tx; q < np; q += 16) Hs[p][q] = H[p * BMAX + q];
+    __syncthreads();
+    if (tid < np) u[tid] = Hs[tid][tid];
+    __syncthreads();
+    int k = 0;
+    for (int i = 0; i < np; ++i) {
+        double rnd = kevd_u64_to_unit(kevd_draw_u64(seed, KEVD_STREAM_REJECT, (uint32_t)round, (uint32_t)i));
+        double hii = Hs[i][i];
+        bool acc = inject ? (hii > tol) : ((u[i] * rnd < hii) && (hii > tol));
+        if (!acc) continue;
+        double f = 1.0 / sqrt(hii);
+        if (tid >= i && tid < np) Lcol[tid] = Hs[tid][i] * f;
+        __syncthreads();
+        for (int jj = i + 1 + ty; jj < np; jj += 16) {
+            double lj = Lcol[jj];
+            for (int mm = i + 1 + tx; mm <= jj; mm += 16) Hs[jj][mm] -= lj * Lcol[mm];
+        }
+        if (tid > i && tid < np) Hs[i][tid] = Lcol[tid];
+        if (tid == i) Ld[i] = Lcol[i];
+        if (tid == 0) accpos[k] = i;
+        ++k;
+        __syncthreads();
+    }
+    int64_t room = st->rank - st->Nr;
+    if ((int64_t)k > room) k = (int)room;
+    // L_c(a, c) = L(accpos[a], accpos[c]) = Hs[accpos[c]][accpos[a]] for a > c,
+    // L_c(a, a) = Ld[accpos[a]].  X = L_c^-1 lives at Hs[a][t], a >= t
+    // (compacted indices, strictly below the upper-triangle storage).
+    for (int a = ty; a < k; a += 16)
+        for (int t = tx; t <= a; t += 16) Hs[a][t] = (a == t) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int a = 0; a < k; ++a) {
+        double inv = 1.0 / Ld[accpos[a]];
+        if (tid <= a) Hs[a][tid] *= inv;  // finalise row a (k <= 128 < 256 threads)
+        __syncthreads();
+        int pa = accpos[a];
+        for (int j = a + 1 + ty; j < k; j += 16) {
+            double l = Hs[pa][accpos[j]];
+            for (int t = tx; t <= a; t += 16) Hs[j][t] -= l * Hs[a][t];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < BMAX * BMAX; e += 256) {
+        int a = e / BMAX, c = e % BMAX;
+        double v = (a < k && c <= a) ? Hs[a][c] : 0.0;
+        LinvT[e] = (T)v;
+        Linv64[e] = v;
+    }
+    if (tid < BMAX) Si[tid] = tid < k ? Sprime[accpos[tid]] : -1;
+    if (tid == 0) st->k = k;
+}
+
+// Pivot rows in the GEMM precision: X(S_i,:), |x|^2, F(S_i, 0:Nr).  Rows
+// past k are zero.
+template <typename T>
+__device__ inline void gather_pivots_body(const T* __restrict__ Xsrc, const double* __restrict__ nrm,
+                                          int32_t dp, const T* __restrict__ F, int64_t ldf,
+                                          const int64_t* __restrict__ Si, const ArpcState* st, T* XS,
+                                          double* nrmS, T* FS, int c)
+{
+    if (st->done) return;
+    int k = st->k;
+    int64_t Nr = st->Nr;
+    int64_t s = c < k ? Si[c] : -1;
+    for (int j = threadIdx.x; j < dp; j += 256) XS[c * dp + j] = s >= 0 ? Xsrc[s * dp + j] : (T)0;
+    if (threadIdx.x == 0) nrmS[c] = s >= 0 ? nrm[s] : 0.0;
+    for (int64_t m = threadIdx.x; m < Nr; m += 256) FS[c * ldf + m] = s >= 0 ? F[s * ldf + m] : (T)0;
+}
+
+// ----------------------------------------------------------------- downdate
+
+// LDS of one downdate workgroup: the GEMM stages, later the T tile, and the
+// per-wave row-norm partials.
+template <typename T> struct DowndateLds {
+    typedef typename Cfg<T>::G G;
+    static constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
+    static constexpr int TP = BN + 16 / (int)sizeof(T);  // T tile pitch
+    static constexpr int ELEMS = G::LDS_ELEMS > BM * TP ? G::LDS_ELEMS : BM * TP;
+    static constexpr int WN = BN / (G::NTN * Mfma<T>::MT);
+};
+
+// The fused Step-3 update for BM rows (ks_arpclm.cu:929-995 and
+// update_diag_block 455-478):
+//   acc  = X_R X_{S_i}^T                    (MFMA, K = dp)
+//   acc  = -K(R, S_i)                       (epilogue: norms, exp)
+//   acc += F_R F_{S_i}^T                    (MFMA, K = Nr)
+//   T    = -acc -> LDS
+//   acc  = T L_c^-T                         (MFMA, K = k)
+//   F(R, Nr:Nr+k) = acc;  d(R) = max(d(R) - rowsq(acc), 0)
+// This is row block blk (rows blk * BM ...) of one problem; it returns
+// early (before any barrier) when the problem is done or accepted nothing
+// this round.
+template <typename T>
+__device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const double* __restrict__ nrm,
+                                     T* __restrict__ F, int64_t ldf, double* __restrict__ d, int64_t n,
+                                     const T* __restrict__ XS, const double* __restrict__ nrmS,
+                                     const int64_t* __restrict__ Si, const T* __restrict__ FS,
+                                     const T* __restrict__ LinvT, const ArpcState* st, GaussScale gs,
+                                     int64_t blk, T* lds, double (*rs)[Cfg<T>::BM])
+{
+    typedef typename Cfg<T>::G G;
+    typedef DowndateLds<T> DL;
+    constexpr int BM = DL::BM, BN = DL::BN, TP = DL::TP, WN = DL::WN;
+    if (st->done) return;
+    int k = st->k;
+    if (k == 0) return;
+    int64_t Nr = st->Nr;
+    int64_t r0 = blk * BM;
+    int64_t rows = n - r0;
+    G g;
+    g.init();
+    g.set_live_cols(k);
+    // distance product
+    g.mainloop(lds, X + r0 * dp, dp, rows, XS, dp, BN, dp);
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int b = 0; b < G::NTN; ++b)
+#pragma unroll
+            for (int v = 0; v < Mfma<T>::ACC; ++v) {
+                int64_t row = r0 + g.row_of(a, v);
+                int col = g.col_of(b);
+                double kv = 0.0;
+                if (row < n && col < k) {
+                    double D = nrm[row] + nrmS[col] - 2.0 * (double)g.acc[a][b][v];
+                    D = D > 0.0 ? D : 0.0;
+                    kv = (row == Si[col]) ? 1.0 : (double)gauss_val(D, gs, (T*)nullptr);
+                }
+                g.acc[a][b][v] = (T)(-kv);
+            }
+    // overlap with the current factor
+    if (Nr > 0) g.mainloop_live(lds, F + r0 * ldf, ldf, rows, FS, ldf, BN, Nr);
+    // T = K(R,S_i) - F_R F_{S_i}^T into LDS
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int b = 0; b < G::NTN; ++b)
+#pragma unroll
+            for (int v = 0; v < Mfma<T>::ACC; ++v)
+                lds[g.row_of(a, v) * TP + g.col_of(b)] = -g.acc[a][b][v];
+    __syncthreads();
+    g.zero();
+    g.lds_global_tri_live(lds, TP, LinvT, BMAX, (k + 7) & ~7);  // L_c^-T is triangular
+    // write the new columns and the squared row norms
+    double part[G::NTM][Mfma<T>::ACC];
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int v = 0; v < Mfma<T>::ACC; ++v) part[a][v] = 0.0;
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int b = 0; b < G::NTN; ++b)
+#pragma unroll
+            for (int v = 0; v < Mfma<T>::ACC; ++v) {
+                int64_t row = r0 + g.row_of(a, v);
+                int col = g.col_of(b);
+                T gv = g.acc[a][b][v];
+                if (col < k) {
+                    if (row < n) F[row * ldf + Nr + col] = gv;
+                    part[a][v] += (double)gv * (double)gv;
+                }
+            }
+    // reduce over the MT lanes that share a row (lane % MT = column)
+    constexpr int MT = Mfma<T>::MT;
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int v = 0; v < Mfma<T>::ACC; ++v) {
+            double s = part[a][v];
+#pragma unroll
+            for (int o = 1; o < MT; o <<= 1) s += __shfl_xor(s, o, 64);
+            if (g.li == 0) rs[g.wn][g.row_of(a, v)] = s;
+        }
+    __syncthreads();
+    for (int r = threadIdx.x; r < BM; r += 256) {
+        int64_t row = r0 + r;
+        if (row < n) {
+            double sq = 0.0;
+#pragma unroll
+            for (int q = 0; q < WN; ++q) sq += rs[q][r];
+            double dd = d[row] - sq;
+            d[row] = dd >= 0.0 ? dd : 0.0;
+        }
+    }
+}
+
+// Block of BMAX threads.
+__device__ inline void commit_body(ArpcState* st, const int64_t* __restrict__ Si, int64_t* pivots,
+                                   int32_t* accepted)
+{
+    if (st->done) return;
+    const int round = st->round;
+    int k = st->k;
+    int64_t Nr = st->Nr;
+    if ((int)threadIdx.x < k) pivots[Nr + threadIdx.x] = Si[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        accepted[round] = k;
+        st->Nr = Nr + k;
+        st->round = round + 1;
+        if (st->Nr >= st->rank) {
+            st->done = 1;
+            st->stop_reason = 1;
+        }
+    }
+}
+
+__device__ inline void final_err_body(const double* psum, int nb, double trK, const ArpcState* st, double* err)
+{
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int j = 0; j < nb; ++j) s += psum[j];
+    if (st->round > 0) err[st->round - 1] = s / trK;
+}
+
+
+inline int ceil_log2(int64_t n)
+{
+    int c = 0;
+    while (((int64_t)1 << c) < n) ++c;
+    return c;
+}
+
+// ------------------------------------------------------------ host side
+
+// Device buffers of one problem's ARPC run (named context scratch).
+struct ArpcBufs {
+    double* d = nullptr;          // residual diagonal, n
+    ArpcState* st = nullptr;
+    int64_t* piv = nullptr;       // rank
+    int32_t* acc = nullptr;       // max_rounds
+    double* err = nullptr;        // max_rounds
+    int64_t* inj = nullptr;       // injected pivots or null
+    int nb = 0;                   // scan chunks
+    double *pmax = nullptr, *psum = nullptr, *H = nullptr, *nrmS = nullptr, *Linv64 = nullptr;
+    uint64_t *prefix = nullptr, *btot = nullptr;
+    int64_t *Sprime = nullptr, *Si = nullptr;
+    void *XS = nullptr, *FS = nullptr, *LinvT = nullptr;  // GEMM precision
+};
+
+// Checks the parameters and result arrays as kevd_arpc does; returns the
+// round cap.
+kevd_status arpc_validate(kevd_ctx* ctx, const kevd_arpc_params* p, const kevd_arpc_result* res,
+                          int* max_rounds);
+// Sizes the factor, allocates the run and round buffers and queues the
+// initial state (d = 1, counters zero) on the context stream.
+kevd_status arpc_prepare(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds, ArpcBufs* b);
+// After the rounds (stream synchronised): copies rank, rounds, pivots,
+// accepted counts and trace errors out and records the factor in ctx.
+kevd_status arpc_finish(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds, const ArpcBufs& b,
+                        kevd_arpc_result* res);
+
+}  // namespace arpck

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void k_rowdot(const T* __restrict__ F, int64_t
 // F^T b' -- so one iteration streams F once instead of twice.  A block
 // owns SK_ROWS rows and walks them SKR at a time; each thread keeps VPT
 // 16-byte column vectors of u and of its F^T b' partial in registers.
-// Per block: bstat = [min y, max |rs - 1|, sum rs].
+// Per block: bstat = [min y, max |rs - 1|, sum rs, min s].
 constexpr int SK_ROWS = 512;
 
 template <typename T> struct SkRows;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
                     f[rr][q] = vec_t{};
             }
     };
-    double mn = INFINITY, dv = 0.0, srs = 0.0;
+    double mn = INFINITY, dv = 0.0, srs = 0.0, smin = INFINITY;
     auto group = [&](const vec_t (&f)[R][VPT], int64_t i0, double (&rd)[4][R]) {
         double fv[R][VPT][V];
 #pragma unroll
@@ -261,6 +261,7 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
                 mn = fmin(mn, y);
                 dv = fmax(dv, fabs(rs - 1.0));
                 srs += rs;
+                smin = fmin(smin, si);
             }
 #pragma unroll
             for (int q = 0; q < VPT; ++q)
@@ -289,31 +290,35 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
             if (c < r) part[(int64_t)blockIdx.x * rpad + c] = acc[q][e];
         }
     if (threadIdx.x == 0) {
-        bstat[3 * blockIdx.x + 0] = mn;
-        bstat[3 * blockIdx.x + 1] = dv;
-        bstat[3 * blockIdx.x + 2] = srs;
+        bstat[4 * blockIdx.x + 0] = mn;
+        bstat[4 * blockIdx.x + 1] = dv;
+        bstat[4 * blockIdx.x + 2] = srs;
+        bstat[4 * blockIdx.x + 3] = smin;
     }
 }
 
-// [min, max, sum] over the nb per-block triples, in block order.
+// [min, max, sum, min] over the nb per-block quadruples, in block order.
 __global__ __launch_bounds__(256) void k_stat_reduce(const double* bstat, int nb, double* out)
 {
-    __shared__ double smn[256], sdv[256], ssm[256];
-    double mn = INFINITY, dv = 0.0, sm = 0.0;
+    __shared__ double smn[256], sdv[256], ssm[256], ssn[256];
+    double mn = INFINITY, dv = 0.0, sm = 0.0, sn = INFINITY;
     for (int j = threadIdx.x; j < nb; j += 256) {
-        mn = fmin(mn, bstat[3 * j]);
-        dv = fmax(dv, bstat[3 * j + 1]);
-        sm += bstat[3 * j + 2];
+        mn = fmin(mn, bstat[4 * j]);
+        dv = fmax(dv, bstat[4 * j + 1]);
+        sm += bstat[4 * j + 2];
+        sn = fmin(sn, bstat[4 * j + 3]);
     }
     smn[threadIdx.x] = mn;
     sdv[threadIdx.x] = dv;
     ssm[threadIdx.x] = sm;
+    ssn[threadIdx.x] = sn;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
             smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + o]);
             sdv[threadIdx.x] = fmax(sdv[threadIdx.x], sdv[threadIdx.x + o]);
             ssm[threadIdx.x] += ssm[threadIdx.x + o];
+            ssn[threadIdx.x] = fmin(ssn[threadIdx.x], ssn[threadIdx.x + o]);
         }
         __syncthreads();
     }
@@ -321,7 +326,24 @@ __global__ __launch_bounds__(256) void k_stat_reduce(const double* bstat, int nb
         out[0] = smn[0];
         out[1] = sdv[0];
         out[2] = ssm[0];
+        out[3] = ssn[0];
     }
+}
+
+// Per-block minimum of x (grid-stride), for a second-level k_block_minmax.
+__global__ __launch_bounds__(256) void k_min_partial(const double* x, int64_t n, double* part)
+{
+    __shared__ double sm[256];
+    double m = INFINITY;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        m = fmin(m, x[i]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
 __global__ void k_scale_d(double* p, int64_t n, double a)
@@ -1057,7 +1079,7 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
     double one = 1.0, zero = 0.0;
     std::vector<double> hres((size_t)k), hth((size_t)l);
     std::vector<int> hinfo(1 + 3 * max_orth);
-    const int maxit = 12, first_check = 4;
+    const int maxit = 16, first_check = 4;
     for (int householder = 0; householder < 2; ++householder) {
         KEVD_HIP(ctx, hipMemsetAsync(info, 0, (1 + 3 * max_orth) * sizeof(int), ctx->stream));
         k_randn<<<(unsigned)((r * l + 255) / 256), 256, 0, ctx->stream>>>(Q, r * l, 0x6b657664ull);
@@ -1065,11 +1087,11 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
         int north = 0;
         KEVD_TRY(orth_basis(ctx, Q, r, l, householder, G, tau, info + 1 + 3 * north++));
         bool conv = false, broken = false;
-        int it = 0;
+        int it = 0, next_check = first_check;
         for (it = 1; it <= maxit; ++it) {
             BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, Lb, R, &one, W, R, Q, R,
                                &zero, Y, R));
-            if (it < first_check) {
+            if (it < next_check && north < max_orth) {
                 KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++));
                 std::swap(Q, Y);
                 continue;
@@ -1095,12 +1117,22 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
                 return kevd_fail(ctx, KEVD_EEIG, "Rayleigh-Ritz eigensolve failed, info " + std::to_string(hinfo[0]));
             double thmax = std::fabs(hth[l - 1]);
             conv = true;
-            for (int c = 0; c < k && conv; ++c) {
+            // Steps until the next check: the residual of Ritz pair c shrinks
+            // by about rho_c = theta_min / theta_c per step (theta_min, the
+            // smallest Ritz value of the block, stands in for lambda_{l+1}).
+            int need = 1;
+            for (int c = 0; c < k; ++c) {
                 double t = hth[l - k + c];
                 double res = std::sqrt(std::max(hres[c], 0.0));
-                if (res > 1e-6 * std::fabs(t) + 1e-14 * thmax) conv = false;
+                double tolc = 1e-6 * std::fabs(t) + 1e-14 * thmax;
+                if (res <= tolc) continue;
+                conv = false;
+                double rho = std::fabs(hth[0]) / std::max(std::fabs(t), 1e-300);
+                int m = rho > 0.0 && rho < 0.9 ? (int)std::ceil(std::log(tolc / res) / std::log(rho)) : 1;
+                need = std::max(need, std::min(m, 6));
             }
             if (conv || it == maxit) break;
+            next_check = it + need;
             KEVD_HIP(ctx, hipMemsetAsync(info, 0, (1 + 3 * max_orth) * sizeof(int), ctx->stream));
             north = 0;
             KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++));
@@ -1245,7 +1277,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     mean /= (double)n;
     if (!(mean > 0.0)) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
     int it = 0;
-    double dev = INFINITY;
+    double dev = INFINITY, smin_fused = NAN;
     int maxit = p->sinkhorn_max_iter > 0 ? p->sinkhorn_max_iter : 1000;
     double tol = p->sinkhorn_tol > 0.0 ? p->sinkhorn_tol : 1e-12;
     constexpr int V = 16 / (int)sizeof(T), VPT_MAX = sizeof(T) == 4 ? 4 : 8;
@@ -1262,8 +1294,8 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
         int64_t rpad = KEVD_ROUND_UP(r, 4);
         double *part, *bstat, *st3;
         KEVD_TRY(ctx_buf_t(ctx, "e_skpart", (size_t)nb * rpad, &part));
-        KEVD_TRY(ctx_buf_t(ctx, "e_skstat", (size_t)3 * nb, &bstat));
-        KEVD_TRY(ctx_buf_t(ctx, "e_skst3", 3, &st3));
+        KEVD_TRY(ctx_buf_t(ctx, "e_skstat", (size_t)4 * nb, &bstat));
+        KEVD_TRY(ctx_buf_t(ctx, "e_skst3", 4, &st3));
         double* u2;
         KEVD_TRY(ctx_buf_t(ctx, "e_u2", r, &u2));
         // s_0 = c0 * 1 and u_0 = F^T s_0 = c0 * (F^T 1), which u holds
@@ -1271,7 +1303,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
         k_fill_d<<<gn, 256, 0, ctx->stream>>>(s, n, 1.0);
         bool damped = false;
         int rises = 0;
-        double prev = INFINITY, st[3];
+        double prev = INFINITY, st[4];
         for (it = 1; it <= maxit; ++it) {
             {
                 ProfScope ps(ctx, TC_MATVEC);
@@ -1291,7 +1323,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
                 k_stat_reduce<<<1, 256, 0, ctx->stream>>>(bstat, nb, st3);
             }
             KEVD_HIP(ctx, hipGetLastError());
-            KEVD_HIP(ctx, hipMemcpyAsync(st, st3, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KEVD_HIP(ctx, hipMemcpyAsync(st, st3, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
             KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
             if (!(st[0] > 0.0)) {
                 // the first pass sees K^ 1 up to a positive constant: a degree
@@ -1299,6 +1331,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
                 return kevd_fail(ctx, KEVD_ENONPOS_Q, "Sinkhorn: non-positive K^ s");
             }
             dev = st[1];
+            smin_fused = st[3];
             if (dev <= tol || it == maxit) break;  // s = bscale * b is the iterate measured
             bool was_damped = damped;
             rises = dev > prev ? rises + 1 : 0;
@@ -1338,13 +1371,22 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     k_inv_sqrt_tail<<<(k + 255) / 256, 256, 0, ctx->stream>>>(lam, r, k, isq);
     KEVD_TRY(ugemm<T>(ctx, Vk, k, isq, nullptr, s, ctx->U, nullptr));
     KEVD_HIP(ctx, hipMemcpyAsync(res->lam, lam, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    double* smm;
-    KEVD_TRY(ctx_buf_t(ctx, "e_smm", 2, &smm));
-    k_block_minmax<<<1, 256, 0, ctx->stream>>>(s, nullptr, (int)n, smm);
-    KEVD_HIP(ctx, hipGetLastError());
-    KEVD_HIP(ctx, hipMemcpyAsync(mm, smm, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    res->min_degree = mm[0];
+    if (std::isnan(smin_fused)) {
+        // two-pass path: min(s) by a block-partial reduction
+        int nbm = (int)std::min<int64_t>((n + 255) / 256, 4096);
+        double *pm, *smm;
+        KEVD_TRY(ctx_buf_t(ctx, "e_spm", nbm, &pm));
+        KEVD_TRY(ctx_buf_t(ctx, "e_smm", 2, &smm));
+        k_min_partial<<<nbm, 256, 0, ctx->stream>>>(s, n, pm);
+        k_block_minmax<<<1, 256, 0, ctx->stream>>>(pm, nullptr, nbm, smm);
+        KEVD_HIP(ctx, hipGetLastError());
+        KEVD_HIP(ctx, hipMemcpyAsync(mm, smm, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        smin_fused = mm[0];
+    } else {
+        KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    res->min_degree = smin_fused;
     return KEVD_OK;
 }
 
