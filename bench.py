@@ -120,6 +120,86 @@ def rel_frobenius(Fa, Fb, chunk=1 << 16):
     return float(np.sqrt(num / den))
 
 
+ENS_SHAPES = [(22.0, 32), (44.0, 64), (88.0, 128)]  # (domain length, grid points)
+
+
+def ensemble_leg(nprob=48, kout=32, seed=1, workers=8, candidates=(32, 64, 128), norm="sinkhorn"):
+    """Uneven-size ensemble: nprob KS problems, N ~ U[4096, 131072],
+    (L, M) from ENS_SHAPES, rank ~ U[256, 1024] (all drawn with seed 1),
+    each with its own median eps and an autotuned block size.  Timed: the
+    batched mixed ARPC (graph-replayed rounds, persistent downdate) plus the
+    concurrent kEVDs, against the same problems run one after another
+    through the single-problem calls.  Per-problem top-kout eigenvalue error
+    against each problem's all-FP64 run."""
+    from concurrent.futures import ThreadPoolExecutor
+    from kernel_evd_amd import Context, ensemble_arpc, ensemble_evd, ks_generate
+    rng = np.random.default_rng(seed)
+    Ns = rng.integers(4096, 131072 + 1, nprob)
+    shp = rng.integers(0, len(ENS_SHAPES), nprob)
+    ranks = rng.integers(256, 1024 + 1, nprob)
+    t = time.time()
+    with ThreadPoolExecutor(16) as ex:
+        data = list(ex.map(lambda i: ks_generate(int(Ns[i]), L_dom=ENS_SHAPES[shp[i]][0], M=ENS_SHAPES[shp[i]][1],
+                                                 dt=0.25, n_discard=10000, stride=4, seed=1000 + i)
+                           .astype(np.float32), range(nprob)))
+    log("ensemble: generated %d KS data sets (%d points) in %.1f s" % (nprob, int(Ns.sum()), time.time() - t))
+    ctxs = [Context(0) for _ in range(nprob)]
+    eps, blocks = [], []
+    t = time.time()
+    for i, c in enumerate(ctxs):
+        c.set_points(data[i])
+        eps.append(c.median_sqdist(seed=i, nsample=1 << 20))
+        blocks.append(c.autotune_block(eps[i], int(ranks[i]), candidates=candidates, seed=i))
+    t_tune = time.time() - t
+    ranks = [int(r) for r in ranks]
+    seeds = list(range(nprob))
+    # batched: warm once, then time
+    ensemble_arpc(ctxs, eps, ranks, blocks, seeds=seeds, prec="mixed")
+    ensemble_evd(ctxs, ranks, k_out=kout, norm=norm, prec="mixed", workers=workers)
+    t0 = time.perf_counter()
+    outs, graph = ensemble_arpc(ctxs, eps, ranks, blocks, seeds=seeds, prec="mixed")
+    t1 = time.perf_counter()
+    evs = ensemble_evd(ctxs, [o.rank for o in outs], k_out=kout, norm=norm, prec="mixed", workers=workers)
+    t2 = time.perf_counter()
+    # the same problems one after another through the single-problem calls
+    t3 = time.perf_counter()
+    for i, c in enumerate(ctxs):
+        a = c.arpc(eps[i], ranks[i], block=blocks[i], seed=seeds[i], prec="mixed")
+        run_evd(c, a.rank, kout, "mixed", norm)
+    t4 = time.perf_counter()
+    # FP64 reference per problem (batched FP64 ARPC, concurrent FP64 kEVD)
+    o64, _ = ensemble_arpc(ctxs, eps, ranks, blocks, seeds=seeds, prec="fp64")
+    e64 = ensemble_evd(ctxs, [o.rank for o in o64], k_out=kout, norm=norm, prec="fp64", workers=workers)
+    errs, errs_same = [], []
+    for i in range(nprob):
+        a, b = evs[i].lam[::-1][:kout], e64[i].lam[::-1][:kout]
+        errs.append(float(np.max(np.abs(a - b) / np.abs(b))))
+    # FP64 on the mixed run's own pivots: the precision effect alone (the
+    # errors above also hold the difference between two pivot sequences)
+    for i, c in enumerate(ctxs):
+        c.arpc(eps[i], outs[i].rank, block=blocks[i], seed=seeds[i], prec="fp64", inject_pivots=outs[i].pivots)
+        e = run_evd(c, outs[i].rank, kout, "fp64", norm)
+        a, b = evs[i].lam[::-1][:kout], e.lam[::-1][:kout]
+        errs_same.append(float(np.max(np.abs(a - b) / np.abs(b))))
+    cols = int(sum(o.rank for o in outs))
+    for c in ctxs:
+        c.close()
+    res = {"problems": nprob, "points_total": int(Ns.sum()), "factor_columns": cols,
+           "arpc_s": round(t1 - t0, 4), "evd_s": round(t2 - t1, 4), "total_s": round(t2 - t0, 4),
+           "factor_columns_per_s": round(cols / (t1 - t0), 1),
+           "columns_per_s_incl_evd": round(cols / (t2 - t0), 1),
+           "sequential_total_s": round(t4 - t3, 4), "batched_speedup": round((t4 - t3) / (t2 - t0), 3),
+           "graph_replay": bool(graph), "evd_workers": workers, "k_out": kout, "norm": norm,
+           "blocks_chosen": {str(b): int(blocks.count(b)) for b in sorted(set(blocks))},
+           "autotune_s": round(t_tune, 2),
+           "eig_rel_err_vs_fp64_max": max(errs), "eig_rel_err_vs_fp64_median": float(np.median(errs)),
+           "eig_rel_err_vs_fp64_same_pivots_max": max(errs_same),
+           "eig_rel_err_vs_fp64_same_pivots_median": float(np.median(errs_same))}
+    log("ensemble: %d problems, %d columns: ARPC %.3f s + EVD %.3f s (sequential %.3f s); max eig err %.2e"
+        % (nprob, cols, t1 - t0, t2 - t1, t4 - t3, max(errs)))
+    return res
+
+
 def cpu_baseline(X, eps, rank, block, kout, n_sample, norm):
     """CPU FP64 oracle (OpenMP C ARPC + NumPy F-form kEVD) on the first
     n_sample points; both stages scale linearly in N (the Sinkhorn iteration
@@ -156,6 +236,8 @@ def main():
                     help="skip the subspace angles, the other normalisation and the factor difference")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble leg")
+    ap.add_argument("--ensemble-only", action="store_true", help="run only the ensemble leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one cold FP32 copy of MIB MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -168,6 +250,9 @@ def main():
 
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_leg(norm=args.norm)}), flush=True)
+        return
     X = ks_data(args.n)
     X32 = X.astype(np.float32)
     ctx = Context(0)
@@ -311,6 +396,11 @@ def main():
             log("factor rel Frobenius (injected pivots) %.3e" % out["factor_rel_frobenius_vs_fp64_injected"])
         ctx64.close()
 
+    if not args.no_ensemble:
+        ctx.close()
+        ctx = None
+        out["ensemble"] = ensemble_leg(norm=norm)
+
     if not args.no_cpu:
         ns = min(args.cpu_sample, args.n)
         ta, te, nt, r = cpu_baseline(X32, eps, args.rank, args.block, args.kout, ns, norm)
@@ -321,7 +411,8 @@ def main():
                                          "+ kEVD %.2f s measured, scaled x%.0f (both stages linear in N)"
                                          % (ns, args.n, ta, te, scale)}
         log("cpu baseline: %.2f s measured on %d points (%d threads)" % (ta + te, ns, nt))
-    ctx.close()
+    if ctx is not None:
+        ctx.close()
     print(json.dumps(out), flush=True)
 
 

@@ -1038,9 +1038,10 @@ kevd_status orth_basis(kevd_ctx* ctx, double* Y, int64_t r, int l, bool househol
 // Leading k eigenpairs of the symmetric r x r FP64 matrix W (column-major).
 // lam (device, r entries) gets eigenvalues ascending, *Vk an r x k block of
 // eigenvectors in the same order (the last column is the leading one).
-// all = true: the full divide-and-conquer eigensolve (every eigenvalue,
-// as the reference's syevd).  Otherwise block subspace iteration on
-// l >= k + 64 columns: Q <- orth(W Q), each step reducing the unwanted
+// all = true (or r < 2 l): the full divide-and-conquer eigensolve (every
+// eigenvalue, as the reference's syevd).  Otherwise block subspace
+// iteration on l = max(k + 64, min(4k, 256)) columns (rounded up to 32):
+// Q <- orth(W Q), each step reducing the unwanted
 // components of eigenvector i by lambda_{l+1} / lambda_i.  Rayleigh-Ritz
 // (H = Q^T W Q = S diag(theta) S^T) runs after the first few steps and
 // then after every step, until the leading k residuals
@@ -1050,7 +1051,8 @@ kevd_status orth_basis(kevd_ctx* ctx, double* Y, int64_t r, int l, bool househol
 kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, double* lam,
                         double** Vk, int* iters)
 {
-    int l = (int)KEVD_ROUND_UP(std::max(k + 64, 256), 32);
+    // block: k + 64 columns at least, 4k (at most 256) when that is larger
+    int l = (int)KEVD_ROUND_UP(std::max(k + 64, std::min(4 * k, 256)), 32);
     *iters = 0;
     if (all || r < 2 * (int64_t)l) {
         KEVD_TRY(eigh(ctx, W, r, lam));
