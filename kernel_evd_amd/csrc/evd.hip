@@ -427,8 +427,11 @@ struct GramTile {
     typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type pair_t;
 
     typename M::acc_t acc[NG][TM][TN];
-    uint4 ri[NVI], rj[NVJ];
-    T wv;                                                   // this thread's staged weight
+    // one chunk's global loads held in registers until stored to LDS
+    struct Stage {
+        uint4 ri[NVI], rj[NVJ];
+        T wv;  // this thread's staged weight
+    };
     int wm, wn, h, li;
 
     __device__ inline void zero()
@@ -453,7 +456,7 @@ struct GramTile {
         }
     }
 
-    __device__ inline void gload(const T* __restrict__ F, int64_t ldf, int64_t r, int64_t ci,
+    __device__ inline void gload(Stage& sg, const T* __restrict__ F, int64_t ldf, int64_t r, int64_t ci,
                                  int64_t cj, int64_t k0, int64_t k_end,
                                  const T* __restrict__ w1, const T* __restrict__ w2)
     {
@@ -462,10 +465,10 @@ struct GramTile {
             int lin = threadIdx.x + 256 * j;
             int kr = lin / VI, cv = lin % VI;
             int64_t row = k0 + kr, col = ci + cv * EPV;
-            ri[j] = make_uint4(0, 0, 0, 0);
+            sg.ri[j] = make_uint4(0, 0, 0, 0);
             if (lin < KC * VI && row < k_end && col < r) {
-                ri[j] = *reinterpret_cast<const uint4*>(F + row * ldf + col);
-                mask(ri[j], col, r);
+                sg.ri[j] = *reinterpret_cast<const uint4*>(F + row * ldf + col);
+                mask(sg.ri[j], col, r);
             }
         }
 #pragma unroll
@@ -473,29 +476,29 @@ struct GramTile {
             int lin = threadIdx.x + 256 * j;
             int kr = lin / VJ, cv = lin % VJ;
             int64_t row = k0 + kr, col = cj + cv * EPV;
-            rj[j] = make_uint4(0, 0, 0, 0);
+            sg.rj[j] = make_uint4(0, 0, 0, 0);
             if (lin < KC * VJ && row < k_end && col < r) {
-                rj[j] = *reinterpret_cast<const uint4*>(F + row * ldf + col);
-                mask(rj[j], col, r);
+                sg.rj[j] = *reinterpret_cast<const uint4*>(F + row * ldf + col);
+                mask(sg.rj[j], col, r);
             }
         }
         // threads [0, NG*KC) stage the chunk's row weights
-        wv = (T)0;
+        sg.wv = (T)0;
         if ((int)threadIdx.x < NG * KC) {
             int g = threadIdx.x / KC, kr = threadIdx.x % KC;
             int64_t row = k0 + kr;
-            if (row < k_end) wv = g == 0 ? w1[row] : w2[row];
+            if (row < k_end) sg.wv = g == 0 ? w1[row] : w2[row];
         }
     }
 
-    __device__ inline void sstore(T* st)
+    __device__ inline void sstore(const Stage& sg, T* st)
     {
 #pragma unroll
         for (int j = 0; j < NVI; ++j) {
             int lin = threadIdx.x + 256 * j;
             if (lin < KC * VI) {
                 int kr = lin / VI, cv = lin % VI;
-                *reinterpret_cast<uint4*>(st + kr * PI + cv * EPV) = ri[j];
+                *reinterpret_cast<uint4*>(st + kr * PI + cv * EPV) = sg.ri[j];
             }
         }
 #pragma unroll
@@ -503,41 +506,59 @@ struct GramTile {
             int lin = threadIdx.x + 256 * j;
             if (lin < KC * VJ) {
                 int kr = lin / VJ, cv = lin % VJ;
-                *reinterpret_cast<uint4*>(st + KC * PI + kr * PJ + cv * EPV) = rj[j];
+                *reinterpret_cast<uint4*>(st + KC * PI + kr * PJ + cv * EPV) = sg.rj[j];
             }
         }
-        if ((int)threadIdx.x < NG * KC) st[WOFF + threadIdx.x] = wv;
+        if ((int)threadIdx.x < NG * KC) st[WOFF + threadIdx.x] = sg.wv;
     }
 
+    // The fragments of K step s + 1 are read from LDS while step s
+    // multiplies, and all weighted A operands of a step are formed before
+    // its MFMAs, so no VALU write waits on an MFMA still reading a register.
     __device__ inline void compute(const T* cur)
     {
-#pragma unroll
-        for (int s = 0; s < KC / M::NH; ++s) {
+        constexpr int NS = KC / M::NH;
+        pair_t araw[2][TM / IL], b[2][TN / IL];
+        T wk[2][NG];
+        auto frag = [&](int q, int s) {
             int kr = s * M::NH + h;
-            pair_t araw[TM / IL], b[TN / IL];
-            T wk[NG];
 #pragma unroll
-            for (int g = 0; g < NG; ++g) wk[g] = cur[WOFF + g * KC + kr];
+            for (int g = 0; g < NG; ++g) wk[q][g] = cur[WOFF + g * KC + kr];
 #pragma unroll
             for (int x = 0; x < TM / IL; ++x)
-                araw[x] = *reinterpret_cast<const pair_t*>(cur + kr * PI + wm * WR + x * M::MT * IL + IL * li);
+                araw[q][x] = *reinterpret_cast<const pair_t*>(cur + kr * PI + wm * WR + x * M::MT * IL + IL * li);
 #pragma unroll
             for (int y = 0; y < TN / IL; ++y)
-                b[y] = *reinterpret_cast<const pair_t*>(
+                b[q][y] = *reinterpret_cast<const pair_t*>(
                     cur + KC * PI + kr * PJ + wn * WC + y * M::MT * IL + IL * li);
+        };
+        frag(0, 0);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int q = s & 1;
+            if (s + 1 < NS) frag(q ^ 1, s + 1);
+            T a[NG][TM];
 #pragma unroll
             for (int g = 0; g < NG; ++g)
 #pragma unroll
                 for (int x = 0; x < TM / IL; ++x) {
-                    T a0 = araw[x].x * wk[g], a1 = araw[x].y * wk[g];
+                    a[g][IL * x] = araw[q][x].x * wk[q][g];
+                    a[g][IL * x + 1] = araw[q][x].y * wk[q][g];
+                }
+            // keep the next step's LDS reads and this step's operand scaling
+            // ahead of its MFMAs (the scheduler would otherwise sink the
+            // reads next to their uses and serialise them)
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+#pragma unroll
+                for (int x = 0; x < TM; ++x)
 #pragma unroll
                     for (int y = 0; y < TN / IL; ++y) {
-                        M::mma(acc[g][IL * x][IL * y], a0, b[y].x);
-                        M::mma(acc[g][IL * x][IL * y + 1], a0, b[y].y);
-                        M::mma(acc[g][IL * x + 1][IL * y], a1, b[y].x);
-                        M::mma(acc[g][IL * x + 1][IL * y + 1], a1, b[y].y);
+                        M::mma(acc[g][x][IL * y], a[g][x], b[q][y].x);
+                        M::mma(acc[g][x][IL * y + 1], a[g][x], b[q][y].y);
                     }
-                }
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 
@@ -619,17 +640,8 @@ void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T*
     g.zero();
     bool first = true;
     int64_t nk = k_end > k_beg ? (k_end - k_beg + G::KC - 1) / G::KC : 0;
-    if (nk > 0) {
-        g.gload(F, ldf, r, ci, cj, k_beg, k_end, w1, w2);
-        g.sstore(lds);
-        __syncthreads();
-    }
     int since = 0;
-    for (int64_t kc = 0; kc < nk; ++kc) {
-        T* cur = lds + (kc & 1) * G::STAGE;
-        T* nxt = lds + ((kc + 1) & 1) * G::STAGE;
-        bool more = kc + 1 < nk;
-        if (more) g.gload(F, ldf, r, ci, cj, k_beg + (kc + 1) * G::KC, k_end, w1, w2);
+    auto step = [&](const T* cur) {
         g.compute(cur);
         since += G::KC;
         if (since >= GramCfg<T>::PROMOTE) {
@@ -637,8 +649,41 @@ void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T*
             first = false;
             since = 0;
         }
-        if (more) g.sstore(nxt);
+    };
+    typename G::Stage s0;
+    if (nk > 0) {
+        g.gload(s0, F, ldf, r, ci, cj, k_beg, k_end, w1, w2);
+        g.sstore(s0, lds);
+    }
+    if constexpr (NG == 1) {
+        // Loads run two chunks ahead (the single Gram has the registers):
+        // chunk c multiplies from stage c & 1 while c + 1 waits in one
+        // register set and c + 2 loads into the other.
+        typename G::Stage s1;
+        if (nk > 1) g.gload(s1, F, ldf, r, ci, cj, k_beg + G::KC, k_end, w1, w2);
         __syncthreads();
+        for (int64_t c = 0; c < nk; c += 2) {
+            if (c + 2 < nk) g.gload(s0, F, ldf, r, ci, cj, k_beg + (c + 2) * G::KC, k_end, w1, w2);
+            step(lds);
+            if (c + 1 < nk) g.sstore(s1, lds + G::STAGE);
+            __syncthreads();
+            if (c + 1 >= nk) break;
+            if (c + 3 < nk) g.gload(s1, F, ldf, r, ci, cj, k_beg + (c + 3) * G::KC, k_end, w1, w2);
+            step(lds + G::STAGE);
+            if (c + 2 < nk) g.sstore(s0, lds);
+            __syncthreads();
+        }
+    } else {
+        __syncthreads();
+        for (int64_t kc = 0; kc < nk; ++kc) {
+            T* cur = lds + (kc & 1) * G::STAGE;
+            T* nxt = lds + ((kc + 1) & 1) * G::STAGE;
+            bool more = kc + 1 < nk;
+            if (more) g.gload(s0, F, ldf, r, ci, cj, k_beg + (kc + 1) * G::KC, k_end, w1, w2);
+            step(cur);
+            if (more) g.sstore(s0, nxt);
+            __syncthreads();
+        }
     }
     if (since > 0 || first) g.promote(out, first);
 }
