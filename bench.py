@@ -12,8 +12,10 @@ resident in HBM before the timed region.
 After the timed steps (untimed): one all-FP64 step for the speed ratio, the
 eigenvalue errors and the leading-eigenvector subspace angles; the EVD in
 the other normalisation on both factors; the mixed ARPC with the FP64
-pivots injected, for the factor's relative Frobenius difference; and the
-CPU FP64 oracle on a bounded sample for the CPU baseline.
+pivots injected, for the factor's relative Frobenius difference; the
+uneven-size ensemble (48 KS problems, batched ARPC + concurrent kEVD, against
+the same problems run one after another); and the CPU FP64 oracle on a
+bounded sample for the CPU baseline.
 
 Prints ONE JSON line.  Single GPU only.
 """
@@ -29,6 +31,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+# Measured HBM bytes per launch at this workload (rocprofv3 FETCH_SIZE x 2.0
+# + WRITE_SIZE x 1.0, the factors a cold 2 GiB copy gives), from
+# profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md.
+TRAFFIC_BYTES = {"k_downdate<float>": 4.620e9, "k_gram<float,1>": 102.23e9,
+                 "k_sinkhorn_pass<float,2>": 8.511e9}
+TRAFFIC_SOURCE = "profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md"
 FP32_MFMA_PEAK_TF = 157.3
 FP64_MFMA_PEAK_TF = 78.6
 
@@ -321,7 +330,9 @@ def main():
                      "achieved": round(dd_flop / (dd_ms * 1e-3) / 1e12, 3) if dd_ms > 0 else None,
                      "peak": FP32_MFMA_PEAK_TF, "unit": "TFLOP/s",
                      "frac": round(dd_flop / (dd_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4) if dd_ms > 0 else None,
-                     "traffic": None,
+                     "traffic": TRAFFIC_BYTES["k_downdate<float>"] if args.n == 1 << 20 and args.rank == 2000 else None,
+                     "traffic_unit": "bytes per launch (average over the step's launches)",
+                     "traffic_source": TRAFFIC_SOURCE,
                      "flop_per_step": dd_flop, "launches_per_step": dd_launch,
                      "algorithmic_bytes_per_launch": downdate_bytes(a_mix.accepted, args.n, 64) / max(dd_launch, 1),
                      "avg_launch_ms": round(dd_ms / max(dd_launch, 1), 4)},
@@ -335,13 +346,17 @@ def main():
         mv_bytes = float(args.n) * args.rank * 4 * (1 + max(1, e_mix.sinkhorn_iters))
         mv_ms = agg["matvec_ms"] / steps
         if mv_ms > 0:
-            out["matvec_roofline"] = {"bound": "hbm", "achieved": round(mv_bytes / (mv_ms * 1e-3) / 1e9, 1),
+            out["matvec_roofline"] = {"bound": "hbm", "kernel": "k_sinkhorn_pass<float,2> + k_colsum",
+                                      "achieved": round(mv_bytes / (mv_ms * 1e-3) / 1e9, 1),
                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                      "frac": round(mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                                      "frac": round(mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                      "traffic_per_pass": TRAFFIC_BYTES["k_sinkhorn_pass<float,2>"]
+                                      if args.n == 1 << 20 and args.rank == 2000 else None}
 
     extras = not args.no_extras
     if extras:
         U_mix = ctx.get_u()
+        run_evd(ctx, a_mix.rank, args.kout, "mixed", other)  # warm (buffers of the other normalisation)
         t0 = time.perf_counter()
         e_other = run_evd(ctx, a_mix.rank, args.kout, "mixed", other)
         out["other_norm"] = {"norm": other, "evd_ms": round(1e3 * (time.perf_counter() - t0), 3)}
@@ -381,6 +396,7 @@ def main():
             out["eigvec_max_angle_top32_rad"] = max_principal_angle(U_mix[:, -k32u:], U64[:, -k32u:])
             out["eigvec_max_angle_top%d_rad" % kk] = max_principal_angle(U_mix[:, -kk:], U64[:, -kk:])
             del U_mix, U64
+            run_evd(ctx64, a64.rank, args.kout, "fp64", other)
             t0 = time.perf_counter()
             e64o = run_evd(ctx64, a64.rank, args.kout, "fp64", other)
             out["other_norm"]["evd_ms_fp64"] = round(1e3 * (time.perf_counter() - t0), 3)
