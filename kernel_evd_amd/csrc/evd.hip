@@ -1047,6 +1047,126 @@ __global__ __launch_bounds__(256) void k_chol_shift(double* G, int l, double c)
     for (int i = threadIdx.x; i < l; i += 256) G[i + (int64_t)i * l] += s;
 }
 
+// Upper Cholesky factor of the symmetric positive definite l x l matrix A
+// (column-major, upper triangle read and overwritten with R, A = R^T R), in
+// one workgroup, for l <= CHOL_LMAX.  Right-looking over 32-row panels:
+//   the 32 x 32 diagonal block is factored by one wave in registers (lane c
+//   owns column c; rows of R travel by shuffles, no barriers);
+//   R12 = R11^-T A12 by forward substitution, one thread per trailing
+//   column, also kept in LDS;
+//   A22 -= R12^T R12 on the upper triangle, read-modify-write through L2.
+// *info = 0, or the 1-based index of the first non-positive pivot (the
+// factorisation stops there), as LAPACK potrf.
+constexpr int CHOL_LMAX = 512;
+__global__ __launch_bounds__(1024) void k_potrf_upper(double* A, int l, int* info)
+{
+    __shared__ double D[32][33];
+    __shared__ double P[32][CHOL_LMAX - 32];
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int j0 = 0; j0 < l; j0 += 32) {
+        const int nb = l - j0 < 32 ? l - j0 : 32, j1 = j0 + nb, nt = l - j1;
+        for (int e = tid; e < nb * nb; e += 1024) {
+            int r = e % nb, c = e / nb;
+            D[r][c] = A[(j0 + r) + (int64_t)(j0 + c) * l];
+        }
+        __syncthreads();
+        if (tid < 32) {
+            const int c = tid;
+            double col[32];
+#pragma unroll
+            for (int r = 0; r < 32; ++r) col[r] = (r < nb && c < nb) ? D[r][c] : 0.0;
+            int fail = 0;
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                if (k < nb && !fail) {
+                    double akk = __shfl(col[k], k, 32);
+                    if (!(akk > 0.0)) {
+                        fail = j0 + k + 1;
+                    } else {
+                        double rkk = sqrt(akk);
+                        double rkc = c == k ? rkk : (c > k ? col[k] / rkk : 0.0);
+                        col[k] = rkc;
+#pragma unroll
+                        for (int i = k + 1; i < 32; ++i) {
+                            double rki = __shfl(rkc, i, 32);
+                            if (c >= i) col[i] -= rki * rkc;
+                        }
+                    }
+                }
+            }
+            if (c == 0 && fail) bad = fail;
+#pragma unroll
+            for (int r = 0; r < 32; ++r)
+                if (r <= c && c < nb) D[r][c] = col[r];
+        }
+        __syncthreads();
+        if (bad) break;
+        for (int e = tid; e < nb * nb; e += 1024) {
+            int r = e % nb, c = e / nb;
+            if (r <= c) A[(j0 + r) + (int64_t)(j0 + c) * l] = D[r][c];
+        }
+        // R12 = R11^-T A12: column c of the panel solves R11^T x = a
+        if (tid < nt) {
+            const int c = j1 + tid;
+            double x[32];
+#pragma unroll
+            for (int k = 0; k < 32; ++k) x[k] = k < nb ? A[(j0 + k) + (int64_t)c * l] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                if (k < nb) {
+                    double v = x[k];
+#pragma unroll
+                    for (int m = 0; m < 32; ++m)
+                        if (m < k) v -= D[m][k] * x[m];
+                    x[k] = v / D[k][k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+                if (k < nb) {
+                    A[(j0 + k) + (int64_t)c * l] = x[k];
+                    P[k][tid] = x[k];
+                }
+        }
+        __syncthreads();
+        // A22 -= R12^T R12 on the upper triangle, in 4 x 4 register tiles
+        // (tile rows ti <= tile columns tc): 8 LDS reads per 16 FMAs
+        const int ntt = (nt + 3) / 4;
+        for (int e = tid; e < ntt * ntt; e += 1024) {
+            int ti = e % ntt, tc = e / ntt;
+            if (ti > tc) continue;
+            double acc[4][4] = {};
+            for (int k = 0; k < nb; ++k) {
+                double pi[4], pc[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    pi[q] = 4 * ti + q < nt ? P[k][4 * ti + q] : 0.0;
+                    pc[q] = 4 * tc + q < nt ? P[k][4 * tc + q] : 0.0;
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc[a][b] += pi[a] * pc[b];
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                int cc = 4 * tc + b;
+                if (cc >= nt) break;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    int ii = 4 * ti + a;
+                    if (ii < nt && ii <= cc) A[(j1 + ii) + (int64_t)(j1 + cc) * l] -= acc[a][b];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *info = bad;
+}
+
 // Orthonormal basis of the columns of the r x l block Y, in place.
 // Householder: rocSOLVER geqrf + orgqr.  Otherwise shifted Cholesky QR 3
 // (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput.
@@ -1057,7 +1177,7 @@ __global__ __launch_bounds__(256) void k_chol_shift(double* G, int l, double c)
 // potrf statuses go to info[0..2]; a nonzero one means the basis is not
 // to be trusted and the caller falls back to Householder.
 kevd_status orth_basis(kevd_ctx* ctx, double* Y, int64_t r, int l, bool householder, double* G,
-                       double* tau, int* info)
+                       double* tau, int* info, int passes = 3)
 {
     rocblas_handle h = blas_handle(ctx);
     const rocblas_int R = (rocblas_int)r, Lb = (rocblas_int)l;
@@ -1068,11 +1188,14 @@ kevd_status orth_basis(kevd_ctx* ctx, double* Y, int64_t r, int l, bool househol
     }
     double one = 1.0, zero = 0.0;
     const double c = 11.0 * ((double)r * l + (double)l * (l + 1)) * 0x1.0p-53;
-    for (int pass = 0; pass < 3; ++pass) {
+    for (int pass = 0; pass < passes; ++pass) {
         BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Y, R, Y,
                            R, &zero, G, Lb));
         if (pass == 0) k_chol_shift<<<1, 256, 0, ctx->stream>>>(G, l, c);
-        BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, Lb, G, Lb, info + pass));
+        if (l <= CHOL_LMAX)
+            k_potrf_upper<<<1, 1024, 0, ctx->stream>>>(G, l, info + pass);
+        else
+            BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, Lb, G, Lb, info + pass));
         BLAS(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                            rocblas_diagonal_non_unit, R, Lb, &one, G, Lb, Y, R));
     }
@@ -1139,7 +1262,12 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
             BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, Lb, R, &one, W, R, Q, R,
                                &zero, Y, R));
             if (it < next_check && north < max_orth) {
-                KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++));
+                // the next product only needs a well-conditioned basis of the
+                // span: one shifted and one plain Cholesky QR pass; the basis
+                // that Rayleigh-Ritz reads gets all three
+                bool before_check = it + 1 >= next_check || north + 1 >= max_orth;
+                KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++,
+                                    before_check ? 3 : 2));
                 std::swap(Q, Y);
                 continue;
             }
