@@ -297,6 +297,120 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
     }
 }
 
+// The same Sinkhorn pass with every wave on whole rows of its own: lane l
+// holds the 16-byte column vectors l, l + 64, ... (VPQ of them) of u and of
+// its F^T s' partial in registers, a row's dot product is a wave reduction,
+// and the next row (with its iterate entry) loads while the current one is
+// processed, so no barrier is taken per row.  The four waves' partials and
+// statistics are combined once per block.  Rows: block b owns SK_ROWS
+// consecutive rows, wave w every fourth of them.  At VPQ = 8 (r <= 2048) it
+// needs ~310 VGPRs, one wave per SIMD; capping it at two waves spills (and
+// moving u to LDS does not lower the count), so it runs at one.  FP32 only:
+// the FP64 factor uses k_sinkhorn_pass.
+template <typename T, int VPQ>
+__global__ __launch_bounds__(256) void k_sinkhorn_rows(const T* __restrict__ F, int64_t ldf, int64_t n,
+                                                       int64_t r, const double* __restrict__ uraw,
+                                                       double uscale, const double* __restrict__ b,
+                                                       double bscale, int damped, double* b_next,
+                                                       double* part, int64_t rpad, double* bstat)
+{
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type vec_t;
+    __shared__ double red[4][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t r0 = (int64_t)blockIdx.x * SK_ROWS;
+    const int64_t r1 = r0 + SK_ROWS < n ? r0 + SK_ROWS : n;
+    double uv[VPQ][V], acc[VPQ][V];
+#pragma unroll
+    for (int q = 0; q < VPQ; ++q)
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            int64_t c = ((int64_t)lane + 64 * q) * V + e;
+            uv[q][e] = c < r ? uscale * uraw[c] : 0.0;
+            acc[q][e] = 0.0;
+        }
+    auto load = [&](vec_t (&f)[VPQ], double& sb, int64_t i) {
+#pragma unroll
+        for (int q = 0; q < VPQ; ++q) {
+            int64_t c = ((int64_t)lane + 64 * q) * V;
+            f[q] = (i < r1 && c < r) ? *reinterpret_cast<const vec_t*>(F + i * ldf + c) : vec_t{};
+        }
+        sb = i < r1 ? b[i] : 1.0;
+    };
+    double mn = INFINITY, dv = 0.0, srs = 0.0, smin = INFINITY;
+    // F values are widened twice (dot, then accumulation) rather than held
+    // as doubles, which would cost VPQ * V * 2 more registers
+    auto fval = [&](const vec_t& f, int q, int k) {
+        int64_t c = ((int64_t)lane + 64 * q) * V + k;
+        return c < r ? (double)reinterpret_cast<const T*>(&f)[k] : 0.0;
+    };
+    auto row = [&](const vec_t (&f)[VPQ], double sb, int64_t i) {
+        double d = 0.0;
+#pragma unroll
+        for (int q = 0; q < VPQ; ++q)
+#pragma unroll
+            for (int k = 0; k < V; ++k) d += fval(f[q], q, k) * uv[q][k];
+        double y = wave_sum(d);
+        double si = bscale * sb;
+        double rs = si * y;
+        double bn = damped ? sqrt(si / y) : 1.0 / y;
+        if (lane == 0) {
+            b_next[i] = bn;
+            mn = fmin(mn, y);
+            dv = fmax(dv, fabs(rs - 1.0));
+            srs += rs;
+            smin = fmin(smin, si);
+        }
+#pragma unroll
+        for (int q = 0; q < VPQ; ++q)
+#pragma unroll
+            for (int k = 0; k < V; ++k) acc[q][k] += bn * fval(f[q], q, k);
+    };
+    vec_t f0[VPQ], f1[VPQ];
+    double s0 = 1.0, s1 = 1.0;
+    int64_t i = r0 + wv;
+    if (i < r1) load(f0, s0, i);
+    for (; i < r1; i += 8) {
+        if (i + 4 < r1) load(f1, s1, i + 4);
+        row(f0, s0, i);
+        if (i + 4 >= r1) break;
+        if (i + 8 < r1) load(f0, s0, i + 8);
+        row(f1, s1, i + 4);
+    }
+    // combine the four waves: partials through LDS one wave at a time, in
+    // wave order (deterministic)
+    __shared__ double cpart[64 * VPQ * V];
+    for (int w = 0; w < 4; ++w) {
+        if (wv == w) {
+#pragma unroll
+            for (int q = 0; q < VPQ; ++q)
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    int idx = (q * 64 + lane) * V + k;
+                    cpart[idx] = w == 0 ? acc[q][k] : cpart[idx] + acc[q][k];
+                }
+            if (lane == 0) {
+                red[w][0] = mn;
+                red[w][1] = dv;
+                red[w][2] = srs;
+                red[w][3] = smin;
+            }
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < 64 * VPQ * V; idx += 256) {
+        int q = idx / (64 * V), rem = idx % (64 * V), ln = rem / V, k = rem % V;
+        int64_t c = ((int64_t)ln + 64 * q) * V + k;
+        if (c < r) part[(int64_t)blockIdx.x * rpad + c] = cpart[idx];
+    }
+    if (threadIdx.x == 0) {
+        bstat[4 * blockIdx.x + 0] = fmin(fmin(red[0][0], red[1][0]), fmin(red[2][0], red[3][0]));
+        bstat[4 * blockIdx.x + 1] = fmax(fmax(red[0][1], red[1][1]), fmax(red[2][1], red[3][1]));
+        bstat[4 * blockIdx.x + 2] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
+        bstat[4 * blockIdx.x + 3] = fmin(fmin(red[0][3], red[1][3]), fmin(red[2][3], red[3][3]));
+    }
+}
+
 // [min, max, sum, min] over the nb per-block quadruples, in block order.
 __global__ __launch_bounds__(256) void k_stat_reduce(const double* bstat, int nb, double* out)
 {
@@ -1482,10 +1596,27 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
         for (it = 1; it <= maxit; ++it) {
             {
                 ProfScope ps(ctx, TC_MATVEC);
+#define KEVD_SK_ROWS(VQ)                                                                                     \
+    k_sinkhorn_rows<T, VQ><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, u, uscale, s,     \
+                                                       bscale, damped ? 1 : 0, s2, part, rpad, bstat)
 #define KEVD_SK_PASS(VP)                                                                                     \
     k_sinkhorn_pass<T, VP><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, u, uscale, s,     \
                                                        bscale, damped ? 1 : 0, s2, part, rpad, bstat)
-                if (vpt <= 1)
+                const int vpq = (int)(((r + V - 1) / V + 63) / 64);
+                bool rows_done = false;
+                if constexpr (sizeof(T) == 4) {
+                    if (vpq <= 8) {
+                        if (vpq <= 2)
+                            KEVD_SK_ROWS(2);
+                        else if (vpq <= 4)
+                            KEVD_SK_ROWS(4);
+                        else
+                            KEVD_SK_ROWS(8);
+                        rows_done = true;
+                    }
+                }
+                if (rows_done) {
+                } else if (vpt <= 1)
                     KEVD_SK_PASS(1);
                 else if (vpt <= 2)
                     KEVD_SK_PASS(2);
@@ -1494,6 +1625,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
                 else
                     KEVD_SK_PASS(VPT_MAX);
 #undef KEVD_SK_PASS
+#undef KEVD_SK_ROWS
                 k_colsum_reduce<<<(unsigned)((r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, r, rpad, u2);
                 k_stat_reduce<<<1, 256, 0, ctx->stream>>>(bstat, nb, st3);
             }
