@@ -82,7 +82,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     typedef DowndateLds<T> DL;
     __shared__ __attribute__((aligned(16))) T lds[DL::ELEMS];
     __shared__ double rs[DL::WN][DL::BM];
-    downdate_body<T>(X, dp, nrm, F, ldf, d, n, XS, nrmS, Si, FS, LinvT, st, gs, blockIdx.x, lds, rs);
+    __shared__ DowndateEp<T> ep;
+    downdate_body<T>(X, dp, nrm, F, ldf, d, n, XS, nrmS, Si, FS, LinvT, st, gs, blockIdx.x, lds, rs, &ep);
 }
 
 __global__ __launch_bounds__(BMAX) void k_commit(ArpcState* st, const int64_t* __restrict__ Si, int64_t* pivots,

@@ -370,6 +370,15 @@ template <typename T> struct DowndateLds {
     static constexpr int WN = BN / (G::NTN * Mfma<T>::MT);
 };
 
+// Epilogue constants of one downdate row block, staged in LDS before the
+// distance product so their global loads overlap it: the block's row norms
+// and the pivots' norms and indices.
+template <typename T> struct DowndateEp {
+    double nrm[Cfg<T>::BM];
+    double nrmS[Cfg<T>::BN];
+    int64_t Si[Cfg<T>::BN];
+};
+
 // The fused Step-3 update for BM rows (ks_arpclm.cu:929-995 and
 // update_diag_block 455-478):
 //   acc  = X_R X_{S_i}^T                    (MFMA, K = dp)
@@ -387,7 +396,7 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
                                      const T* __restrict__ XS, const double* __restrict__ nrmS,
                                      const int64_t* __restrict__ Si, const T* __restrict__ FS,
                                      const T* __restrict__ LinvT, const ArpcState* st, GaussScale gs,
-                                     int64_t blk, T* lds, double (*rs)[Cfg<T>::BM])
+                                     int64_t blk, T* lds, double (*rs)[Cfg<T>::BM], DowndateEp<T>* ep)
 {
     typedef typename Cfg<T>::G G;
     typedef DowndateLds<T> DL;
@@ -398,10 +407,15 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
     int64_t Nr = st->Nr;
     int64_t r0 = blk * BM;
     int64_t rows = n - r0;
+    for (int i = threadIdx.x; i < BM; i += 256) ep->nrm[i] = r0 + i < n ? nrm[r0 + i] : 0.0;
+    for (int i = threadIdx.x; i < BN; i += 256) {
+        ep->nrmS[i] = nrmS[i];
+        ep->Si[i] = Si[i];
+    }
     G g;
     g.init();
     g.set_live_cols(k);
-    // distance product
+    // distance product (its barriers also publish ep)
     g.mainloop(lds, X + r0 * dp, dp, rows, XS, dp, BN, dp);
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
@@ -413,9 +427,9 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
                 int col = g.col_of(b);
                 double kv = 0.0;
                 if (row < n && col < k) {
-                    double D = nrm[row] + nrmS[col] - 2.0 * (double)g.acc[a][b][v];
+                    double D = ep->nrm[row - r0] + ep->nrmS[col] - 2.0 * (double)g.acc[a][b][v];
                     D = D > 0.0 ? D : 0.0;
-                    kv = (row == Si[col]) ? 1.0 : (double)gauss_val(D, gs, (T*)nullptr);
+                    kv = (row == ep->Si[col]) ? 1.0 : (double)gauss_val(D, gs, (T*)nullptr);
                 }
                 g.acc[a][b][v] = (T)(-kv);
             }

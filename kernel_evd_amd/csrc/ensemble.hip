@@ -125,6 +125,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     typedef DowndateLds<T> DL;
     __shared__ __attribute__((aligned(16))) T lds[DL::ELEMS];
     __shared__ double rs[DL::WN][DL::BM];
+    __shared__ DowndateEp<T> ep;
     __shared__ int item_s;
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(ctr, 1);
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         int2 w = items[it];
         const EnsProb<T>& q = P[w.x];
         downdate_body<T>(q.X, q.dp, q.nrm, q.F, q.ldf, q.d, q.n, q.XS, q.nrmS, q.Si, q.FS, q.LinvT, q.st, q.gs,
-                         w.y, lds, rs);
+                         w.y, lds, rs, &ep);
     }
 }
 
