@@ -14,8 +14,9 @@ eigenvalue errors and the leading-eigenvector subspace angles; the EVD in
 the other normalisation on both factors; the mixed ARPC with the FP64
 pivots injected, for the factor's relative Frobenius difference; the
 uneven-size ensemble (48 KS problems, batched ARPC + concurrent kEVD, against
-the same problems run one after another); and the CPU FP64 oracle on a
-bounded sample for the CPU baseline.
+the same problems run one after another); the CPU FP64 oracle on a bounded
+sample for the CPU baseline; and the small configuration (N = 8192, FP64,
+block size 1, rank 512) beside the dense NumPy reference EVD.
 
 Prints ONE JSON line.  Single GPU only.
 """
@@ -206,6 +207,49 @@ def ensemble_leg(nprob=48, kout=32, seed=1, workers=8, candidates=(32, 64, 128),
            "eig_rel_err_vs_fp64_same_pivots_median": float(np.median(errs_same))}
     log("ensemble: %d problems, %d columns: ARPC %.3f s + EVD %.3f s (sequential %.3f s); max eig err %.2e"
         % (nprob, cols, t1 - t0, t2 - t1, t4 - t3, max(errs)))
+    return res
+
+
+def c0_leg(n=8192, rank=512, kout=32, seed=0):
+    """N = 8192 KS snapshots (FP64), eps = the exact median
+    squared distance.  GPU: all-FP64 RPCholesky with block size 1, rank 512,
+    then the top-32 two-step kEVD.  CPU: the dense NumPy restatement of
+    ks_reference.py:21-40 (the reference's only CPU path) on the same
+    points, timed on the host's BLAS threads; it is also the check of the
+    GPU eigenvalues (which carry the rank-512 approximation error)."""
+    from kernel_evd_amd import Context, ks_generate
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    X = ks_generate(n, L_dom=22.0, M=64, dt=0.25, n_discard=10000, stride=4, seed=seed)
+    with Context(0) as c:
+        c.set_points(X)
+        eps = c.median_sqdist(seed=0, nsample=0)
+        for rep in range(2):  # the first run sizes the buffers
+            t0 = time.perf_counter()
+            a = c.arpc(eps, rank, block=1, seed=seed, tol=1e-10, prec="fp64")
+            t1 = time.perf_counter()
+            e = c.evd(a.rank, k_out=kout, prec="fp64", norm="ref", want_U=False)
+            t2 = time.perf_counter()
+    t3 = time.perf_counter()
+    K = orc.dense_kernel(X, eps)
+    w, _, _ = orc.dense_reference_evd(K)
+    t4 = time.perf_counter()
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    gpu_top = e.lam[::-1][:kout]
+    rel = np.abs(gpu_top - w[:kout]) / np.abs(w[:kout])
+    res = {"workload": "N=%d KS (FP64), eps exact median; GPU ARPC b=1 r=%d FP64 + top-%d two-step kEVD" % (n, rank, kout),
+           "gpu_arpc_s": round(t1 - t0, 4), "gpu_evd_s": round(t2 - t1, 4), "rounds": int(a.rounds),
+           "trace_err": float(a.trace_err[-1]),
+           "cpu_dense_reference_s": round(t4 - t3, 3), "cpu_threads": int(threads),
+           "eig_rel_err_vs_dense_top%d_max" % kout: float(rel.max()),
+           "eig_rel_err_vs_dense_top8_max": float(rel[:8].max()),
+           "lambda_max_gpu": float(gpu_top[0])}
+    log("C0: GPU ARPC %.3f s (%d rounds) + EVD %.3f s; dense CPU reference %.1f s (%d threads); top-%d err %.2e"
+        % (t1 - t0, a.rounds, t2 - t1, t4 - t3, threads, kout, rel.max()))
     return res
 
 
@@ -427,6 +471,7 @@ def main():
                                          "+ kEVD %.2f s measured, scaled x%.0f (both stages linear in N)"
                                          % (ns, args.n, ta, te, scale)}
         log("cpu baseline: %.2f s measured on %d points (%d threads)" % (ta + te, ns, nt))
+        out["c0_small"] = c0_leg()
     if ctx is not None:
         ctx.close()
     print(json.dumps(out), flush=True)

@@ -547,6 +547,7 @@ struct GramTile {
         T wv;  // this thread's staged weight
     };
     int wm, wn, h, li;
+    bool live = true;  // false: this wave's block lies above the diagonal (never read)
 
     __device__ inline void zero()
     {
@@ -631,6 +632,7 @@ struct GramTile {
     // its MFMAs, so no VALU write waits on an MFMA still reading a register.
     __device__ inline void compute(const T* cur)
     {
+        if (!live) return;
         constexpr int NS = KC / M::NH;
         pair_t araw[2][TM / IL], b[2][TN / IL];
         T wk[2][NG];
@@ -751,6 +753,9 @@ void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T*
     g.wn = w & 1;
     g.h = lane / Mfma<T>::MT;
     g.li = lane % Mfma<T>::MT;
+    // on tiles that straddle the diagonal, a wave block with every row index
+    // below every column index only holds entries the reduction never reads
+    g.live = ci + (int64_t)(g.wm + 1) * G::WR - 1 >= cj + (int64_t)g.wn * G::WC;
     g.zero();
     bool first = true;
     int64_t nk = k_end > k_beg ? (k_end - k_beg + G::KC - 1) / G::KC : 0;
