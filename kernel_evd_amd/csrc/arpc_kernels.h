@@ -367,7 +367,7 @@ template <typename T> struct DowndateLds {
     static constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
     static constexpr int TP = BN + 16 / (int)sizeof(T);  // T tile pitch
     static constexpr int ELEMS = G::LDS_ELEMS > BM * TP ? G::LDS_ELEMS : BM * TP;
-    static constexpr int WN = BN / (G::NTN * Mfma<T>::MT);
+    static constexpr int WN = BN / (G::NTN * G::M::MT);
 };
 
 // Epilogue constants of one downdate row block, staged in LDS before the
@@ -422,7 +422,7 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
 #pragma unroll
         for (int b = 0; b < G::NTN; ++b)
 #pragma unroll
-            for (int v = 0; v < Mfma<T>::ACC; ++v) {
+            for (int v = 0; v < G::M::ACC; ++v) {
                 int64_t row = r0 + g.row_of(a, v);
                 int col = g.col_of(b);
                 double kv = 0.0;
@@ -441,23 +441,23 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
 #pragma unroll
         for (int b = 0; b < G::NTN; ++b)
 #pragma unroll
-            for (int v = 0; v < Mfma<T>::ACC; ++v)
+            for (int v = 0; v < G::M::ACC; ++v)
                 lds[g.row_of(a, v) * TP + g.col_of(b)] = -g.acc[a][b][v];
     __syncthreads();
     g.zero();
-    g.lds_global_tri_live(lds, TP, LinvT, BMAX, (k + 7) & ~7);  // L_c^-T is triangular
+    g.lds_global_tri_live(lds, TP, LinvT, BMAX, (k + G::KSUB - 1) / G::KSUB * G::KSUB);  // L_c^-T is triangular
     // write the new columns and the squared row norms
-    double part[G::NTM][Mfma<T>::ACC];
+    double part[G::NTM][G::M::ACC];
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
 #pragma unroll
-        for (int v = 0; v < Mfma<T>::ACC; ++v) part[a][v] = 0.0;
+        for (int v = 0; v < G::M::ACC; ++v) part[a][v] = 0.0;
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
 #pragma unroll
         for (int b = 0; b < G::NTN; ++b)
 #pragma unroll
-            for (int v = 0; v < Mfma<T>::ACC; ++v) {
+            for (int v = 0; v < G::M::ACC; ++v) {
                 int64_t row = r0 + g.row_of(a, v);
                 int col = g.col_of(b);
                 T gv = g.acc[a][b][v];
@@ -467,11 +467,11 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
                 }
             }
     // reduce over the MT lanes that share a row (lane % MT = column)
-    constexpr int MT = Mfma<T>::MT;
+    constexpr int MT = G::M::MT;
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
 #pragma unroll
-        for (int v = 0; v < Mfma<T>::ACC; ++v) {
+        for (int v = 0; v < G::M::ACC; ++v) {
             double s = part[a][v];
 #pragma unroll
             for (int o = 1; o < MT; o <<= 1) s += __shfl_xor(s, o, 64);

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void k_bw_sweep(const T* __restrict__ XR, cons
 {
     typedef typename Cfg<T>::G G;
     constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
-    constexpr int NV = G::NTM * G::NTN * Mfma<T>::ACC;  // distances per lane
+    constexpr int NV = G::NTM * G::NTN * G::M::ACC;  // distances per lane
     __shared__ __attribute__((aligned(16))) T lds[G::LDS_ELEMS];
     __shared__ double acc64[4][NEPS_MAX];
     int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_bw_sweep(const T* __restrict__ XR, cons
 #pragma unroll
             for (int b = 0; b < G::NTN; ++b)
 #pragma unroll
-                for (int v = 0; v < Mfma<T>::ACC; ++v) {
+                for (int v = 0; v < G::M::ACC; ++v) {
                     int64_t row = r0 + g.row_of(a, v);
                     int64_t col = c0 + g.col_of(b);
                     T dv = (T)INFINITY;  // exp(-inf) = 0 for padding
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_bw_sweep(const T* __restrict__ XR, cons
                         if (ridx[row] == col) Dd = 0.0;
                         dv = (T)Dd;
                     }
-                    D[(a * G::NTN + b) * Mfma<T>::ACC + v] = dv;
+                    D[(a * G::NTN + b) * G::M::ACC + v] = dv;
                 }
         for (int e0 = 0; e0 < neps; e0 += ECH) {
             T s[ECH];

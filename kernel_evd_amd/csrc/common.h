@@ -67,6 +67,7 @@ __host__ __device__ inline uint64_t kevd_mulhi64(uint64_t a, uint64_t b)
 // ------------------------------------------------------------------- MFMA
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 // Per-precision MFMA shape.  Both use the "8 K values per sub-step" scheme:
@@ -112,9 +113,32 @@ template <> struct Mfma<double> {
     __host__ __device__ static constexpr int row_off(int v) { return 4 * v; }
 };
 
+// FP32 16x16 tiles (v_mfma_f32_16x16x4_f32): same rate as 32x32x2, finer
+// column granularity for GEMMs whose live column count varies (the
+// downdate's accepted pivots, the U product's k_out + 1 columns).
+struct Mfma16f {
+    static constexpr int MT = 16;
+    static constexpr int NH = 4;
+    static constexpr int VEC = 4;
+    static constexpr int ACC = 4;
+    typedef floatx4 acc_t;
+    typedef float4 vec_t;
+    __device__ static inline void mma(acc_t& c, float a, float b)
+    {
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static inline float get(const vec_t& v, int j)
+    {
+        return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
+    }
+    // row 4 h + v (read off the hardware with a single-MFMA probe)
+    __device__ static inline int row_lane(int h) { return 4 * h; }
+    __host__ __device__ static constexpr int row_off(int v) { return v; }
+};
+
 // Row (within the MT x MT tile) of accumulator value v held by lane group h;
 // the column is lane % MT.  FP32 32x32: 8 (v / 4) + 4 h + v % 4; FP64 16x16:
-// h + 4 v (checked against the kernel-block parity tests).
+// h + 4 v; FP32 16x16: 4 h + v (checked against the kernel-block parity tests).
 template <typename T>
 __device__ inline int acc_row(int v, int h)
 {

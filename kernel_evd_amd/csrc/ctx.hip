@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void k_kblock(const T* XR, const double* nR, c
 #pragma unroll
         for (int b = 0; b < TN; ++b)
 #pragma unroll
-            for (int v = 0; v < Mfma<T>::ACC; ++v) {
+            for (int v = 0; v < G::M::ACC; ++v) {
                 int64_t row = r0 + g.row_of(a, v);
                 int64_t col = c0 + g.col_of(b);
                 if (row < nr && col < nc) {

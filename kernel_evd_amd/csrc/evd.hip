@@ -863,7 +863,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int b = 0; b < G::NTN; ++b)
 #pragma unroll
-            for (int v = 0; v < Mfma<T>::ACC; ++v) {
+            for (int v = 0; v < G::M::ACC; ++v) {
                 int64_t row = r0 + g.row_of(a, v);
                 int col = c0 + g.col_of(b);
                 if (row >= n || col >= kc) continue;

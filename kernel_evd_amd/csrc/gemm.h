@@ -12,12 +12,13 @@
 
 #include "common.h"
 
-template <typename T, int BM, int BN, int KC, int WM, int WN, int TM, int TN>
+template <typename T, int BM, int BN, int KC, int WM, int WN, int TM, int TN, typename MF = Mfma<T>>
 struct NTGemm {
-    typedef Mfma<T> M;
+    typedef MF M;
     typedef typename M::acc_t acc_t;
     typedef typename M::vec_t vec_t;
     static constexpr int MT = M::MT;
+    static constexpr int KSUB = M::NH * M::VEC;            // K values per sub-step
     static constexpr int NTM = TM, NTN = TN, NBM = BM, NBN = BN, NKC = KC;
     static constexpr int EPV = 16 / (int)sizeof(T);       // elements per 16 B
     static constexpr int PITCH = KC + EPV;                // LDS row pitch (elements)
@@ -29,7 +30,7 @@ struct NTGemm {
     static constexpr int LDS_ELEMS = 2 * STAGE;
     static_assert(WM * WN == 4, "4 waves");
     static_assert(WM * TM * MT == BM && WN * TN * MT == BN, "tile shape");
-    static_assert(KC % 8 == 0 && BM % RPP == 0 && BN % RPP == 0, "load shape");
+    static_assert(KC % KSUB == 0 && BM % RPP == 0 && BN % RPP == 0, "load shape");
 
     acc_t acc[TM][TN];
     int wm, wn, h, li;
@@ -59,7 +60,7 @@ struct NTGemm {
     // tile (a, b) held by this lane.
     __device__ inline int row_of(int a, int v) const
     {
-        return wm * TM * MT + a * MT + acc_row<T>(v, h);
+        return wm * TM * MT + a * MT + M::row_lane(h) + M::row_off(v);
     }
     __device__ inline int col_of(int b) const { return wn * TN * MT + b * MT + li; }
 
@@ -113,17 +114,17 @@ struct NTGemm {
     __device__ inline void compute(const T* As, const T* Bs)
     {
 #pragma unroll
-        for (int s = 0; s < KC / 8; ++s) {
+        for (int s = 0; s < KC / KSUB; ++s) {
             vec_t a[TM], b[TN];
 #pragma unroll
             for (int t = 0; t < TM; ++t)
                 a[t] = *reinterpret_cast<const vec_t*>(
-                    As + (wm * TM * MT + t * MT + li) * PITCH + s * 8 + h * M::VEC);
+                    As + (wm * TM * MT + t * MT + li) * PITCH + s * KSUB + h * M::VEC);
 #pragma unroll
             for (int t = 0; t < TN; ++t)
                 if (t < nact)
                     b[t] = *reinterpret_cast<const vec_t*>(
-                        Bs + (wn * TN * MT + t * MT + li) * PITCH + s * 8 + h * M::VEC);
+                        Bs + (wn * TN * MT + t * MT + li) * PITCH + s * KSUB + h * M::VEC);
 #pragma unroll
             for (int y = 0; y < TN; ++y)
                 if (y < nact)
@@ -190,17 +191,45 @@ struct NTGemm {
     template <int NACT>
     __device__ inline void compute_n(const T* As, const T* Bs)
     {
-        constexpr int NS = KC / 8;
+        constexpr int NS = KC / KSUB;
+        // double-buffer the fragments only while they are small (<= 24
+        // VGPRs per set); a wide sub-step (16x16 tiles: 64 MFMAs) hides the
+        // LDS latency of the next one's reads on its own
+        constexpr bool DB = (TM + NACT) * (int)sizeof(vec_t) / 4 <= 24;
+        if constexpr (!DB) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                vec_t a[TM], b[NACT];
+#pragma unroll
+                for (int t = 0; t < TM; ++t)
+                    a[t] = *reinterpret_cast<const vec_t*>(
+                        As + (wm * TM * MT + t * MT + li) * PITCH + s * KSUB + h * M::VEC);
+#pragma unroll
+                for (int t = 0; t < NACT; ++t)
+                    b[t] = *reinterpret_cast<const vec_t*>(
+                        Bs + (wn * TN * MT + t * MT + li) * PITCH + s * KSUB + h * M::VEC);
+#pragma unroll
+                for (int j = 0; j < M::VEC; ++j)
+#pragma unroll
+                    for (int y = 0; y < NACT; ++y)
+#pragma unroll
+                        for (int x = 0; x < TM; ++x) M::mma(acc[x][y], M::get(a[x], j), M::get(b[y], j));
+                // keep the next sub-step's reads below this one's MFMAs (hoisted,
+                // both sets would be live at once)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            return;
+        }
         vec_t a[2][TM], b[2][NACT];
         auto frag = [&](int q, int s) {
 #pragma unroll
             for (int t = 0; t < TM; ++t)
                 a[q][t] = *reinterpret_cast<const vec_t*>(
-                    As + (wm * TM * MT + t * MT + li) * PITCH + s * 8 + h * M::VEC);
+                    As + (wm * TM * MT + t * MT + li) * PITCH + s * KSUB + h * M::VEC);
 #pragma unroll
             for (int t = 0; t < NACT; ++t)
                 b[q][t] = *reinterpret_cast<const vec_t*>(
-                    Bs + (wn * TN * MT + t * MT + li) * PITCH + s * 8 + h * M::VEC);
+                    Bs + (wn * TN * MT + t * MT + li) * PITCH + s * KSUB + h * M::VEC);
         };
         frag(0, 0);
 #pragma unroll
@@ -227,7 +256,6 @@ struct NTGemm {
     {
         int64_t nk = (K + KC - 1) / KC, nfull = K / KC;
         if (nk <= 0) return;
-        uint4 ra0[NVA], rb0[NVB], ra1[NVA], rb1[NVB];
         auto load = [&](uint4 (&ra)[NVA], uint4 (&rb)[NVB], int64_t c) {
             if (c < nfull) {
                 gload_full<NVA>(ra, A, lda, arows, c * KC);
@@ -237,6 +265,30 @@ struct NTGemm {
                 gload<NVB>(rb, B, ldb, brows, c * KC, K);
             }
         };
+        // wide fragment sets (16x16 tiles) leave no room for a second
+        // register set: loads run one chunk ahead
+        constexpr bool DEEP = (TM + NACT) * (int)sizeof(vec_t) / 4 <= 24;
+        if constexpr (!DEEP) {
+            uint4 ra[NVA], rb[NVB];
+            load(ra, rb, 0);
+            sstore<NVA>(ra, lds);
+            sstore<NVB>(rb, lds + BM * PITCH);
+            __syncthreads();
+            for (int64_t c = 0; c < nk; ++c) {
+                T* cur = lds + (c & 1) * STAGE;
+                T* nxt = lds + ((c + 1) & 1) * STAGE;
+                bool more = c + 1 < nk;
+                if (more) load(ra, rb, c + 1);
+                compute_n<NACT>(cur, cur + BM * PITCH);
+                if (more) {
+                    sstore<NVA>(ra, nxt);
+                    sstore<NVB>(rb, nxt + BM * PITCH);
+                }
+                __syncthreads();
+            }
+            return;
+        }
+        uint4 ra0[NVA], rb0[NVB], ra1[NVA], rb1[NVB];
         load(ra0, rb0, 0);
         sstore<NVA>(ra0, lds);
         sstore<NVB>(rb0, lds + BM * PITCH);
@@ -278,21 +330,22 @@ struct NTGemm {
     // acc += Ts Bg^T as lds_global, for a B operand that is lower triangular
     // (Bg[j][m] = 0 for m > j, row j = output column j), NACT live column
     // tiles and K <= NACT * MT.  Column tile y only needs K < MT (y + 1), so
-    // sub-step s (8 K values) runs the tiles y >= s / (MT / 8): about half
+    // sub-step s (KSUB K values) runs the tiles y >= s / (MT / KSUB): about half
     // the MFMAs of the full product.  B fragments are read from global
     // memory one sub-step ahead.
     template <int NACT>
     __device__ inline void lds_global_tri(const T* Ts, int tp, const T* Bg, int64_t ldb, int K)
     {
-        constexpr int SPT = MT / 8, NSM = NACT * SPT;
-        const int ns = K / 8;
+        constexpr int SPT = MT / KSUB, NSM = NACT * SPT;
+        static_assert(SPT >= 1, "a column tile spans at least one sub-step");
+        const int ns = K / KSUB;
         vec_t b[2][NACT];
         auto bload = [&](int q, int s) {
 #pragma unroll
             for (int t = 0; t < NACT; ++t)
                 if (t >= s / SPT)
                     b[q][t] = *reinterpret_cast<const vec_t*>(
-                        Bg + (int64_t)(wn * TN * MT + t * MT + li) * ldb + s * 8 + h * M::VEC);
+                        Bg + (int64_t)(wn * TN * MT + t * MT + li) * ldb + s * KSUB + h * M::VEC);
         };
         if (ns > 0) bload(0, 0);
 #pragma unroll
@@ -303,7 +356,7 @@ struct NTGemm {
 #pragma unroll
                 for (int t = 0; t < TM; ++t)
                     a[t] = *reinterpret_cast<const vec_t*>(
-                        Ts + (wm * TM * MT + t * MT + li) * tp + s * 8 + h * M::VEC);
+                        Ts + (wm * TM * MT + t * MT + li) * tp + s * KSUB + h * M::VEC);
 #pragma unroll
                 for (int j = 0; j < M::VEC; ++j)
 #pragma unroll
@@ -331,7 +384,7 @@ struct NTGemm {
     // s + 1 is loaded while sub-step s multiplies, hiding the L2 latency.
     __device__ inline void lds_global(const T* Ts, int tp, const T* Bg, int64_t ldb, int K)
     {
-        int ns = K / 8;
+        int ns = K / KSUB;
         if (ns <= 0) return;
         vec_t b[TN], bn[TN];
 #pragma unroll
@@ -344,13 +397,13 @@ struct NTGemm {
                 for (int t = 0; t < TN; ++t)
                     if (t < nact)
                         bn[t] = *reinterpret_cast<const vec_t*>(
-                            Bg + (int64_t)(wn * TN * MT + t * MT + li) * ldb + (s + 1) * 8 + h * M::VEC);
+                            Bg + (int64_t)(wn * TN * MT + t * MT + li) * ldb + (s + 1) * KSUB + h * M::VEC);
             }
             vec_t a[TM];
 #pragma unroll
             for (int t = 0; t < TM; ++t)
                 a[t] = *reinterpret_cast<const vec_t*>(
-                    Ts + (wm * TM * MT + t * MT + li) * tp + s * 8 + h * M::VEC);
+                    Ts + (wm * TM * MT + t * MT + li) * tp + s * KSUB + h * M::VEC);
 #pragma unroll
             for (int y = 0; y < TN; ++y)
                 if (y < nact)
@@ -372,7 +425,7 @@ struct NTGemm {
 // 16x16).
 template <typename T> struct Cfg;
 template <> struct Cfg<float> {
-    typedef NTGemm<float, 128, 128, 32, 4, 1, 1, 4> G;
+    typedef NTGemm<float, 128, 128, 32, 4, 1, 2, 8, Mfma16f> G;
     static constexpr int BM = 128, BN = 128, KC = 32;
 };
 template <> struct Cfg<double> {
