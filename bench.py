@@ -14,9 +14,13 @@ eigenvalue errors and the leading-eigenvector subspace angles; the EVD in
 the other normalisation on both factors; the mixed ARPC with the FP64
 pivots injected, for the factor's relative Frobenius difference; the
 uneven-size ensemble (48 KS problems, batched ARPC + concurrent kEVD, against
-the same problems run one after another); the CPU FP64 oracle on a bounded
-sample for the CPU baseline; and the small configuration (N = 8192, FP64,
-block size 1, rank 512) beside the dense NumPy reference EVD.
+the same problems run one after another); the bandwidth sweep (16,384 rows x
+262,144 snapshots x 64 eps) against the measured v_exp_f32 rate and FP64;
+the CPU FP64 oracle on a bounded sample for the CPU baseline; and the small
+configuration (N = 8192, FP64, block size 1, rank 512) beside the dense
+NumPy reference EVD.  The device's sustained MFMA, exp and copy rates are
+measured once (kevd_measure_peaks) and each roofline fraction is given
+against them as well as against the published peaks.
 
 Prints ONE JSON line.  Single GPU only.
 """
@@ -253,6 +257,73 @@ def c0_leg(n=8192, rank=512, kout=32, seed=0):
     return res
 
 
+def c1_leg(X, peaks, n=262144, nrows=16384, neps=64, cpu_rows=256, with_cpu=True, seed=0):
+    """S(eps) = sum over 16,384 seeded rows x all N = 262,144 KS
+    snapshots of exp(-|x_i - x_c|^2 / eps), for 64 eps log-spaced over
+    [1e-3, 1e3] x the median squared distance, all from one fused launch
+    (FP32 MFMA distances, v_exp_f32, FP64 sums).  The points are the first N
+    snapshots of the trajectory, which is the N-point generation.  Reported:
+    exp evaluations per second against the measured v_exp_f32 rate, the
+    log-log slope against the GPU FP64 path on all rows and against the CPU
+    FP64 oracle (brute force, OpenMP) on the first cpu_rows rows, and the
+    oracle's time on those rows."""
+    from kernel_evd_amd import Context, bandwidth_post
+    Xn = np.ascontiguousarray(X[:n], dtype=np.float32)
+    rows = np.sort(np.random.default_rng(seed).choice(n, nrows, replace=False)).astype(np.int64)
+    with Context(0) as c:
+        c.set_points(Xn)
+        med = c.median_sqdist(seed=0, nsample=1 << 22)
+        eps = med * np.logspace(-3, 3, neps)
+
+        def timed(r, e, prec, reps):
+            c.bandwidth_sweep(r, e, prec)  # sizes the buffers
+            best, S = 1e30, None
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                S = c.bandwidth_sweep(r, e, prec)
+                best = min(best, time.perf_counter() - t0)
+            return S, best
+
+        S32, t32 = timed(rows, eps, "mixed", 3)
+        _, t16 = timed(rows, eps[:16], "mixed", 3)
+        S64, t64 = timed(rows, eps, "fp64", 1)
+        Ssub = c.bandwidth_sweep(rows[:cpu_rows], eps, "mixed")
+    evals = float(nrows) * n * neps
+    flop_dist = 2.0 * nrows * n * 64
+    # 16 eps are one register chunk; the 48 more cost (t32 - t16), so the
+    # distance tile alone is about t16 - (t32 - t16) / 3
+    t_dist = max(t16 - (t32 - t16) / 3.0, 1e-9)
+    _, sl32, eopt, mdim = bandwidth_post(eps, S32)
+    _, sl64, eopt64, mdim64 = bandwidth_post(eps, S64)
+    res = {"workload": "bandwidth sweep: %d seeded rows x N=%d KS snapshots (dim 64), %d eps log-spaced "
+                       "over [1e-3, 1e3] x median, one fused launch" % (nrows, n, neps),
+           "ms": round(1e3 * t32, 3), "exp_evals": evals, "evals_per_s": evals / t32,
+           "exp_peak_measured": peaks.get("exp_f32_per_s") if peaks else None,
+           "frac_of_measured_exp_peak": round(evals / t32 / peaks["exp_f32_per_s"], 4) if peaks else None,
+           "ms_16_eps": round(1e3 * t16, 3), "distance_part_ms_est": round(1e3 * t_dist, 3),
+           "distance_mfma_frac_est": round(flop_dist / t_dist / 1e12 / FP32_MFMA_PEAK_TF, 4),
+           "fp64_ms": round(1e3 * t64, 3),
+           "S_rel_err_vs_fp64_max": float(np.max(np.abs(S32 - S64) / S64)),
+           "slope_abs_err_vs_fp64_max": float(np.max(np.abs(sl32 - sl64))),
+           "eopt": float(eopt), "eopt_fp64": float(eopt64), "mdim": float(mdim), "mdim_fp64": float(mdim64)}
+    log("C1: sweep %.2f ms (%.3g exps/s), FP64 %.1f ms; slope err vs FP64 %.2e"
+        % (1e3 * t32, evals / t32, 1e3 * t64, res["slope_abs_err_vs_fp64_max"]))
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        t0 = time.perf_counter()
+        Scpu = orc.bandwidth(Xn.astype(np.float64), rows[:cpu_rows], eps)
+        tc = time.perf_counter() - t0
+        _, slc, _, _ = bandwidth_post(eps, Scpu)
+        _, sls, _, _ = bandwidth_post(eps, Ssub)
+        res["cpu_fp64"] = {"rows": cpu_rows, "s": round(tc, 3), "threads": int(orc.num_threads()),
+                           "scaled_to_all_rows_s": round(tc * nrows / cpu_rows, 1),
+                           "S_rel_err_max": float(np.max(np.abs(Ssub - Scpu) / Scpu)),
+                           "slope_abs_err_max": float(np.max(np.abs(sls - slc)))}
+        log("C1 CPU FP64: %d rows in %.2f s; slope err %.2e" % (cpu_rows, tc, res["cpu_fp64"]["slope_abs_err_max"]))
+    return res
+
+
 def cpu_baseline(X, eps, rank, block, kout, n_sample, norm):
     """CPU FP64 oracle (OpenMP C ARPC + NumPy F-form kEVD) on the first
     n_sample points; both stages scale linearly in N (the Sinkhorn iteration
@@ -290,6 +361,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble leg")
+    ap.add_argument("--no-bandwidth", action="store_true", help="skip the bandwidth-sweep leg")
     ap.add_argument("--ensemble-only", action="store_true", help="run only the ensemble leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one cold FP32 copy of MIB MiB first (PMC byte-counter calibration)")
@@ -328,6 +400,8 @@ def main():
             agg[k] = agg.get(k, 0.0) + v
     ttotal = time.perf_counter() - tstart
     ctx.set_profiling(False)
+    peaks = ctx.measure_peaks()
+    log("measured peaks: %s" % {k: round(v, 4 if v < 1e6 else -9) for k, v in peaks.items()})
     ms_step = 1e3 * ttotal / args.steps
     a_mix, e_mix = a, e
     steps = args.steps
@@ -379,11 +453,23 @@ def main():
                      "traffic_source": TRAFFIC_SOURCE,
                      "flop_per_step": dd_flop, "launches_per_step": dd_launch,
                      "algorithmic_bytes_per_launch": downdate_bytes(a_mix.accepted, args.n, 64) / max(dd_launch, 1),
-                     "avg_launch_ms": round(dd_ms / max(dd_launch, 1), 4)},
+                     "avg_launch_ms": round(dd_ms / max(dd_launch, 1), 4),
+                     "peak_measured": round(peaks["mfma_f32_tflops"], 2),
+                     "frac_of_measured": round(dd_flop / (dd_ms * 1e-3) / 1e12 / peaks["mfma_f32_tflops"], 4)
+                     if dd_ms > 0 else None},
         "gram_roofline": {"kernel": "k_gram<float,%d>" % (1 if norm == "sinkhorn" else 2),
                           "achieved": round(gr_flop / (gr_ms * 1e-3) / 1e12, 3) if gr_ms > 0 else None,
                           "peak": FP32_MFMA_PEAK_TF, "unit": "TFLOP/s",
-                          "frac": round(gr_flop / (gr_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4) if gr_ms > 0 else None},
+                          "frac": round(gr_flop / (gr_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4) if gr_ms > 0 else None,
+                          "frac_of_measured": round(gr_flop / (gr_ms * 1e-3) / 1e12 / peaks["mfma_f32_tflops"], 4)
+                          if gr_ms > 0 else None},
+        "measured_peaks": {"mfma_f32_16x16x4_tflops": round(peaks["mfma_f32_tflops"], 2),
+                           "mfma_f32_32x32x2_tflops": round(peaks["mfma_f32_32_tflops"], 2),
+                           "mfma_f64_tflops": round(peaks["mfma_f64_tflops"], 2),
+                           "exp_f32_per_s": peaks["exp_f32_per_s"],
+                           "hbm_copy_gbs": round(peaks["hbm_copy_gbs"], 1),
+                           "how": "kevd_measure_peaks: register-only MFMA / v_exp_f32 chains on every CU, "
+                                  "4 GiB device copy (read + written bytes)"},
     }
     if norm == "sinkhorn":
         # F^T 1 once, then one fused pass over F per Sinkhorn iteration
@@ -394,6 +480,8 @@ def main():
                                       "achieved": round(mv_bytes / (mv_ms * 1e-3) / 1e9, 1),
                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                       "frac": round(mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                      "frac_of_measured_copy": round(mv_bytes / (mv_ms * 1e-3) / 1e9
+                                                                     / peaks["hbm_copy_gbs"], 4),
                                       "traffic_per_pass": TRAFFIC_BYTES["k_sinkhorn_pass<float,2>"]
                                       if args.n == 1 << 20 and args.rank == 2000 else None}
 
@@ -460,6 +548,12 @@ def main():
         ctx.close()
         ctx = None
         out["ensemble"] = ensemble_leg(norm=norm)
+
+    if not args.no_bandwidth:
+        if ctx is not None:
+            ctx.close()
+            ctx = None
+        out["c1_bandwidth"] = c1_leg(X, peaks, with_cpu=not args.no_cpu)
 
     if not args.no_cpu:
         ns = min(args.cpu_sample, args.n)

@@ -275,6 +275,21 @@ kevd_status kevd_ensemble_evd(kevd_ctx** ctxs, int32_t nprob, const kevd_evd_par
 kevd_status kevd_autotune_block(kevd_ctx* ctx, const kevd_arpc_params* base, const int32_t* candidates,
                                 int32_t ncand, int32_t* best_block, double* cols_per_s);
 
+/* ---------------------------------------------------------------- peaks
+ * Box-measured ceilings for roofline fractions (no counterpart in the
+ * reference, which reports none): the rate of each instruction or access
+ * pattern a hot kernel is bound by, sustained over every CU of the
+ * context's device.  copy_bytes in [16 MiB, 64 GiB] sizes the HBM copy
+ * (two buffers of that size are allocated and freed inside the call). */
+typedef struct {
+    double mfma_f32_tflops;     /* v_mfma_f32_16x16x4_f32, TFLOP/s          */
+    double mfma_f32_32_tflops;  /* v_mfma_f32_32x32x2_f32, TFLOP/s          */
+    double mfma_f64_tflops;     /* v_mfma_f64_16x16x4_f64, TFLOP/s          */
+    double exp_f32_per_s;       /* v_exp_f32 results per second            */
+    double hbm_copy_gbs;        /* device copy, read + written GB/s        */
+} kevd_peaks;
+kevd_status kevd_measure_peaks(kevd_ctx* ctx, int64_t copy_bytes, kevd_peaks* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,11 @@ class Timings(C.Structure):
                 ("dense_ms", dbl), ("u_ms", dbl), ("eig_iters", i32), ("eig_householder", i32)]
 
 
+class Peaks(C.Structure):
+    _fields_ = [("mfma_f32_tflops", dbl), ("mfma_f32_32_tflops", dbl), ("mfma_f64_tflops", dbl),
+                ("exp_f32_per_s", dbl), ("hbm_copy_gbs", dbl)]
+
+
 # name -> (restype, argtypes); every function declared in include/kevd.h
 SIGNATURES = {
     "kevd_create": (C.c_int, [C.c_int, P(C.c_void_p)]),
@@ -77,6 +82,7 @@ SIGNATURES = {
     "kevd_ensemble_arpc": (C.c_int, [P(C.c_void_p), i32, P(ArpcParams), P(ArpcResult), P(i32)]),
     "kevd_ensemble_evd": (C.c_int, [P(C.c_void_p), i32, P(EvdParams), P(EvdResult), i32]),
     "kevd_autotune_block": (C.c_int, [C.c_void_p, P(ArpcParams), P(i32), i32, P(i32), P(dbl)]),
+    "kevd_measure_peaks": (C.c_int, [C.c_void_p, i64, P(Peaks)]),
 }
 
 _lib = None

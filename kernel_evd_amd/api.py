@@ -204,6 +204,13 @@ class Context:
                                                   C.byref(best), _p(rates, C.c_double) if measure else None))
         return (best.value, rates) if measure else best.value
 
+    def measure_peaks(self, copy_bytes=4 << 30):
+        """Sustained MFMA (FP32 16x16 / 32x32, FP64), v_exp_f32 and HBM copy
+        rates of this context's device (kevd_measure_peaks)."""
+        pk = L.Peaks()
+        self._check(self._lib.kevd_measure_peaks(self._h, int(copy_bytes), C.byref(pk)))
+        return {name: getattr(pk, name) for name, _ in L.Peaks._fields_}
+
     # ------------------------------------------------------------ timing
     def set_profiling(self, on=True):
         self._check(self._lib.kevd_set_profiling(self._h, 1 if on else 0))

@@ -24,7 +24,7 @@ LIB = os.path.join(PKG, "libkevd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["ctx.hip", "arpc.hip", "factor.hip", "evd.hip", "bandwidth.hip", "ensemble.hip"]
+HIP_SOURCES = ["ctx.hip", "arpc.hip", "factor.hip", "evd.hip", "bandwidth.hip", "ensemble.hip", "peaks.hip"]
 CPP_SOURCES = ["ksgen.cpp"]
 DRIVERS = ["kevd_gen.cpp", "kevd_arpc.cpp", "kevd_evd.cpp", "kevd_bandwidth.cpp"]
 
@@ -34,6 +34,9 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + INC, "-I" + CSRC,
 ]
+# per-file device flags: the bandwidth sweep feeds MFMA results straight to
+# v_exp_f32, so its MFMAs write ordinary VGPRs (no accumulator-register reads)
+FILE_FLAGS = {"bandwidth.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-I" + INC, "-I" + CSRC,
              "-Wall", "-Wextra"]
 
@@ -50,7 +53,7 @@ def _compile(src):
     name = os.path.basename(src)
     obj = os.path.join(OBJ, name + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj]
+        cmd = [HIPCC] + HIP_FLAGS + FILE_FLAGS.get(name, []) + ["-c", src, "-o", obj]
     else:
         cmd = ["g++"] + CXX_FLAGS + ["-c", src, "-o", obj]
     _run(cmd)

@@ -6,8 +6,18 @@
 // bandwidth (kernelsum_comp 276-325, launched Nbw times at 566-572); here a
 // block computes a 128 x 128 distance tile on MFMA (|x|^2 + |y|^2 - 2 x.y,
 // FP64 norms), keeps it in registers and evaluates every eps_j against it.
-// Per (tile, 16 eps) the lane partials are summed across the wave and added
-// into FP64 accumulators; blocks write FP64 partials reduced in fixed order.
+//
+// FP32: the exponent arguments d * c_j (c_j = -log2(e) / eps_j) are MFMA
+// outer products, so the vector ALU issues only v_exp_f32 and one add per
+// evaluation.  A v_mfma_f32_32x32x2_f32 takes one distance register of the
+// wave as its A operand (A[i][k] = lane i + 32 k) and 32 coefficients as B,
+// with B zero in the other half of the lanes, so C[i][j] = d(lane i [+32])
+// * c_j: 32 distances x 32 eps per instruction.  Lane l receives only
+// column j = l % 32 of C, i.e. one eps per lane and set of 32, so each lane
+// sums its own eps over its 16 outputs; per tile the FP32 partials of the
+// two lanes of an eps are added into FP64 sums in LDS, and the waves and
+// blocks are reduced once at the end in fixed order.
+// FP64: the same tile, exp(d * c_j) per value, FP64 sums per chunk of 16 eps.
 #include <cmath>
 #include <vector>
 
@@ -17,21 +27,24 @@
 namespace {
 
 constexpr int NEPS_MAX = 128;
-constexpr int ECH = 16;  // eps values per register chunk
+constexpr int ECH = 16;      // FP64: eps values per register chunk
+constexpr int EPS_SET = 32;  // FP32: eps per outer-product set
+constexpr int NSET_MAX = NEPS_MAX / EPS_SET;
 
-template <typename T>
-__device__ inline T kexp(T d, T c);
-template <>
-__device__ inline float kexp<float>(float d, float c)
+// Distance of tile entry (a, b, v) of g, as T for the exp stage: FP64
+// norms minus twice the MFMA dot product, clamped at zero, exactly zero for
+// a row's own column; entries outside the rows / columns get `pad`.
+template <typename G, typename T>
+__device__ inline T tile_dist(const G& g, int a, int b, int v, int64_t r0, int64_t c0, int64_t nrows,
+                              int64_t cend, const double* nR, const double* nrm, const int64_t* ridx, T pad)
 {
-    // c = -log2(e) / eps; v_exp_f32 computes 2^x
-    return __builtin_amdgcn_exp2f(d * c);
-}
-template <>
-__device__ inline double kexp<double>(double d, double c)
-{
-    // c = -1 / eps
-    return exp(d * c);
+    int64_t row = r0 + g.row_of(a, v);
+    int64_t col = c0 + g.col_of(b);
+    if (row >= nrows || col >= cend) return pad;
+    double Dd = nR[row] + nrm[col] - 2.0 * (double)g.acc[a][b][v];
+    Dd = Dd > 0.0 ? Dd : 0.0;
+    if (ridx[row] == col) Dd = 0.0;
+    return (T)Dd;
 }
 
 template <typename T>
@@ -61,18 +74,9 @@ __global__ __launch_bounds__(256) void k_bw_sweep(const T* __restrict__ XR, cons
 #pragma unroll
             for (int b = 0; b < G::NTN; ++b)
 #pragma unroll
-                for (int v = 0; v < G::M::ACC; ++v) {
-                    int64_t row = r0 + g.row_of(a, v);
-                    int64_t col = c0 + g.col_of(b);
-                    T dv = (T)INFINITY;  // exp(-inf) = 0 for padding
-                    if (row < nrows && col < cend) {
-                        double Dd = nR[row] + nrm[col] - 2.0 * (double)g.acc[a][b][v];
-                        Dd = Dd > 0.0 ? Dd : 0.0;
-                        if (ridx[row] == col) Dd = 0.0;
-                        dv = (T)Dd;
-                    }
-                    D[(a * G::NTN + b) * G::M::ACC + v] = dv;
-                }
+                for (int v = 0; v < G::M::ACC; ++v)
+                    D[(a * G::NTN + b) * G::M::ACC + v] =
+                        tile_dist(g, a, b, v, r0, c0, nrows, cend, nR, nrm, ridx, (T)INFINITY);
         for (int e0 = 0; e0 < neps; e0 += ECH) {
             T s[ECH];
 #pragma unroll
@@ -84,7 +88,7 @@ __global__ __launch_bounds__(256) void k_bw_sweep(const T* __restrict__ XR, cons
             for (int q = 0; q < NV; ++q) {
                 T d = D[q];
 #pragma unroll
-                for (int j = 0; j < ECH; ++j) s[j] += kexp<T>(d, c[j]);
+                for (int j = 0; j < ECH; ++j) s[j] += exp(d * c[j]);  // c = -1 / eps
             }
 #pragma unroll
             for (int j = 0; j < ECH; ++j) {
@@ -101,6 +105,245 @@ __global__ __launch_bounds__(256) void k_bw_sweep(const T* __restrict__ XR, cons
     double* out = part + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * NEPS_MAX;
     for (int j = threadIdx.x; j < NEPS_MAX; j += 256)
         out[j] = (acc64[0][j] + acc64[1][j]) + (acc64[2][j] + acc64[3][j]);
+}
+
+// bcoef: per set s of 32 eps, two 64-lane B operands: [2s] holds c_{32s+l}
+// in lanes l < 32 and 0 above, [2s+1] holds 0 below and c_{32s+l-32} above.
+// Padding distances are FLT_MAX: times a zero B entry that is exactly 0, and
+// times any |c| >= 1e-30 (eps <= 1.4e30, checked on the host) it rounds to
+// -inf or below -3e8, whose exp2 is 0.
+__global__ __launch_bounds__(256) void k_bw_sweep_f32(const float* __restrict__ XR, const double* __restrict__ nR,
+                                                      const int64_t* __restrict__ ridx, int64_t nrows,
+                                                      const float* __restrict__ X, const double* __restrict__ nrm,
+                                                      int64_t n, int32_t dp, const float* __restrict__ bcoef,
+                                                      int nset, int64_t cols_per_block, double* part)
+{
+    typedef Cfg<float>::G G;
+    constexpr int BM = Cfg<float>::BM, BN = Cfg<float>::BN;
+    constexpr int NV = G::NTM * G::NTN * G::M::ACC;  // distances per lane
+    __shared__ __attribute__((aligned(16))) float lds[G::LDS_ELEMS];
+    // FP64 sums of eps 32 s + m per wave; 4 KB, so that two blocks fit a CU
+    __shared__ double lacc[4][NSET_MAX][EPS_SET];
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane < EPS_SET)
+        for (int s = 0; s < NSET_MAX; ++s) lacc[w][s][lane] = 0.0;
+    int64_t r0 = (int64_t)blockIdx.x * BM;
+    int64_t cbeg = (int64_t)blockIdx.y * cols_per_block;
+    int64_t cend = cbeg + cols_per_block < n ? cbeg + cols_per_block : n;
+    const floatx16 zero = {};
+    G g;
+    for (int64_t c0 = cbeg; c0 < cend; c0 += BN) {
+        g.init();
+        g.mainloop(lds, XR + r0 * dp, dp, nrows - r0, X + c0 * dp, dp, cend - c0, dp);
+        float D[NV];
+#pragma unroll
+        for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+            for (int b = 0; b < G::NTN; ++b)
+#pragma unroll
+                for (int v = 0; v < G::M::ACC; ++v)
+                    D[(a * G::NTN + b) * G::M::ACC + v] =
+                        tile_dist(g, a, b, v, r0, c0, nrows, cend, nR, nrm, ridx, 3.402823466e38f);
+        for (int s = 0; s < nset; ++s) {
+            const float blo = bcoef[(2 * s) * 64 + lane], bhi = bcoef[(2 * s + 1) * 64 + lane];
+            float ps0 = 0.f, ps1 = 0.f, ps2 = 0.f, ps3 = 0.f;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                floatx16 lo = __builtin_amdgcn_mfma_f32_32x32x2f32(D[q], blo, zero, 0, 0, 0);
+                floatx16 hi = __builtin_amdgcn_mfma_f32_32x32x2f32(D[q], bhi, zero, 0, 0, 0);
+#pragma unroll
+                for (int u = 0; u < 16; u += 4) {
+                    ps0 += __builtin_amdgcn_exp2f(lo[u]);
+                    ps1 += __builtin_amdgcn_exp2f(lo[u + 1]);
+                    ps2 += __builtin_amdgcn_exp2f(lo[u + 2]);
+                    ps3 += __builtin_amdgcn_exp2f(lo[u + 3]);
+                    ps0 += __builtin_amdgcn_exp2f(hi[u]);
+                    ps1 += __builtin_amdgcn_exp2f(hi[u + 1]);
+                    ps2 += __builtin_amdgcn_exp2f(hi[u + 2]);
+                    ps3 += __builtin_amdgcn_exp2f(hi[u + 3]);
+                }
+            }
+            // lanes m and m + 32 hold the same eps
+            double x = (double)((ps0 + ps1) + (ps2 + ps3));
+            x += __shfl_xor(x, 32, 64);
+            if (lane < EPS_SET) lacc[w][s][lane] += x;
+        }
+        // the next mainloop overwrites the LDS stage; its first barrier
+        // orders that after every wave's last read of this tile
+    }
+    __syncthreads();
+    double* out = part + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * NEPS_MAX;
+    for (int j = threadIdx.x; j < NEPS_MAX; j += 256) {
+        int s = j / EPS_SET, m = j % EPS_SET;
+        out[j] = s < nset ? (lacc[0][s][m] + lacc[1][s][m]) + (lacc[2][s][m] + lacc[3][s][m]) : 0.0;
+    }
+}
+
+// FP32 sweep for dp <= 64 (dp = DPT): the row block's A fragments stay in
+// registers for the whole launch, and each 128-column tile of X is loaded
+// into registers one tile ahead (issued before the exp stage of the current
+// tile, stored to the other LDS buffer after it), so the distance product
+// costs LDS reads and 4 DPT MFMAs per wave instead of a pipelined GEMM with
+// exposed global latency.  Distances, layout and the exp stage are those of
+// k_bw_sweep_f32 (16x16x4 tiles: wave w owns rows 32 w .. 32 w + 31, lane
+// (li, h) = (lane % 16, lane / 16) holds rows 16 t + 4 h + v and columns
+// 16 y + li of the tile).
+template <int DPT>
+__global__ __launch_bounds__(256) void k_bw_sweep_f32r(const float* __restrict__ XR, const double* __restrict__ nR,
+                                                       const int64_t* __restrict__ ridx, int64_t nrows,
+                                                       const float* __restrict__ X, const double* __restrict__ nrm,
+                                                       int64_t n, const float* __restrict__ bcoef, int nset,
+                                                       int64_t cols_per_block, double* part)
+{
+    constexpr int BM = 128, BN = 128, PB = DPT + 4, NS = DPT / 16;
+    constexpr int Q4 = DPT / 4;               // float4 per point
+    constexpr int NLD = BN * Q4 / 256;        // float4 loads per thread per tile
+    constexpr int NV = 2 * 8 * 4;             // distances per lane
+    static_assert(DPT % 16 == 0 && (BN * Q4) % 256 == 0, "tile shape");
+    __shared__ __attribute__((aligned(16))) float Bs[2][BN * PB];
+    __shared__ double lacc[4][NSET_MAX][EPS_SET];
+    __shared__ float bsh[NSET_MAX * 2][64];  // B operands of the products (LDS: no vmcnt waits)
+    __shared__ double rnorm[BM];             // the block's row norms and own columns
+    __shared__ int64_t rself[BM];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, h = lane >> 4;
+    if (lane < EPS_SET)
+        for (int s = 0; s < NSET_MAX; ++s) lacc[w][s][lane] = 0.0;
+    for (int i = threadIdx.x; i < nset * 2 * 64; i += 256) bsh[i / 64][i % 64] = bcoef[i];
+    const int64_t r0 = (int64_t)blockIdx.x * BM;
+    if (threadIdx.x < BM) {
+        int64_t row = r0 + threadIdx.x;
+        bool ok = row < nrows;
+        rnorm[threadIdx.x] = ok ? nR[row] : 0.0;
+        rself[threadIdx.x] = ok ? ridx[row] : -1;
+    }
+    const int64_t cbeg = (int64_t)blockIdx.y * cols_per_block;
+    const int64_t cend = cbeg + cols_per_block < n ? cbeg + cols_per_block : n;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    // Rows and columns past the ends are read from the last valid row /
+    // column (clamped addresses, no selects that would force a wait on the
+    // loads); the distance epilogue masks them.
+    float4 a[2][NS];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        int64_t row = r0 + w * 32 + t * 16 + li;
+        row = row < nrows ? row : nrows - 1;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) a[t][s] = *reinterpret_cast<const float4*>(XR + row * DPT + s * 16 + h * 4);
+    }
+    // next tile's points: float4 f = threadIdx.x + 256 i of the tile is
+    // point f / Q4, components 4 (f % Q4) ..
+    floatx4 pre[NLD];
+#define BW_GLOAD(C0)                                                                              \
+    _Pragma("unroll") for (int i = 0; i < NLD; ++i)                                               \
+    {                                                                                             \
+        int f = threadIdx.x + 256 * i, k4 = f % Q4;                                               \
+        int64_t col = (C0) + f / Q4;                                                              \
+        col = col < cend ? col : cend - 1;                                                        \
+        pre[i] = *reinterpret_cast<const floatx4*>(X + col * DPT + k4 * 4);                       \
+    }
+#define BW_SSTORE(BUF)                                                                            \
+    _Pragma("unroll") for (int i = 0; i < NLD; ++i)                                               \
+    {                                                                                             \
+        int f = threadIdx.x + 256 * i, col = f / Q4, k4 = f % Q4;                                 \
+        *reinterpret_cast<floatx4*>((BUF) + col * PB + k4 * 4) = pre[i];                          \
+    }
+    if (cbeg < cend) {
+        BW_GLOAD(cbeg)
+        BW_SSTORE(Bs[0])
+    }
+    const floatx16 zero = {};
+    int buf = 0;
+    for (int64_t c0 = cbeg; c0 < cend; c0 += BN, buf ^= 1) {
+        // tile c0 is in Bs[buf]; every wave has left the previous tile, so
+        // Bs[buf ^ 1] is free for the next one
+        __syncthreads();
+        const bool more = c0 + BN < cend;
+        floatx4 acc[2][8];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int y = 0; y < 8; ++y) acc[t][y] = floatx4{0.f, 0.f, 0.f, 0.f};
+        const float* B = Bs[buf];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            float4 b[8];
+#pragma unroll
+            for (int y = 0; y < 8; ++y)
+                b[y] = *reinterpret_cast<const float4*>(B + (y * 16 + li) * PB + s * 16 + h * 4);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int y = 0; y < 8; ++y)
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        Mfma16f::mma(acc[t][y], Mfma16f::get(a[t][s], j), Mfma16f::get(b[y], j));
+        }
+        // distances: FP64 norms of the lane's 8 columns (clamped, loaded
+        // together) and rows (LDS); padding as in k_bw_sweep_f32
+        double ncol[8];
+#pragma unroll
+        for (int y = 0; y < 8; ++y) {
+            int64_t col = c0 + y * 16 + li;
+            ncol[y] = nrm[col < cend ? col : cend - 1];
+        }
+        float D[NV];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int rl = w * 32 + t * 16 + 4 * h + v;
+                const double nr = rnorm[rl];
+                const int64_t self = rself[rl];
+                const bool rok = r0 + rl < nrows;
+#pragma unroll
+                for (int y = 0; y < 8; ++y) {
+                    int64_t col = c0 + y * 16 + li;
+                    double Dd = nr + ncol[y] - 2.0 * (double)acc[t][y][v];
+                    Dd = Dd > 0.0 ? Dd : 0.0;
+                    Dd = self == col ? 0.0 : Dd;
+                    D[(t * 8 + y) * 4 + v] = rok && col < cend ? (float)Dd : 3.402823466e38f;
+                }
+            }
+        // the next tile's loads come after the epilogue's, so that no wait
+        // on those (in-order counter) waits on the prefetch too
+        if (more) {
+            BW_GLOAD(c0 + BN)
+        }
+        for (int s = 0; s < nset; ++s) {
+            const float blo = bsh[2 * s][lane], bhi = bsh[2 * s + 1][lane];
+            float ps0 = 0.f, ps1 = 0.f, ps2 = 0.f, ps3 = 0.f;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                floatx16 lo = __builtin_amdgcn_mfma_f32_32x32x2f32(D[q], blo, zero, 0, 0, 0);
+                floatx16 hi = __builtin_amdgcn_mfma_f32_32x32x2f32(D[q], bhi, zero, 0, 0, 0);
+#pragma unroll
+                for (int u = 0; u < 16; u += 4) {
+                    ps0 += __builtin_amdgcn_exp2f(lo[u]);
+                    ps1 += __builtin_amdgcn_exp2f(lo[u + 1]);
+                    ps2 += __builtin_amdgcn_exp2f(lo[u + 2]);
+                    ps3 += __builtin_amdgcn_exp2f(lo[u + 3]);
+                    ps0 += __builtin_amdgcn_exp2f(hi[u]);
+                    ps1 += __builtin_amdgcn_exp2f(hi[u + 1]);
+                    ps2 += __builtin_amdgcn_exp2f(hi[u + 2]);
+                    ps3 += __builtin_amdgcn_exp2f(hi[u + 3]);
+                }
+            }
+            double x = (double)((ps0 + ps1) + (ps2 + ps3));
+            x += __shfl_xor(x, 32, 64);
+            if (lane < EPS_SET) lacc[w][s][lane] += x;
+        }
+        if (more) {
+            BW_SSTORE(Bs[buf ^ 1])
+        }
+    }
+#undef BW_GLOAD
+#undef BW_SSTORE
+    __syncthreads();
+    double* out = part + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * NEPS_MAX;
+    for (int j = threadIdx.x; j < NEPS_MAX; j += 256) {
+        int s = j / EPS_SET, m = j % EPS_SET;
+        out[j] = s < nset ? (lacc[0][s][m] + lacc[1][s][m]) + (lacc[2][s][m] + lacc[3][s][m]) : 0.0;
+    }
 }
 
 __global__ void k_bw_reduce(const double* part, int nblk, double* S)
@@ -128,11 +371,23 @@ kevd_status sweep(kevd_ctx* ctx, const int64_t* ri, int64_t nrows, const double*
     KEVD_TRY(ctx_buf_t(ctx, "bw_nR", nrows, &nR));
     KEVD_TRY(kblock_gather(ctx, ri, nrows, f32 ? KEVD_MIXED : KEVD_FP64, XR, nR));
     int nep = (neps + ECH - 1) / ECH * ECH;
-    std::vector<T> hc((size_t)nep, (T)0);
-    for (int j = 0; j < neps; ++j)
-        hc[j] = f32 ? (T)(-1.4426950408889634074 / eps[j]) : (T)(-1.0 / eps[j]);
-    KEVD_TRY(ctx_buf_t(ctx, "bw_coef", nep, &coef));
-    KEVD_HIP(ctx, hipMemcpyAsync(coef, hc.data(), nep * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    int nset = (neps + EPS_SET - 1) / EPS_SET;
+    // FP64: the coefficients padded to whole chunks of ECH; FP32: the B
+    // operands of the outer-product sets (see k_bw_sweep_f32)
+    size_t ncoef = f32 ? (size_t)nset * 2 * 64 : (size_t)nep;
+    std::vector<T> hc(ncoef, (T)0);
+    for (int j = 0; j < neps; ++j) {
+        if (f32) {
+            float c = (float)(-1.4426950408889634074 / eps[j]);
+            int st = j / EPS_SET, m = j % EPS_SET;
+            hc[(size_t)(2 * st) * 64 + m] = (T)c;
+            hc[(size_t)(2 * st + 1) * 64 + 32 + m] = (T)c;
+        } else {
+            hc[j] = (T)(-1.0 / eps[j]);
+        }
+    }
+    KEVD_TRY(ctx_buf_t(ctx, "bw_coef", ncoef, &coef));
+    KEVD_HIP(ctx, hipMemcpyAsync(coef, hc.data(), ncoef * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
     int64_t rt = (nrows + BM - 1) / BM;
     int64_t ctiles = (n + BN - 1) / BN;
     int64_t want = 4096 / (rt > 0 ? rt : 1);
@@ -145,8 +400,18 @@ kevd_status sweep(kevd_ctx* ctx, const int64_t* ri, int64_t nrows, const double*
     KEVD_TRY(ctx_buf_t(ctx, "bw_S", NEPS_MAX, &S));
     const T* X = f32 ? (const T*)ctx->X32 : (const T*)ctx->X64;
     const double* nrm = f32 ? ctx->nrm32 : ctx->nrm64;
-    k_bw_sweep<T><<<dim3((unsigned)rt, (unsigned)ncs), 256, 0, ctx->stream>>>(XR, nR, ri, nrows, X, nrm, n, dp,
-                                                                            coef, nep, cpb, part);
+    dim3 grid((unsigned)rt, (unsigned)ncs);
+    if constexpr (sizeof(T) == 4) {
+        const float *XRf = (const float*)XR, *Xf = (const float*)X, *cf = (const float*)coef;
+        if (dp == 32)
+            k_bw_sweep_f32r<32><<<grid, 256, 0, ctx->stream>>>(XRf, nR, ri, nrows, Xf, nrm, n, cf, nset, cpb, part);
+        else if (dp == 64)
+            k_bw_sweep_f32r<64><<<grid, 256, 0, ctx->stream>>>(XRf, nR, ri, nrows, Xf, nrm, n, cf, nset, cpb, part);
+        else
+            k_bw_sweep_f32<<<grid, 256, 0, ctx->stream>>>(XRf, nR, ri, nrows, Xf, nrm, n, dp, cf, nset, cpb, part);
+    }
+    else
+        k_bw_sweep<T><<<grid, 256, 0, ctx->stream>>>(XR, nR, ri, nrows, X, nrm, n, dp, coef, nep, cpb, part);
     k_bw_reduce<<<1, NEPS_MAX, 0, ctx->stream>>>(part, nblk, S);
     KEVD_HIP(ctx, hipGetLastError());
     KEVD_HIP(ctx, hipMemcpyAsync(S_out, S, neps * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -164,8 +429,8 @@ extern "C" kevd_status kevd_bandwidth_sweep(kevd_ctx* ctx, const int64_t* rows, 
     if (!rows || nrows < 1 || !eps || neps < 1 || neps > NEPS_MAX || !S_out)
         return kevd_fail(ctx, KEVD_EINVAL, "bandwidth: need rows, 1 <= neps <= 128 and outputs");
     for (int32_t j = 0; j < neps; ++j)
-        if (!(eps[j] > 0.0) || !std::isfinite(eps[j]))
-            return kevd_fail(ctx, KEVD_EINVAL, "bandwidth: eps values must be positive and finite");
+        if (!(eps[j] > 0.0) || !(eps[j] <= 1e30))
+            return kevd_fail(ctx, KEVD_EINVAL, "bandwidth: eps values must lie in (0, 1e30]");
     for (int64_t i = 0; i < nrows; ++i)
         if (rows[i] < 0 || rows[i] >= ctx->n)
             return kevd_fail(ctx, KEVD_EINVAL, "bandwidth: row index out of range");

@@ -17,7 +17,7 @@ def _grid(med, lo=1e-3, hi=1e3, k=64):
     return med * np.logspace(np.log10(lo), np.log10(hi), k)
 
 
-@pytest.mark.parametrize("n,dim,nrows", [(1000, 8, 130), (3001, 64, 257), (517, 3, 517)])
+@pytest.mark.parametrize("n,dim,nrows", [(1000, 8, 130), (3001, 64, 257), (517, 3, 517), (700, 100, 64), (1500, 40, 200)])
 def test_sweep_matches_oracle(gpu_ctx, n, dim, nrows):
     X = clustered_points(n, dim, seed=n)
     gpu_ctx.set_points(X)
