@@ -40,8 +40,8 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per launch at this workload (rocprofv3 FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, the factors a cold 2 GiB copy gives), from
 # profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md.
-TRAFFIC_BYTES = {"k_downdate<float>": 4.620e9, "k_gram<float,1>": 102.23e9,
-                 "k_sinkhorn_pass<float,2>": 8.511e9}
+TRAFFIC_BYTES = {"k_downdate<float>": 4.285e9, "k_gram<float,1>": 123.27e9,
+                 "k_sinkhorn_rows<float,8>": 8.507e9}
 TRAFFIC_SOURCE = "profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md"
 FP32_MFMA_PEAK_TF = 157.3
 FP64_MFMA_PEAK_TF = 78.6
@@ -362,6 +362,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the uneven-size ensemble leg")
     ap.add_argument("--no-bandwidth", action="store_true", help="skip the bandwidth-sweep leg")
+    ap.add_argument("--bandwidth-only", action="store_true", help="run only the bandwidth-sweep leg")
     ap.add_argument("--ensemble-only", action="store_true", help="run only the ensemble leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run one cold FP32 copy of MIB MiB first (PMC byte-counter calibration)")
@@ -377,6 +378,12 @@ def main():
         copy_calibration(args.copy_calibration)
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_leg(norm=args.norm)}), flush=True)
+        return
+    if args.bandwidth_only:
+        # the first 262,144 snapshots of the trajectory are the 262,144-point generation
+        with Context(0) as c:
+            pk = c.measure_peaks()
+        print(json.dumps({"c1_bandwidth": c1_leg(ks_data(262144), pk, with_cpu=not args.no_cpu)}), flush=True)
         return
     X = ks_data(args.n)
     X32 = X.astype(np.float32)
@@ -468,21 +475,22 @@ def main():
                            "mfma_f64_tflops": round(peaks["mfma_f64_tflops"], 2),
                            "exp_f32_per_s": peaks["exp_f32_per_s"],
                            "hbm_copy_gbs": round(peaks["hbm_copy_gbs"], 1),
+                           "hbm_read_gbs": round(peaks["hbm_read_gbs"], 1),
                            "how": "kevd_measure_peaks: register-only MFMA / v_exp_f32 chains on every CU, "
-                                  "4 GiB device copy (read + written bytes)"},
+                                  "4 GiB device copy (read + written bytes) and 4 GiB streaming read"},
     }
     if norm == "sinkhorn":
         # F^T 1 once, then one fused pass over F per Sinkhorn iteration
         mv_bytes = float(args.n) * args.rank * 4 * (1 + max(1, e_mix.sinkhorn_iters))
         mv_ms = agg["matvec_ms"] / steps
         if mv_ms > 0:
-            out["matvec_roofline"] = {"bound": "hbm", "kernel": "k_sinkhorn_pass<float,2> + k_colsum",
+            out["matvec_roofline"] = {"bound": "hbm", "kernel": "k_sinkhorn_rows<float,8> + k_colsum",
                                       "achieved": round(mv_bytes / (mv_ms * 1e-3) / 1e9, 1),
                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                       "frac": round(mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                                      "frac_of_measured_copy": round(mv_bytes / (mv_ms * 1e-3) / 1e9
-                                                                     / peaks["hbm_copy_gbs"], 4),
-                                      "traffic_per_pass": TRAFFIC_BYTES["k_sinkhorn_pass<float,2>"]
+                                      "frac_of_measured_read": round(mv_bytes / (mv_ms * 1e-3) / 1e9
+                                                                     / peaks["hbm_read_gbs"], 4),
+                                      "traffic_per_pass": TRAFFIC_BYTES["k_sinkhorn_rows<float,8>"]
                                       if args.n == 1 << 20 and args.rank == 2000 else None}
 
     extras = not args.no_extras

@@ -280,13 +280,15 @@ kevd_status kevd_autotune_block(kevd_ctx* ctx, const kevd_arpc_params* base, con
  * reference, which reports none): the rate of each instruction or access
  * pattern a hot kernel is bound by, sustained over every CU of the
  * context's device.  copy_bytes in [16 MiB, 64 GiB] sizes the HBM copy
- * (two buffers of that size are allocated and freed inside the call). */
+ * and read (two buffers of that size are allocated and freed inside the
+ * call). */
 typedef struct {
     double mfma_f32_tflops;     /* v_mfma_f32_16x16x4_f32, TFLOP/s          */
     double mfma_f32_32_tflops;  /* v_mfma_f32_32x32x2_f32, TFLOP/s          */
     double mfma_f64_tflops;     /* v_mfma_f64_16x16x4_f64, TFLOP/s          */
     double exp_f32_per_s;       /* v_exp_f32 results per second            */
     double hbm_copy_gbs;        /* device copy, read + written GB/s        */
+    double hbm_read_gbs;        /* streaming read of one buffer, GB/s      */
 } kevd_peaks;
 kevd_status kevd_measure_peaks(kevd_ctx* ctx, int64_t copy_bytes, kevd_peaks* out);
 

@@ -346,13 +346,22 @@ __global__ __launch_bounds__(256) void k_bw_sweep_f32r(const float* __restrict__
     }
 }
 
-__global__ void k_bw_reduce(const double* part, int nblk, double* S)
+// S[j] = sum over blocks of part[b][j]: one workgroup per eps, a strided
+// sum per thread and a fixed-shape tree, so the result does not depend on
+// scheduling.
+__global__ __launch_bounds__(256) void k_bw_reduce(const double* part, int nblk, double* S)
 {
-    int j = threadIdx.x;
-    if (j >= NEPS_MAX) return;
+    __shared__ double red[256];
+    const int j = blockIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * NEPS_MAX + j];
-    S[j] = s;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += part[(int64_t)b * NEPS_MAX + j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S[j] = red[0];
 }
 
 template <typename T>
@@ -412,7 +421,7 @@ kevd_status sweep(kevd_ctx* ctx, const int64_t* ri, int64_t nrows, const double*
     }
     else
         k_bw_sweep<T><<<grid, 256, 0, ctx->stream>>>(XR, nR, ri, nrows, X, nrm, n, dp, coef, nep, cpb, part);
-    k_bw_reduce<<<1, NEPS_MAX, 0, ctx->stream>>>(part, nblk, S);
+    k_bw_reduce<<<neps, 256, 0, ctx->stream>>>(part, nblk, S);
     KEVD_HIP(ctx, hipGetLastError());
     KEVD_HIP(ctx, hipMemcpyAsync(S_out, S, neps * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -13,6 +13,9 @@
 //             negation is a free source modifier, the value stays in (0, 1]).
 //   HBM copy  a 16-byte-per-lane grid-stride copy of a buffer far larger
 //             than the L2 / MALL; read + written bytes per second.
+//   HBM read  the same grid-stride pattern reading only (an XOR of the
+//             words per lane is written once), the ceiling of the
+//             read-dominated streaming passes.
 #include "internal.h"
 
 namespace {
@@ -91,6 +94,17 @@ __global__ __launch_bounds__(PK_THREADS) void k_peak_copy(const uint4* __restric
     for (int64_t i = (int64_t)blockIdx.x * PK_THREADS + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+__global__ __launch_bounds__(PK_THREADS) void k_peak_read(const uint4* __restrict__ src, int64_t n16, uint32_t* out)
+{
+    int64_t stride = (int64_t)gridDim.x * PK_THREADS;
+    uint32_t x = 0;
+    for (int64_t i = (int64_t)blockIdx.x * PK_THREADS + threadIdx.x; i < n16; i += stride) {
+        uint4 v = src[i];
+        x ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    out[blockIdx.x * PK_THREADS + threadIdx.x] = x;
+}
+
 __global__ void k_peak_fill(uint4* p, int64_t n16)
 {
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -166,10 +180,18 @@ extern "C" kevd_status kevd_measure_peaks(kevd_ctx* ctx, int64_t copy_bytes, kev
     kevd_status st = best_time(ctx, 5, [&] {
         k_peak_copy<<<cblk, PK_THREADS, 0, s>>>((const uint4*)src, (uint4*)dst, n16);
     }, &t);
+    double tr = 0.0;
+    if (st == KEVD_OK) {
+        out->hbm_copy_gbs = 2.0 * (double)n16 * 16.0 / t * 1e-9;
+        // sink: cblk * 256 words, inside the peak_sink buffer (nblk >= cblk)
+        st = best_time(ctx, 5, [&] {
+            k_peak_read<<<cblk, PK_THREADS, 0, s>>>((const uint4*)src, n16, (uint32_t*)sink);
+        }, &tr);
+    }
     (void)hipStreamSynchronize(s);
     (void)hipFree(src);
     (void)hipFree(dst);
     if (st != KEVD_OK) return st;
-    out->hbm_copy_gbs = 2.0 * (double)n16 * 16.0 / t * 1e-9;
+    out->hbm_read_gbs = (double)n16 * 16.0 / tr * 1e-9;
     return KEVD_OK;
 }

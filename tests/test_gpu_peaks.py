@@ -13,7 +13,7 @@ def test_measure_peaks_plausible(gpu_ctx):
     print("peaks", pk, flush=True)
     bounds = {"mfma_f32_tflops": (40.0, 157.3), "mfma_f32_32_tflops": (40.0, 157.3),
               "mfma_f64_tflops": (20.0, 78.6), "exp_f32_per_s": (1e12, 80e12),
-              "hbm_copy_gbs": (1500.0, 8000.0)}
+              "hbm_copy_gbs": (1500.0, 8000.0), "hbm_read_gbs": (1500.0, 8000.0)}
     for k, (lo, hi) in bounds.items():
         assert lo < pk[k] < 1.1 * hi, (k, pk[k])
 
