@@ -73,14 +73,50 @@ def build_library(jobs=8):
     return LIB
 
 
+def find_hdf5():
+    """(include dir, lib dir) of an HDF5 C library (the reference's file
+    format), or None: the drivers then read and write .npy only."""
+    for prefix in (os.environ.get("HDF5_DIR", ""), "/opt/conda", "/usr", "/usr/local"):
+        if not prefix:
+            continue
+        for inc in (os.path.join(prefix, "include"), os.path.join(prefix, "include", "hdf5", "serial")):
+            if not os.path.exists(os.path.join(inc, "hdf5.h")):
+                continue
+            for lib in (os.path.join(prefix, "lib"), os.path.join(prefix, "lib", "x86_64-linux-gnu"),
+                        os.path.join(prefix, "lib", "x86_64-linux-gnu", "hdf5", "serial")):
+                if os.path.exists(os.path.join(lib, "libhdf5.so")):
+                    return inc, lib
+    return None
+
+
 def build_drivers(jobs=8):
     os.makedirs(BIN, exist_ok=True)
-    common = os.path.join(DRV, "npy.cpp")
+    common = [os.path.join(DRV, "npy.cpp"), os.path.join(DRV, "arrays.cpp")]
+    h5 = find_hdf5()
+    h5flags = ([], [])
+    if h5:
+        # Link the library by path and resolve it at run time through a
+        # private directory holding only its soname: a search path to the
+        # whole prefix (conda's) would also shadow the system libstdc++
+        # that the ROCm libraries need.
+        import glob
+        import re
+        sonames = [f for f in glob.glob(os.path.join(h5[1], "libhdf5.so.*"))
+                   if re.fullmatch(r"libhdf5\.so\.\d+", os.path.basename(f))]
+        if sonames:
+            priv = os.path.join(BIN, "hdf5")
+            os.makedirs(priv, exist_ok=True)
+            link = os.path.join(priv, os.path.basename(sonames[0]))
+            if os.path.lexists(link):
+                os.remove(link)
+            os.symlink(os.path.realpath(sonames[0]), link)
+            h5flags = (["-DKEVD_HAVE_HDF5", "-isystem", h5[0]], [sonames[0], "-Wl,-rpath,$ORIGIN/hdf5"])
 
     def one(d):
         exe = os.path.join(BIN, os.path.splitext(d)[0].replace("_", "-"))
-        _run(["g++"] + CXX_FLAGS + ["-o", exe, os.path.join(DRV, d), common,
-              "-L" + PKG, "-lkevd", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+        _run(["g++"] + CXX_FLAGS + h5flags[0] + ["-o", exe, os.path.join(DRV, d)] + common +
+             ["-L" + PKG, "-lkevd", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + os.path.join(ROCM, "lib")] +
+             h5flags[1])
         return exe
 
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -3,11 +3,15 @@
 // L^-1 files out; the reference's per-round table on stdout
 // (ks_arpclm.cu:1129-1136).
 //
-//   kevd-arpc --points X.npy (--eps E | --eps-median [--median-pairs P])
+//   kevd-arpc --points X.npy|u.h5 [--delay Nq] [--grid-points M]
+//             (--eps E | --eps-median [--median-pairs P])
 //             --rank R [--block 128] [--seed 0] [--tol 1e-10]
-//             [--precision mixed|fp64] [--max-rounds K] [--inject-pivots S.npy]
-//             [--factor-out F.npy] [--pivots-out S.npy] [--linv-out Linv.npy]
+//             [--precision mixed|fp64] [--max-rounds K] [--inject-pivots S.npy|spiv.h5]
+//             [--factor-out F.npy] [--pivots-out S.npy|spiv.h5] [--linv-out Linv.npy|linv.h5]
 //             [--device 0]
+// File arguments are .npy, or the reference's HDF5 files (ks_arpclm.cu:
+// 628-644): the snapshot file's /u (delay embedded with --delay), and /spiv
+// (unsigned 64-bit) and /linv (FP64, r x r column-major, flattened) out.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -15,7 +19,7 @@
 
 #include "args.h"
 #include "kevd.h"
-#include "npy.h"
+#include "arrays.h"
 
 static int fail(kevd_ctx* c, kevd_status s, const char* what)
 {
@@ -28,33 +32,21 @@ int main(int argc, char** argv)
 {
     Args a(argc, argv);
     if (a.has("help")) {
-        std::printf("kevd-arpc --points X.npy (--eps E | --eps-median) --rank R [--block 128] [--seed 0]\n"
-                    "          [--tol 1e-10] [--precision mixed|fp64] [--max-rounds K]\n"
-                    "          [--inject-pivots S.npy] [--factor-out F.npy] [--pivots-out S.npy]\n"
-                    "          [--linv-out Linv.npy] [--median-pairs 4194304] [--device 0]\n");
+        std::printf("kevd-arpc --points X.npy|u.h5 [--delay Nq] [--grid-points M] (--eps E | --eps-median)\n"
+                    "          --rank R [--block 128] [--seed 0] [--tol 1e-10] [--precision mixed|fp64]\n"
+                    "          [--max-rounds K] [--inject-pivots S.npy|spiv.h5] [--factor-out F.npy]\n"
+                    "          [--pivots-out S.npy|spiv.h5] [--linv-out Linv.npy|linv.h5]\n"
+                    "          [--median-pairs 4194304] [--device 0]\n");
         return 0;
     }
     auto t0 = std::chrono::steady_clock::now();
-    NpyArray X;
-    try {
-        X = npy_load(a.need("points"));
-    } catch (const std::exception& e) {
-        std::fprintf(stderr, "%s\n", e.what());
-        return 1;
-    }
-    if (X.shape.size() != 2) {
-        std::fprintf(stderr, "--points must be a 2-D array\n");
-        return 1;
-    }
     kevd_ctx* c = nullptr;
     kevd_status s = kevd_create((int)a.integer("device", 0), &c);
     if (s != KEVD_OK) return fail(nullptr, s, "kevd_create");
-    if (X.dtype == "<f4")
-        s = kevd_set_points_f32(c, reinterpret_cast<const float*>(X.data.data()), X.rows(), (int32_t)X.cols());
-    else {
-        std::vector<double> x = npy_as_f64(X);
-        s = kevd_set_points(c, x.data(), X.rows(), (int32_t)X.cols());
-    }
+    int64_t npts = 0;
+    int32_t dim = 0;
+    s = (kevd_status)set_points_from(c, a.need("points"), a.integer("delay", 0), a.integer("grid-points", 0), &npts,
+                                     &dim);
     if (s != KEVD_OK) return fail(c, s, "kevd_set_points");
     double eps = a.num("eps", 0.0);
     if (a.has("eps-median") || !(eps > 0.0)) {
@@ -72,7 +64,13 @@ int main(int argc, char** argv)
     p.max_rounds = (int32_t)a.integer("max-rounds", 0);
     std::vector<int64_t> inj;
     if (a.has("inject-pivots")) {
-        inj = npy_as_i64(npy_load(a.str("inject-pivots")));
+        try {
+            inj = npy_as_i64(load_array(a.str("inject-pivots"), "/spiv"));
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "%s\n", e.what());
+            kevd_destroy(c);
+            return 1;
+        }
         p.inject_pivots = inj.data();
         p.n_inject = (int64_t)inj.size();
     }
@@ -86,7 +84,7 @@ int main(int argc, char** argv)
     r.trace_err = err.data();
     r.rounds_cap = mr;
     std::printf("Points: %lld x %lld  eps %.6g  rank %lld  block %d  seed %llu  precision %s\n\n",
-                (long long)X.rows(), (long long)X.cols(), eps, (long long)p.rank, p.block,
+                (long long)npts, (long long)dim, eps, (long long)p.rank, p.block,
                 (unsigned long long)p.seed, p.prec == KEVD_FP64 ? "fp64" : "mixed");
     s = kevd_arpc(c, &p, &r);
     if (s != KEVD_OK) return fail(c, s, "kevd_arpc");
@@ -97,20 +95,26 @@ int main(int argc, char** argv)
         cum += acc[i];
         std::printf("%d %d %lld %.5e\n", i, acc[i], cum, err[i]);
     }
-    if (a.has("factor-out")) {
-        bool f32 = p.prec == KEVD_MIXED;
-        std::vector<char> F((size_t)X.rows() * r.rank * (f32 ? 4 : 8));
-        s = kevd_get_factor(c, F.data(), r.rank, p.prec);
-        if (s != KEVD_OK) return fail(c, s, "kevd_get_factor");
-        npy_save(a.str("factor-out"), f32 ? "<f4" : "<f8", {X.rows(), r.rank}, F.data());
-    }
-    if (a.has("pivots-out")) npy_save(a.str("pivots-out"), "<i8", {r.rank}, piv.data());
-    if (a.has("linv-out")) {
-        std::vector<double> L((size_t)r.rank * r.rank);
-        s = kevd_get_linv(c, L.data());
-        if (s != KEVD_OK) return fail(c, s, "kevd_get_linv");
-        // column-major r x r, flattened (the reference's /linv layout)
-        npy_save(a.str("linv-out"), "<f8", {r.rank * r.rank}, L.data());
+    try {
+        if (a.has("factor-out")) {
+            bool f32 = p.prec == KEVD_MIXED;
+            std::vector<char> F((size_t)npts * r.rank * (f32 ? 4 : 8));
+            s = kevd_get_factor(c, F.data(), r.rank, p.prec);
+            if (s != KEVD_OK) return fail(c, s, "kevd_get_factor");
+            save_array(a.str("factor-out"), f32 ? "<f4" : "<f8", {npts, r.rank}, F.data(), "/F");
+        }
+        if (a.has("pivots-out")) save_array(a.str("pivots-out"), "<u8", {r.rank}, piv.data(), "/spiv");
+        if (a.has("linv-out")) {
+            std::vector<double> L((size_t)r.rank * r.rank);
+            s = kevd_get_linv(c, L.data());
+            if (s != KEVD_OK) return fail(c, s, "kevd_get_linv");
+            // column-major r x r, flattened (the reference's /linv layout)
+            save_array(a.str("linv-out"), "<f8", {r.rank * r.rank}, L.data(), "/linv");
+        }
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        kevd_destroy(c);
+        return 1;
     }
     kevd_timings tm;
     kevd_get_timings(c, &tm);

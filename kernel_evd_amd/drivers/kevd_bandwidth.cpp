@@ -4,7 +4,8 @@
 //   "%.5e %.5e %.5e" (bandwidth, log sum, slope) for i = 1..n-2,
 //   a blank line, eopt, mdim.
 //
-//   kevd-bandwidth --points X.npy [--rows 16384 [--row-seed 0] | --all-pairs]
+//   kevd-bandwidth --points X.npy|u.h5 [--delay Nq] [--grid-points M]
+//                  [--rows 16384 [--row-seed 0] | --all-pairs]
 //                  [--grid log [--lo 1e-3] [--hi 1e3] [--count 64] | --grid ref36]
 //                  [--scale median|<value>] [--precision mixed|fp64]
 //                  [--out bw0.out] [--sums-out S.npy] [--device 0]
@@ -20,7 +21,7 @@
 
 #include "args.h"
 #include "kevd.h"
-#include "npy.h"
+#include "arrays.h"
 
 static uint64_t philox_u64(uint64_t seed, uint32_t stream, uint32_t round, uint32_t idx)
 {
@@ -49,29 +50,19 @@ int main(int argc, char** argv)
 {
     Args a(argc, argv);
     if (a.has("help")) {
-        std::printf("kevd-bandwidth --points X.npy [--rows 16384 [--row-seed 0] | --all-pairs]\n"
+        std::printf("kevd-bandwidth --points X.npy|u.h5 [--delay Nq] [--grid-points M]\n"
+                    "               [--rows 16384 [--row-seed 0] | --all-pairs]\n"
                     "               [--grid log [--lo 1e-3] [--hi 1e3] [--count 64] | --grid ref36]\n"
                     "               [--scale median|<value>] [--precision mixed|fp64] [--out bw0.out]\n");
         return 0;
     }
-    NpyArray X;
-    try {
-        X = npy_load(a.need("points"));
-    } catch (const std::exception& e) {
-        std::fprintf(stderr, "%s\n", e.what());
-        return 1;
-    }
-    int64_t n = X.rows();
-    int32_t dim = (int32_t)X.cols();
     kevd_ctx* c = nullptr;
     kevd_status s = kevd_create((int)a.integer("device", 0), &c);
     if (s != KEVD_OK) return fail(nullptr, s, "kevd_create");
-    if (X.dtype == "<f4")
-        s = kevd_set_points_f32(c, reinterpret_cast<const float*>(X.data.data()), n, dim);
-    else {
-        std::vector<double> x = npy_as_f64(X);
-        s = kevd_set_points(c, x.data(), n, dim);
-    }
+    int64_t n = 0;
+    int32_t dim = 0;
+    s = (kevd_status)set_points_from(c, a.need("points"), a.integer("delay", 0), a.integer("grid-points", 0), &n,
+                                     &dim);
     if (s != KEVD_OK) return fail(c, s, "kevd_set_points");
     // rows: all, or a Philox-keyed partial Fisher-Yates sample without replacement
     std::vector<int64_t> rows;

@@ -4,16 +4,21 @@
 //
 //   kevd-gen --n N [--L 22] [--M 64] [--dt 0.25] [--discard 10000]
 //            [--stride 4] [--seed 0] [--ic-scale 0.1] [--delay Nq] [--f32]
-//            --out X.npy
+//            --out X.npy|u.h5
 // Without --delay: N x M snapshots.  With --delay Nq: N + Nq - 1 snapshots
 // are generated and N*M points of dimension Nq written.
+// An .h5 output is the reference's snapshot file (ks_datagen.py:91-103): /u
+// holds the N snapshots flattened with the time index fastest (FP64, or
+// FP32 with --f32 as in its _u32 file) and /t their times (FP64); delay
+// embedding then happens where the file is read (--delay of the other
+// drivers), so --delay is not taken with an .h5 output.
 #include <chrono>
 #include <cstdio>
 #include <vector>
 
 #include "args.h"
 #include "kevd.h"
-#include "npy.h"
+#include "arrays.h"
 
 int main(int argc, char** argv)
 {
@@ -38,6 +43,11 @@ int main(int argc, char** argv)
         std::fprintf(stderr, "--n must be >= 1\n");
         return 1;
     }
+    const bool h5 = is_h5(out);
+    if (h5 && nq > 0) {
+        std::fprintf(stderr, "an .h5 output holds the snapshot series; give --delay to the driver that reads it\n");
+        return 1;
+    }
     int64_t nsnap = nq > 0 ? n + nq - 1 : n;
     std::vector<double> U((size_t)nsnap * p.M);
     auto t0 = std::chrono::steady_clock::now();
@@ -45,6 +55,29 @@ int main(int argc, char** argv)
     if (s != KEVD_OK) {
         std::fprintf(stderr, "kevd_ks_generate: %s\n", kevd_status_string(s));
         return exit_code(s);
+    }
+    if (h5) {
+        // u[g * n + t] = snapshot t at grid point g; t_i = (discard + i stride) dt
+        std::vector<double> u((size_t)n * p.M), t((size_t)n);
+        for (int64_t i = 0; i < n; ++i) {
+            t[(size_t)i] = (double)(p.n_discard + i * p.stride) * p.dt;
+            for (int32_t g = 0; g < p.M; ++g) u[(size_t)(g * n + i)] = U[(size_t)(i * p.M + g)];
+        }
+        try {
+            if (a.has("f32")) {
+                std::vector<float> f(u.begin(), u.end());
+                save_array(out, "<f4", {n * p.M}, f.data(), "/u", false);
+            } else {
+                save_array(out, "<f8", {n * p.M}, u.data(), "/u", false);
+            }
+            save_array(out, "<f8", {n}, t.data(), "/t", true);
+        } catch (const std::exception& e) {
+            std::fprintf(stderr, "%s\n", e.what());
+            return 1;
+        }
+        std::printf("wrote %s: /u %lld x %d (time fastest), /t %lld\n", out.c_str(), (long long)n, p.M,
+                    (long long)n);
+        return 0;
     }
     std::vector<double> X;
     int64_t rows = nsnap, cols = p.M;
@@ -60,9 +93,9 @@ int main(int argc, char** argv)
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (a.has("f32")) {
         std::vector<float> f(X.begin(), X.end());
-        npy_save(out, "<f4", {rows, cols}, f.data());
+        save_array(out, "<f4", {rows, cols}, f.data(), "/X");
     } else {
-        npy_save(out, "<f8", {rows, cols}, X.data());
+        save_array(out, "<f8", {rows, cols}, X.data(), "/X");
     }
     std::printf("wrote %s: %lld x %lld (%.2f s)\n", out.c_str(), (long long)rows, (long long)cols, secs);
     return 0;
