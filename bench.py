@@ -285,14 +285,15 @@ def c1_leg(X, peaks, n=262144, nrows=16384, neps=64, cpu_rows=256, with_cpu=True
             return S, best
 
         S32, t32 = timed(rows, eps, "mixed", 3)
-        _, t16 = timed(rows, eps[:16], "mixed", 3)
+        _, t1set = timed(rows, eps[:32], "mixed", 3)
         S64, t64 = timed(rows, eps, "fp64", 1)
         Ssub = c.bandwidth_sweep(rows[:cpu_rows], eps, "mixed")
     evals = float(nrows) * n * neps
     flop_dist = 2.0 * nrows * n * 64
-    # 16 eps are one register chunk; the 48 more cost (t32 - t16), so the
-    # distance tile alone is about t16 - (t32 - t16) / 3
-    t_dist = max(t16 - (t32 - t16) / 3.0, 1e-9)
+    # the FP32 sweep evaluates eps in sets of 32; the second set costs
+    # (t32 - t1set), so the part that does not scale with eps (distance
+    # tile, its epilogue, loads, reductions) is about 2 t1set - t32
+    t_fixed = max(2.0 * t1set - t32, 1e-9)
     _, sl32, eopt, mdim = bandwidth_post(eps, S32)
     _, sl64, eopt64, mdim64 = bandwidth_post(eps, S64)
     res = {"workload": "bandwidth sweep: %d seeded rows x N=%d KS snapshots (dim 64), %d eps log-spaced "
@@ -300,8 +301,9 @@ def c1_leg(X, peaks, n=262144, nrows=16384, neps=64, cpu_rows=256, with_cpu=True
            "ms": round(1e3 * t32, 3), "exp_evals": evals, "evals_per_s": evals / t32,
            "exp_peak_measured": peaks.get("exp_f32_per_s") if peaks else None,
            "frac_of_measured_exp_peak": round(evals / t32 / peaks["exp_f32_per_s"], 4) if peaks else None,
-           "ms_16_eps": round(1e3 * t16, 3), "distance_part_ms_est": round(1e3 * t_dist, 3),
-           "distance_mfma_frac_est": round(flop_dist / t_dist / 1e12 / FP32_MFMA_PEAK_TF, 4),
+           "ms_32_eps": round(1e3 * t1set, 3), "eps_independent_ms_est": round(1e3 * t_fixed, 3),
+           "distance_mfma_frac_of_eps_independent_part": round(flop_dist / t_fixed / 1e12 / FP32_MFMA_PEAK_TF, 4),
+           "eps_part_evals_per_s": evals / (t32 - t_fixed) if t32 > t_fixed else None,
            "fp64_ms": round(1e3 * t64, 3),
            "S_rel_err_vs_fp64_max": float(np.max(np.abs(S32 - S64) / S64)),
            "slope_abs_err_vs_fp64_max": float(np.max(np.abs(sl32 - sl64))),
