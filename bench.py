@@ -16,7 +16,9 @@ pivots injected, for the factor's relative Frobenius difference; the
 uneven-size ensemble (48 KS problems, batched ARPC + concurrent kEVD, against
 the same problems run one after another); the bandwidth sweep (16,384 rows x
 262,144 snapshots x 64 eps) against the measured v_exp_f32 rate and FP64;
-the CPU FP64 oracle on a bounded sample for the CPU baseline; and the small
+the LM form of ARPC (no factor stored, K(R,S) regenerated every round) on
+the same points; the CPU FP64 oracle on a bounded sample for the CPU
+baseline; and the small
 configuration (N = 8192, FP64, block size 1, rank 512) beside the dense
 NumPy reference EVD.  The device's sustained MFMA, exp and copy rates are
 measured once (kevd_measure_peaks) and each roofline fraction is given
@@ -326,6 +328,28 @@ def c1_leg(X, peaks, n=262144, nrows=16384, neps=64, cpu_rows=256, with_cpu=True
     return res
 
 
+def lm_leg(ctx, eps, rank, block, trace_err_ff, prec="mixed"):
+    """The LM form of ARPC (regenerate=1, the reference's own algorithm: S
+    and L^-1 kept, K(R,S) regenerated every round) on the same points,
+    rank, block and seed: its time, trace error against the factor form's,
+    the factor memory it does without, and how well its L^-1 inverts K(S,S)
+    (max |L^-1 K(S,S) L^-T - I|, every quantity on that path being FP64)."""
+    ctx.arpc(eps, rank, block=block, seed=0, prec=prec, regenerate=True)  # sizes the buffers
+    t0 = time.perf_counter()
+    a = ctx.arpc(eps, rank, block=block, seed=0, prec=prec, regenerate=True)
+    t = time.perf_counter() - t0
+    Li = ctx.get_linv(a.rank)
+    K = ctx.kernel_block(a.pivots, a.pivots, eps, prec="fp64")
+    ident = float(np.abs(Li @ K @ Li.T - np.eye(a.rank)).max())
+    res = {"s": round(t, 4), "rank": int(a.rank), "rounds": int(a.rounds), "trace_err": float(a.trace_err[-1]),
+           "trace_err_rel_diff_vs_factor_form": float(abs(a.trace_err[-1] - trace_err_ff) / trace_err_ff),
+           "factor_bytes_not_stored": int(ctx.n * ((rank + 63) // 64 * 64) * (4 if prec == "mixed" else 8)),
+           "linv_identity_err_max": ident}
+    log("LM ARPC: %.3f s, trace err %.3e (factor form %.3e), |L K L^T - I| %.1e"
+        % (t, a.trace_err[-1], trace_err_ff, ident))
+    return res
+
+
 def cpu_baseline(X, eps, rank, block, kout, n_sample, norm):
     """CPU FP64 oracle (OpenMP C ARPC + NumPy F-form kEVD) on the first
     n_sample points; both stages scale linearly in N (the Sinkhorn iteration
@@ -472,6 +496,16 @@ def main():
                           "frac": round(gr_flop / (gr_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4) if gr_ms > 0 else None,
                           "frac_of_measured": round(gr_flop / (gr_ms * 1e-3) / 1e12 / peaks["mfma_f32_tflops"], 4)
                           if gr_ms > 0 else None},
+        # the step's three GEMMs on the factor together: the downdate that
+        # builds F, the Gram F^T S^2 F (or both two-step Grams) and U = S F V
+        "factor_gemms": {"kernels": ["k_downdate<float>", "k_gram<float,%d>" % (1 if norm == "sinkhorn" else 2),
+                                     "k_ugemm<float>"],
+                         "flop": dd_flop + gr_flop + 2.0 * args.n * a_mix.rank * (args.kout + 1),
+                         "ms": round(dd_ms + gr_ms + agg["u_ms"] / steps, 3),
+                         "achieved": round((dd_flop + gr_flop + 2.0 * args.n * a_mix.rank * (args.kout + 1))
+                                           / ((dd_ms + gr_ms + agg["u_ms"] / steps) * 1e-3) / 1e12, 3)
+                         if dd_ms + gr_ms > 0 else None,
+                         "unit": "TFLOP/s", "peak": FP32_MFMA_PEAK_TF},
         "measured_peaks": {"mfma_f32_16x16x4_tflops": round(peaks["mfma_f32_tflops"], 2),
                            "mfma_f32_32x32x2_tflops": round(peaks["mfma_f32_32_tflops"], 2),
                            "mfma_f64_tflops": round(peaks["mfma_f64_tflops"], 2),
@@ -481,6 +515,8 @@ def main():
                            "how": "kevd_measure_peaks: register-only MFMA / v_exp_f32 chains on every CU, "
                                   "4 GiB device copy (read + written bytes) and 4 GiB streaming read"},
     }
+    fg = out["factor_gemms"]
+    fg["frac"] = round(fg["achieved"] / FP32_MFMA_PEAK_TF, 4) if fg["achieved"] else None
     if norm == "sinkhorn":
         # F^T 1 once, then one fused pass over F per Sinkhorn iteration
         mv_bytes = float(args.n) * args.rank * 4 * (1 + max(1, e_mix.sinkhorn_iters))
@@ -553,6 +589,9 @@ def main():
             del F32, F64
             log("factor rel Frobenius (injected pivots) %.3e" % out["factor_rel_frobenius_vs_fp64_injected"])
         ctx64.close()
+
+    if extras:
+        out["lm_arpc"] = lm_leg(ctx, eps, args.rank, args.block, float(a_mix.trace_err[-1]))
 
     if not args.no_ensemble:
         ctx.close()

@@ -101,6 +101,7 @@ typedef struct {
     int32_t max_rounds;       /* guard; <= 0 means 4*ceil(r/b)+64           */
     const int64_t* inject_pivots; /* optional fixed pivot list (host)       */
     int64_t n_inject;         /* its length (>= rank when given)            */
+    int32_t regenerate;       /* 1: the LM form (below); 0: store F         */
 } kevd_arpc_params;
 
 typedef struct {
@@ -119,6 +120,14 @@ typedef struct {
  * Rounds run until the rank reaches params.rank (the last accepted block is
  * truncated), the residual mass is zero, or max_rounds is reached. */
 kevd_status kevd_arpc(kevd_ctx* ctx, const kevd_arpc_params* p, kevd_arpc_result* res);
+/* regenerate = 1 runs the reference's own LM form (ks_arpclm.cu:793-1046:
+ * L^-1 kept, K(R,S) regenerated from the points every round, M and the
+ * appended L^-1 block row of :907-914 and :480-534) instead of storing F:
+ * device memory O(n + r^2) instead of n * r, for point sets whose factor does
+ * not fit.  Same rounds, sampling and rejection; pivots, accepted counts and
+ * trace errors come out as usual, L^-1 through kevd_get_linv; the context
+ * then holds no factor (kevd_get_factor and kevd_evd report EINVAL until a
+ * factor is set, e.g. by kevd_arpc with the pivots injected). */
 
 /* Copy the factor to the host: n x rank row-major with leading dimension ld
  * (>= rank), as FP64 (`as` = KEVD_FP64) or FP32 (`as` = KEVD_MIXED). */

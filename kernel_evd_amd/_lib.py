@@ -21,7 +21,7 @@ P = C.POINTER
 class ArpcParams(C.Structure):
     _fields_ = [("eps", dbl), ("rank", i64), ("block", i32), ("seed", u64), ("tol", dbl),
                 ("prec", C.c_int), ("max_rounds", i32), ("inject_pivots", P(i64)),
-                ("n_inject", i64)]
+                ("n_inject", i64), ("regenerate", i32)]
 
 
 class ArpcResult(C.Structure):

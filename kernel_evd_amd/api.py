@@ -102,8 +102,11 @@ class Context:
 
     # -------------------------------------------------------------- ARPC
     def arpc(self, eps, rank, block=128, seed=0, tol=1e-10, prec="mixed", max_rounds=0,
-             inject_pivots=None):
+             inject_pivots=None, regenerate=False):
+        """kevd_arpc.  regenerate=True runs the LM form (no factor stored;
+        L^-1 through get_linv)."""
         p = L.ArpcParams()
+        p.regenerate = 1 if regenerate else 0
         p.eps = eps
         p.rank = rank
         p.block = block

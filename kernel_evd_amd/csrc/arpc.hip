@@ -13,13 +13,21 @@
 // The per-round state (rank, accepted count, stop flag) lives on the device,
 // so a round issues no host synchronisation; the host polls a pinned copy of
 // the state two rounds behind and stops launching once the device says so.
+//
+// With params.regenerate the rounds run in the reference's own LM form (no
+// factor stored; F(S',:), the overlap and L^-1 regenerated or maintained
+// from the points, see arpc_kernels.h): run_rounds_lm below.
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <type_traits>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "arpc_kernels.h"
+
+rocblas_handle blas_handle(kevd_ctx* ctx);
 
 namespace {
 
@@ -101,6 +109,77 @@ __global__ void k_fill(double* p, int64_t n, double v)
 {
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------- LM variant
+
+__global__ __launch_bounds__(256) void k_lm_kprop(const double* __restrict__ X64, int32_t dp,
+                                                  const int64_t* __restrict__ Sprime, const int64_t* __restrict__ piv,
+                                                  const ArpcState* st, double neg_inv_eps, double* P, int64_t ldp)
+{
+    lm_kprop_body(X64, dp, Sprime, piv, st, neg_inv_eps, P, ldp, blockIdx.y,
+                  (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_lm_form_H(const double* __restrict__ X64, int32_t dp,
+                                                   const double* __restrict__ P2, int64_t ldp,
+                                                   const int64_t* __restrict__ Sprime, const ArpcState* st,
+                                                   double neg_inv_eps, double* H)
+{
+    form_H_body<double>(X64, dp, P2, ldp, Sprime, st, neg_inv_eps, H, blockIdx.x, blockIdx.y, true);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lm_rejchol(const double* __restrict__ H, ArpcState* st, uint64_t seed,
+                                                    double tol, int inject, const int64_t* __restrict__ Sprime,
+                                                    int64_t* Si, T* LinvT, double* Linv64, int32_t* Spos)
+{
+    rejchol_body<T>(H, st, seed, tol, inject, Sprime, Si, LinvT, Linv64, Spos);
+}
+
+// X(S_i,:) and its norms for the prologue (rows past k zero), and the
+// accepted rows of F(S',:) plus the pivot-point append.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lm_gather(const double* __restrict__ P2, int64_t ldp,
+                                                   const int32_t* __restrict__ Spos, const T* __restrict__ Xsrc,
+                                                   const double* __restrict__ nrm, int32_t dp,
+                                                   const int64_t* __restrict__ Si, const ArpcState* st, double* FSd,
+                                                   T* XP, double* nrmP, T* XS, double* nrmS)
+{
+    if (st->done) return;
+    int c = blockIdx.x;
+    int64_t s = c < st->k ? Si[c] : -1;
+    for (int j = threadIdx.x; j < dp; j += 256) XS[c * dp + j] = s >= 0 ? Xsrc[s * dp + j] : (T)0;
+    if (threadIdx.x == 0) nrmS[c] = s >= 0 ? nrm[s] : 0.0;
+    lm_gather_body<T>(P2, ldp, Spos, Xsrc, nrm, dp, Si, st, FSd, XP, nrmP, c);
+}
+
+__global__ __launch_bounds__(256) void k_lm_linv_append(const double* __restrict__ Lc,
+                                                        const double* __restrict__ MTd, int64_t ldp,
+                                                        const ArpcState* st, double* Linv, int64_t ldl)
+{
+    lm_linv_append_body(Lc, MTd, ldp, st, Linv, ldl, blockIdx.y, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_lm_cast(const double* __restrict__ src, float* dst, int64_t count)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) dst[i] = (float)src[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_downdate_lm(
+    const T* __restrict__ X, int32_t dp, const double* __restrict__ nrm, double* __restrict__ d, int64_t n,
+    const T* __restrict__ XS, const double* __restrict__ nrmS, const int64_t* __restrict__ Si,
+    const T* __restrict__ XP, const double* __restrict__ nrmP, const int64_t* __restrict__ piv,
+    const T* __restrict__ MT, int64_t ldm, const T* __restrict__ LinvT, const ArpcState* st, GaussScale gs)
+{
+    typedef DowndateLds<T> DL;
+    __shared__ __attribute__((aligned(16))) T lds[DL::ELEMS];
+    __shared__ double rs[DL::WN][DL::BM];
+    __shared__ DowndateEp<T> ep;
+    downdate_lm_body<T>(X, dp, nrm, d, n, XS, nrmS, Si, XP, nrmP, piv, MT, ldm, LinvT, st, gs, blockIdx.x, lds,
+                        rs, &ep);
 }
 
 template <typename T>
@@ -192,6 +271,121 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
     return status;
 }
 
+#define LM_BLAS(call)                                                                 \
+    do {                                                                              \
+        rocblas_status b_ = (call);                                                   \
+        if (b_ != rocblas_status_success)                                             \
+            return kevd_fail(ctx, KEVD_EHIP, std::string(#call) + " failed: " +       \
+                                                 rocblas_status_to_string(b_));       \
+    } while (0)
+
+// The rounds of the LM form.  Each round needs the rank so far on the host
+// (it sizes the FP64 triangular products), so the state is read back after
+// every round, as the reference's round loop does.
+template <typename T>
+kevd_status run_rounds_lm(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds, int64_t n_inject,
+                          const ArpcBufs& B)
+{
+    int64_t n = ctx->n;
+    int nb = B.nb;
+    const int32_t dp = ctx->dp;
+    const T* Xsrc = std::is_same<T, float>::value ? (const T*)ctx->X32 : (const T*)ctx->X64;
+    const double* nrm = std::is_same<T, float>::value ? ctx->nrm32 : ctx->nrm64;
+    GaussScale gs{-1.0 / p->eps, -1.4426950408889634074 / p->eps};
+    int shift_base = 62 - ceil_log2(n);
+    double trK = (double)n;
+    hipStream_t s = ctx->stream;
+    rocblas_handle h = blas_handle(ctx);
+    if (!h) return kevd_fail(ctx, KEVD_EHIP, "rocblas handle creation failed");
+
+    // row length of every r-long row: room for a whole last pivot chunk
+    const int64_t ldp = KEVD_ROUND_UP(p->rank + LM_CH, 64);
+    const int64_t ldl = ldp;
+    double *P, *P2, *FSd, *MTd, *Linv, *nrmP;
+    T *XP, *MT;
+    int32_t* Spos;
+    KEVD_TRY(ctx_buf_t(ctx, "lm_P", (size_t)BMAX * ldp, &P));
+    KEVD_TRY(ctx_buf_t(ctx, "lm_P2", (size_t)BMAX * ldp, &P2));
+    KEVD_TRY(ctx_buf_t(ctx, "lm_FSd", (size_t)BMAX * ldp, &FSd));
+    KEVD_TRY(ctx_buf_t(ctx, "lm_MTd", (size_t)BMAX * ldp, &MTd));
+    KEVD_TRY(ctx_buf_t(ctx, "lm_Linv", (size_t)ldl * ldl, &Linv));
+    KEVD_TRY(ctx_buf_t(ctx, "lm_nrmP", (size_t)ldp, &nrmP));
+    KEVD_TRY(ctx_buf_t(ctx, "lm_XP", (size_t)ldp * dp, &XP));
+    KEVD_TRY(ctx_buf_t(ctx, "lm_Spos", BMAX, &Spos));
+    if (std::is_same<T, float>::value) KEVD_TRY(ctx_buf_t(ctx, "lm_MT", (size_t)BMAX * ldp, &MT));
+    else MT = (T*)MTd;
+    // zero: rows past k and columns past Nr must read as exact zeros
+    for (double* z : {P, P2, FSd, MTd, Linv}) {
+        size_t cnt = z == Linv ? (size_t)ldl * ldl : (size_t)BMAX * ldp;
+        KEVD_HIP(ctx, hipMemsetAsync(z, 0, cnt * sizeof(double), s));
+    }
+    KEVD_HIP(ctx, hipMemsetAsync(XP, 0, (size_t)ldp * dp * sizeof(T), s));
+    KEVD_HIP(ctx, hipMemsetAsync(nrmP, 0, (size_t)ldp * sizeof(double), s));
+    if ((void*)MT != (void*)MTd) KEVD_HIP(ctx, hipMemsetAsync(MT, 0, (size_t)BMAX * ldp * sizeof(T), s));
+
+    ArpcState* host_st = nullptr;
+    KEVD_TRY(ctx_pinned(ctx, sizeof(ArpcState), (void**)&host_st));
+    const double one = 1.0;
+    unsigned gdown = (unsigned)((n + Cfg<T>::BM - 1) / Cfg<T>::BM);
+    int64_t Nr = 0;
+    for (int rd = 0; rd < max_rounds; ++rd) {
+        {
+            ProfScope ps(ctx, TC_SAMPLE);
+            k_reduce_d<<<nb, 256, 0, s>>>(B.d, n, B.pmax, B.psum, B.st, 0);
+            k_weights_scan<<<nb, 256, 0, s>>>(B.d, n, B.pmax, nb, shift_base, B.prefix, B.btot, B.st);
+            k_draw<<<1, 256, 0, s>>>(B.psum, B.btot, nb, B.prefix, n, trK, p->seed, p->block, B.Sprime, B.st,
+                                     B.err, B.inj, n_inject);
+        }
+        {
+            ProfScope ps(ctx, TC_REJECT);
+            if (Nr > 0) {
+                // F(S',:) = K(S',S) L^-T in FP64 (column-major view: P2^T = L^-1 P^T)
+                k_lm_kprop<<<dim3((unsigned)((Nr + 255) / 256), BMAX), 256, 0, s>>>(
+                    ctx->X64, dp, B.Sprime, B.piv, B.st, gs.neg_inv_eps, P, ldp);
+                LM_BLAS(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                                      rocblas_diagonal_non_unit, (rocblas_int)Nr, BMAX, &one, Linv,
+                                      (rocblas_int)ldl, P, (rocblas_int)ldp, P2, (rocblas_int)ldp));
+            }
+            k_lm_form_H<<<dim3(BMAX / 16, BMAX / 16), 256, 0, s>>>(ctx->X64, dp, P2, ldp, B.Sprime, B.st,
+                                                                 gs.neg_inv_eps, B.H);
+            k_lm_rejchol<T><<<1, 256, 0, s>>>(B.H, B.st, p->seed, p->tol, B.inj != nullptr, B.Sprime, B.Si,
+                                              (T*)B.LinvT, B.Linv64, Spos);
+            k_lm_gather<T><<<BMAX, 256, 0, s>>>(P2, ldp, Spos, Xsrc, nrm, dp, B.Si, B.st, FSd, XP, nrmP,
+                                                (T*)B.XS, B.nrmS);
+            if (Nr > 0) {
+                // M^T = F_{S_i} L^-1 (column-major view: MTd^T = L^-T FSd^T)
+                LM_BLAS(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                      rocblas_diagonal_non_unit, (rocblas_int)Nr, BMAX, &one, Linv,
+                                      (rocblas_int)ldl, FSd, (rocblas_int)ldp, MTd, (rocblas_int)ldp));
+                if ((void*)MT != (void*)MTd)
+                    k_lm_cast<<<(unsigned)((BMAX * ldp + 255) / 256), 256, 0, s>>>(MTd, (float*)MT, BMAX * ldp);
+            }
+            k_lm_linv_append<<<dim3((unsigned)((Nr + BMAX + 255) / 256), BMAX), 256, 0, s>>>(B.Linv64, MTd, ldp,
+                                                                                            B.st, Linv, ldl);
+        }
+        {
+            ProfScope ps(ctx, TC_DOWNDATE);
+            k_downdate_lm<T><<<gdown, 256, 0, s>>>(Xsrc, dp, nrm, B.d, n, (const T*)B.XS, B.nrmS, B.Si, XP, nrmP,
+                                                   B.piv, MT, ldp, (const T*)B.LinvT, B.st, gs);
+        }
+        k_commit<<<1, BMAX, 0, s>>>(B.st, B.Si, B.piv, B.acc);
+        KEVD_HIP(ctx, hipGetLastError());
+        KEVD_HIP(ctx, hipMemcpyAsync(host_st, B.st, sizeof(ArpcState), hipMemcpyDeviceToHost, s));
+        KEVD_HIP(ctx, hipStreamSynchronize(s));
+        Nr = host_st->Nr;
+        if (host_st->done) break;
+    }
+    k_reduce_d<<<nb, 256, 0, s>>>(B.d, n, B.pmax, B.psum, B.st, 1);
+    k_final_err<<<1, 64, 0, s>>>(B.psum, nb, trK, B.st, B.err);
+    KEVD_HIP(ctx, hipGetLastError());
+    KEVD_HIP(ctx, hipStreamSynchronize(s));
+    ctx->lm_linv = Linv;
+    ctx->lm_ldl = ldl;
+    return KEVD_OK;
+}
+
+#undef LM_BLAS
+
 }  // namespace
 
 namespace arpck {
@@ -229,21 +423,25 @@ kevd_status arpc_validate(kevd_ctx* ctx, const kevd_arpc_params* p, const kevd_a
 kevd_status arpc_prepare(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds, ArpcBufs* B)
 {
     KEVD_HIP(ctx, hipSetDevice(ctx->device));
-    // factor storage
+    // factor storage (none in the LM form: that is its point)
     int64_t ldf = KEVD_ROUND_UP(p->rank, 64);
     size_t esz = p->prec == KEVD_MIXED ? sizeof(float) : sizeof(double);
     size_t fbytes = (size_t)ctx->n * ldf * esz;
     KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->F && ctx->F_bytes < fbytes) {
-        KEVD_HIP(ctx, hipFree(ctx->F));
-        ctx->F = nullptr;
-        ctx->F_bytes = 0;
-    }
-    if (!ctx->F) {
-        KEVD_HIP(ctx, hipMalloc(&ctx->F, fbytes));
-        ctx->F_bytes = fbytes;
+    if (!p->regenerate) {
+        if (ctx->F && ctx->F_bytes < fbytes) {
+            KEVD_HIP(ctx, hipFree(ctx->F));
+            ctx->F = nullptr;
+            ctx->F_bytes = 0;
+        }
+        if (!ctx->F) {
+            KEVD_HIP(ctx, hipMalloc(&ctx->F, fbytes));
+            ctx->F_bytes = fbytes;
+        }
     }
     ctx->have_factor = false;
+    ctx->lm = false;
+    ctx->lm_linv = nullptr;
     ctx->ldf = ldf;
     ctx->fprec = p->prec;
     ctx->rank = 0;
@@ -273,7 +471,7 @@ kevd_status arpc_prepare(kevd_ctx* ctx, const kevd_arpc_params* p, int max_round
     KEVD_TRY(ctx_buf_t(ctx, "a_Linv64", BMAX * BMAX, &B->Linv64));
     KEVD_TRY(ctx_buf_t(ctx, "a_nrmS", BMAX, &B->nrmS));
     KEVD_TRY(ctx_buf(ctx, "a_XS", (size_t)BMAX * ctx->dp * esz, &B->XS));
-    KEVD_TRY(ctx_buf(ctx, "a_FS", (size_t)BMAX * ldf * esz, &B->FS));
+    KEVD_TRY(ctx_buf(ctx, "a_FS", (size_t)BMAX * (p->regenerate ? 64 : ldf) * esz, &B->FS));
     KEVD_TRY(ctx_buf(ctx, "a_LinvT", (size_t)BMAX * BMAX * esz, &B->LinvT));
 
     ArpcState init{};
@@ -324,10 +522,20 @@ extern "C" kevd_status kevd_arpc(kevd_ctx* ctx, const kevd_arpc_params* p, kevd_
     prof_reset(ctx);
     arpck::ArpcBufs B;
     KEVD_TRY(arpck::arpc_prepare(ctx, p, max_rounds, &B));
-    kevd_status s = p->prec == KEVD_MIXED ? run_rounds<float>(ctx, p, max_rounds, p->n_inject, B)
-                                          : run_rounds<double>(ctx, p, max_rounds, p->n_inject, B);
+    kevd_status s;
+    if (p->regenerate)
+        s = p->prec == KEVD_MIXED ? run_rounds_lm<float>(ctx, p, max_rounds, p->n_inject, B)
+                                  : run_rounds_lm<double>(ctx, p, max_rounds, p->n_inject, B);
+    else
+        s = p->prec == KEVD_MIXED ? run_rounds<float>(ctx, p, max_rounds, p->n_inject, B)
+                                  : run_rounds<double>(ctx, p, max_rounds, p->n_inject, B);
     if (s != KEVD_OK) return s;
     KEVD_TRY(arpck::arpc_finish(ctx, p, max_rounds, B, res));
+    if (p->regenerate) {
+        // the LM form keeps S and L^-1, not F
+        ctx->have_factor = false;
+        ctx->lm = true;
+    }
     auto t1 = std::chrono::steady_clock::now();
     ctx->tm.arpc_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     prof_resolve(ctx);

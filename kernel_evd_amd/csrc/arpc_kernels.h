@@ -210,11 +210,12 @@ __device__ inline void draw_body(const double* __restrict__ psum, const uint64_t
 
 // H(p,q) = K(S'_p, S'_q) - sum_{m<Nr} F(S'_p, m) F(S'_q, m) in FP64, one
 // 16 x 16 tile per block.  K from the FP64 centred points, direct
-// differences.  Rows/columns past the proposal count are zero.
+// differences.  Rows/columns past the proposal count are zero.  With
+// frow_by_position the rows of F are the proposals' own rows (row p).
 template <typename T>
 __device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, const T* __restrict__ F,
                                    int64_t ldf, const int64_t* __restrict__ Sprime, const ArpcState* st,
-                                   double neg_inv_eps, double* H, int bx, int by)
+                                   double neg_inv_eps, double* H, int bx, int by, bool frow_by_position = false)
 {
     if (st->done) return;
     int np = st->nprop;
@@ -236,8 +237,9 @@ __device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, c
             int64_t m = m0 + cc;
             double vp = 0.0, vq = 0.0;
             if (m < Nr) {
-                if (pp < np) vp = (double)F[Sprime[pp] * ldf + m];
-                if (qq < np) vq = (double)F[Sprime[qq] * ldf + m];
+                // F(S'_p, :) is row S'_p of F, or row p of the proposals' rows
+                if (pp < np) vp = (double)F[(frow_by_position ? pp : Sprime[pp]) * ldf + m];
+                if (qq < np) vq = (double)F[(frow_by_position ? qq : Sprime[qq]) * ldf + m];
             }
             Fp[rr][cc] = vp;
             Fq[rr][cc] = vq;
@@ -278,7 +280,7 @@ __device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, c
 template <typename T>
 __device__ inline void rejchol_body(const double* __restrict__ H, ArpcState* st, uint64_t seed, double tol,
                                     int inject, const int64_t* __restrict__ Sprime, int64_t* Si, T* LinvT,
-                                    double* Linv64)
+                                    double* Linv64, int32_t* Spos = nullptr)
 {
     if (st->done) return;
     const int round = st->round;
@@ -338,6 +340,8 @@ __device__ inline void rejchol_body(const double* __restrict__ H, ArpcState* st,
         Linv64[e] = v;
     }
     if (tid < BMAX) Si[tid] = tid < k ? Sprime[accpos[tid]] : -1;
+    // positions in S' of the accepted proposals, when asked for
+    if (Spos && tid < BMAX) Spos[tid] = tid < k ? accpos[tid] : 0;
     if (tid == 0) st->k = k;
 }
 
@@ -379,40 +383,23 @@ template <typename T> struct DowndateEp {
     int64_t Si[Cfg<T>::BN];
 };
 
-// The fused Step-3 update for BM rows (ks_arpclm.cu:929-995 and
-// update_diag_block 455-478):
-//   acc  = X_R X_{S_i}^T                    (MFMA, K = dp)
-//   acc  = -K(R, S_i)                       (epilogue: norms, exp)
-//   acc += F_R F_{S_i}^T                    (MFMA, K = Nr)
-//   T    = -acc -> LDS
-//   acc  = T L_c^-T                         (MFMA, K = k)
-//   F(R, Nr:Nr+k) = acc;  d(R) = max(d(R) - rowsq(acc), 0)
-// This is row block blk (rows blk * BM ...) of one problem; it returns
-// early (before any barrier) when the problem is done or accepted nothing
-// this round.
+// First stage of the Step-3 update of BM rows: the epilogue constants go to
+// LDS and acc = -K(R, S_i) (distance product on MFMA, K = dp, then norms
+// and the compensated exp; exactly 1 on a row's own pivot).
 template <typename T>
-__device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const double* __restrict__ nrm,
-                                     T* __restrict__ F, int64_t ldf, double* __restrict__ d, int64_t n,
-                                     const T* __restrict__ XS, const double* __restrict__ nrmS,
-                                     const int64_t* __restrict__ Si, const T* __restrict__ FS,
-                                     const T* __restrict__ LinvT, const ArpcState* st, GaussScale gs,
-                                     int64_t blk, T* lds, double (*rs)[Cfg<T>::BM], DowndateEp<T>* ep)
+__device__ inline void dd_prologue(typename Cfg<T>::G& g, const T* __restrict__ X, int32_t dp,
+                                   const double* __restrict__ nrm, int64_t n, const T* __restrict__ XS,
+                                   const double* __restrict__ nrmS, const int64_t* __restrict__ Si, int k,
+                                   GaussScale gs, int64_t r0, T* lds, DowndateEp<T>* ep)
 {
     typedef typename Cfg<T>::G G;
-    typedef DowndateLds<T> DL;
-    constexpr int BM = DL::BM, BN = DL::BN, TP = DL::TP, WN = DL::WN;
-    if (st->done) return;
-    int k = st->k;
-    if (k == 0) return;
-    int64_t Nr = st->Nr;
-    int64_t r0 = blk * BM;
+    constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
     int64_t rows = n - r0;
     for (int i = threadIdx.x; i < BM; i += 256) ep->nrm[i] = r0 + i < n ? nrm[r0 + i] : 0.0;
     for (int i = threadIdx.x; i < BN; i += 256) {
         ep->nrmS[i] = nrmS[i];
         ep->Si[i] = Si[i];
     }
-    G g;
     g.init();
     g.set_live_cols(k);
     // distance product (its barriers also publish ep)
@@ -433,8 +420,20 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
                 }
                 g.acc[a][b][v] = (T)(-kv);
             }
-    // overlap with the current factor
-    if (Nr > 0) g.mainloop_live(lds, F + r0 * ldf, ldf, rows, FS, ldf, BN, Nr);
+}
+
+// Last stage: with acc = -(K(R,S_i) - F_R F_{S_i}^T) (the overlap added by
+// the caller, which leaves the LDS free behind a barrier), T goes to LDS,
+// G = T L_c^-T (L_c^-T is triangular), G is written to F(R, Nr:Nr+k) when F
+// is given, and d(R) = max(d(R) - rowsq(G), 0).
+template <typename T>
+__device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64_t ldf, double* __restrict__ d,
+                                 int64_t n, const T* __restrict__ LinvT, int k, int64_t Nr, int64_t r0, T* lds,
+                                 double (*rs)[Cfg<T>::BM])
+{
+    typedef typename Cfg<T>::G G;
+    typedef DowndateLds<T> DL;
+    constexpr int BM = DL::BM, TP = DL::TP, WN = DL::WN;
     // T = K(R,S_i) - F_R F_{S_i}^T into LDS
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
@@ -462,7 +461,7 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
                 int col = g.col_of(b);
                 T gv = g.acc[a][b][v];
                 if (col < k) {
-                    if (row < n) F[row * ldf + Nr + col] = gv;
+                    if (F && row < n) F[row * ldf + Nr + col] = gv;
                     part[a][v] += (double)gv * (double)gv;
                 }
             }
@@ -488,6 +487,188 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
             d[row] = dd >= 0.0 ? dd : 0.0;
         }
     }
+}
+
+// The fused Step-3 update for BM rows (ks_arpclm.cu:929-995 and
+// update_diag_block 455-478):
+//   acc  = X_R X_{S_i}^T                    (MFMA, K = dp)
+//   acc  = -K(R, S_i)                       (epilogue: norms, exp)
+//   acc += F_R F_{S_i}^T                    (MFMA, K = Nr)
+//   T    = -acc -> LDS
+//   acc  = T L_c^-T                         (MFMA, K = k)
+//   F(R, Nr:Nr+k) = acc;  d(R) = max(d(R) - rowsq(acc), 0)
+// This is row block blk (rows blk * BM ...) of one problem; it returns
+// early (before any barrier) when the problem is done or accepted nothing
+// this round.
+template <typename T>
+__device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const double* __restrict__ nrm,
+                                     T* __restrict__ F, int64_t ldf, double* __restrict__ d, int64_t n,
+                                     const T* __restrict__ XS, const double* __restrict__ nrmS,
+                                     const int64_t* __restrict__ Si, const T* __restrict__ FS,
+                                     const T* __restrict__ LinvT, const ArpcState* st, GaussScale gs,
+                                     int64_t blk, T* lds, double (*rs)[Cfg<T>::BM], DowndateEp<T>* ep)
+{
+    typedef typename Cfg<T>::G G;
+    constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
+    if (st->done) return;
+    int k = st->k;
+    if (k == 0) return;
+    int64_t Nr = st->Nr;
+    int64_t r0 = blk * BM;
+    G g;
+    dd_prologue<T>(g, X, dp, nrm, n, XS, nrmS, Si, k, gs, r0, lds, ep);
+    // overlap with the current factor (ends with a barrier)
+    if (Nr > 0) g.mainloop_live(lds, F + r0 * ldf, ldf, n - r0, FS, ldf, BN, Nr);
+    dd_finish<T>(g, F, ldf, d, n, LinvT, k, Nr, r0, lds, rs);
+}
+
+// ------------------------------------------- regenerating (LM) variant
+// The form ks_arpclm.cu itself runs: no factor is stored.  With the pivots S
+// (Nr so far) and L^-1 (K(S,S) = L L^T) kept, F = K(:,S) L^-T is never
+// formed; its rows and products are regenerated from the points:
+//   F(S',:)         = K(S',S) L^-T                     (FP64, proposals)
+//   F_R F_{S_i}^T   = K(R,S) M,  M = L^-T F_{S_i}^T      (Sgemm T, :965-972)
+//   L^-1 <- [[L^-1, 0], [-L_c^-1 F_{S_i} L^-1, L_c^-1]]  (append_Linv_*, :480-534)
+// Memory is O(n + r^2) instead of the factor's n r.
+
+// Pivot chunk of the regenerated overlap: LM_CH pivots per distance tile.
+constexpr int LM_CH = 32;
+template <typename T> struct LmCfg;
+template <> struct LmCfg<float> {
+    typedef NTGemm<float, 128, LM_CH, 32, 4, 1, 2, 2, Mfma16f> GD;
+};
+template <> struct LmCfg<double> {
+    typedef NTGemm<double, 64, LM_CH, 16, 4, 1, 1, 2> GD;
+};
+template <typename T> struct DowndateLmLds {
+    typedef typename LmCfg<T>::GD GD;
+    static constexpr int TPC = LM_CH + 16 / (int)sizeof(T);  // K-chunk tile pitch
+    static constexpr int ELEMS = GD::LDS_ELEMS + Cfg<T>::BM * TPC;
+    static_assert(ELEMS <= DowndateLds<T>::ELEMS, "the chunk stage fits the downdate LDS");
+};
+
+// P(p, m) = K(S'_p, S_m) in FP64 (direct differences of the FP64 points),
+// row-major BMAX x ldp; zero for p >= nprop.  Thread per (p, m), m < Nr.
+__device__ inline void lm_kprop_body(const double* __restrict__ X64, int32_t dp, const int64_t* __restrict__ Sprime,
+                                     const int64_t* __restrict__ piv, const ArpcState* st, double neg_inv_eps,
+                                     double* P, int64_t ldp, int p, int64_t m)
+{
+    if (st->done || m >= st->Nr) return;
+    double kv = 0.0;
+    if (p < st->nprop) {
+        int64_t sp = Sprime[p], sm = piv[m];
+        if (sp == sm) {
+            kv = 1.0;
+        } else {
+            const double* a = X64 + sp * dp;
+            const double* b = X64 + sm * dp;
+            double q = 0.0;
+            for (int j = 0; j < dp; ++j) {
+                double t = a[j] - b[j];
+                q += t * t;
+            }
+            kv = exp(q * neg_inv_eps);
+        }
+    }
+    P[p * ldp + m] = kv;
+}
+
+// Rows of the accepted proposals: FSd(c, :) = F(S'_{pos_c}, :) (FP64, rows
+// past k zero), and the accepted points appended to the pivot points XP and
+// their norms (at Nr + c).  Block per c < BMAX.
+template <typename T>
+__device__ inline void lm_gather_body(const double* __restrict__ P2, int64_t ldp, const int32_t* __restrict__ Spos,
+                                      const T* __restrict__ Xsrc, const double* __restrict__ nrm, int32_t dp,
+                                      const int64_t* __restrict__ Si, const ArpcState* st, double* FSd, T* XP,
+                                      double* nrmP, int c)
+{
+    if (st->done) return;
+    int k = st->k;
+    int64_t Nr = st->Nr;
+    const double* src = c < k ? P2 + (int64_t)Spos[c] * ldp : nullptr;
+    for (int64_t m = threadIdx.x; m < Nr; m += 256) FSd[c * ldp + m] = src ? src[m] : 0.0;
+    if (c < k) {
+        int64_t s = Si[c];
+        for (int j = threadIdx.x; j < dp; j += 256) XP[(Nr + c) * dp + j] = Xsrc[s * dp + j];
+        if (threadIdx.x == 0) nrmP[Nr + c] = nrm[s];
+    }
+}
+
+// L^-1 (row-major, lower, FP64, ld ldl) gains rows Nr .. Nr+k-1:
+// -L_c^-1 MTd in columns < Nr (MTd = F_{S_i} L^-1, k x Nr) and L_c^-1 on the
+// diagonal block.  Thread per (c, m), m < Nr + k.
+__device__ inline void lm_linv_append_body(const double* __restrict__ Lc, const double* __restrict__ MTd,
+                                           int64_t ldp, const ArpcState* st, double* Linv, int64_t ldl, int c,
+                                           int64_t m)
+{
+    if (st->done) return;
+    int k = st->k;
+    int64_t Nr = st->Nr;
+    if (c >= k || m >= Nr + k) return;
+    double v;
+    if (m < Nr) {
+        v = 0.0;
+        for (int q = 0; q <= c; ++q) v -= Lc[c * BMAX + q] * MTd[q * ldp + m];
+    } else {
+        int q = (int)(m - Nr);
+        v = q <= c ? Lc[c * BMAX + q] : 0.0;
+    }
+    Linv[(Nr + c) * ldl + m] = v;
+}
+
+// The Step-3 update of the LM form: as downdate_body, with the overlap
+// F_R F_{S_i}^T = K(R,S) M regenerated LM_CH pivots at a time (distance tile
+// of X_R against the pivot points XP, norms and exp into LDS, then MFMA with
+// the rows of M^T = MT, read from L2), and no factor written.
+template <typename T>
+__device__ inline void downdate_lm_body(const T* __restrict__ X, int32_t dp, const double* __restrict__ nrm,
+                                        double* __restrict__ d, int64_t n, const T* __restrict__ XS,
+                                        const double* __restrict__ nrmS, const int64_t* __restrict__ Si,
+                                        const T* __restrict__ XP, const double* __restrict__ nrmP,
+                                        const int64_t* __restrict__ piv, const T* __restrict__ MT, int64_t ldm,
+                                        const T* __restrict__ LinvT, const ArpcState* st, GaussScale gs,
+                                        int64_t blk, T* lds, double (*rs)[Cfg<T>::BM], DowndateEp<T>* ep)
+{
+    typedef typename Cfg<T>::G G;
+    typedef typename LmCfg<T>::GD GD;
+    constexpr int BM = Cfg<T>::BM, TPC = DowndateLmLds<T>::TPC;
+    if (st->done) return;
+    int k = st->k;
+    if (k == 0) return;
+    int64_t Nr = st->Nr;
+    int64_t r0 = blk * BM;
+    G g;
+    dd_prologue<T>(g, X, dp, nrm, n, XS, nrmS, Si, k, gs, r0, lds, ep);
+    T* ldsD = lds;
+    T* Tc = lds + GD::LDS_ELEMS;
+    for (int64_t c0 = 0; c0 < Nr; c0 += LM_CH) {
+        const int cw = Nr - c0 < LM_CH ? (int)(Nr - c0) : LM_CH;
+        GD gd;
+        gd.init();
+        // the previous chunk's reads of ldsD ended behind its barrier; those
+        // of Tc end before any wave passes this mainloop's barriers
+        gd.mainloop(ldsD, X + r0 * dp, dp, n - r0, XP + c0 * dp, dp, cw, dp);
+#pragma unroll
+        for (int a = 0; a < GD::NTM; ++a)
+#pragma unroll
+            for (int b = 0; b < GD::NTN; ++b)
+#pragma unroll
+                for (int v = 0; v < GD::M::ACC; ++v) {
+                    int rl = gd.row_of(a, v);
+                    int col = gd.col_of(b);
+                    double kv = 0.0;
+                    if (r0 + rl < n && col < cw) {
+                        double D = ep->nrm[rl] + nrmP[c0 + col] - 2.0 * (double)gd.acc[a][b][v];
+                        D = D > 0.0 ? D : 0.0;
+                        kv = (r0 + rl == piv[c0 + col]) ? 1.0 : (double)gauss_val(D, gs, (T*)nullptr);
+                    }
+                    Tc[rl * TPC + col] = (T)kv;
+                }
+        __syncthreads();
+        g.lds_global(Tc, TPC, MT + c0, ldm, LM_CH);
+    }
+    __syncthreads();  // every wave is done with Tc before T overwrites the LDS
+    dd_finish<T>(g, nullptr, 0, d, n, LinvT, k, Nr, r0, lds, rs);
 }
 
 // Block of BMAX threads.

@@ -329,6 +329,8 @@ extern "C" kevd_status kevd_ensemble_arpc(kevd_ctx** ctxs, int32_t nprob, const 
             return kevd_fail(m, KEVD_EINVAL, "ensemble: every problem must use the same precision");
         if (params[i].inject_pivots)
             return kevd_fail(m, KEVD_EINVAL, "ensemble: injected pivots are a single-problem feature");
+        if (params[i].regenerate)
+            return kevd_fail(m, KEVD_EINVAL, "ensemble: the LM form (regenerate) is a single-problem feature");
         for (int j = 0; j < i; ++j)
             if (ctxs[j] == ctxs[i]) return kevd_fail(m, KEVD_EINVAL, "ensemble: a context appears twice");
     }

@@ -100,6 +100,7 @@ extern "C" kevd_status kevd_set_factor(kevd_ctx* ctx, const void* F_host, int64_
     ctx->rank = r;
     ctx->fprec = prec;
     ctx->have_factor = true;
+    ctx->lm = false;
     ctx->pivots.clear();
     return KEVD_OK;
 }
@@ -115,8 +116,30 @@ __global__ __launch_bounds__(256) void k_gather_L(const T* F, int64_t ldf, const
     L[a + c * r] = c <= a ? (double)F[piv[a] * ldf + c] : 0.0;
 }
 
+// Column-major r x r copy of the LM form's L^-1 (row-major, ld ldl).
+__global__ __launch_bounds__(256) void k_linv_colmajor(const double* Linv, int64_t ldl, int64_t r, double* out)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= r * r) return;
+    int64_t c = i / r, a = i % r;
+    out[a + c * r] = Linv[a * ldl + c];
+}
+
 extern "C" kevd_status kevd_get_linv(kevd_ctx* ctx, double* Linv_host)
 {
+    if (ctx && Linv_host && ctx->lm && ctx->lm_linv && ctx->pivots.size() == (size_t)ctx->rank && ctx->rank > 0) {
+        // the LM form kept L^-1 itself
+        KEVD_HIP(ctx, hipSetDevice(ctx->device));
+        int64_t r = ctx->rank;
+        double* out = nullptr;
+        KEVD_TRY(ctx_buf_t(ctx, "l_L", (size_t)r * r, &out));
+        k_linv_colmajor<<<(unsigned)((r * r + 255) / 256), 256, 0, ctx->stream>>>(ctx->lm_linv, ctx->lm_ldl, r, out);
+        KEVD_HIP(ctx, hipGetLastError());
+        KEVD_HIP(ctx, hipMemcpyAsync(Linv_host, out, (size_t)r * r * sizeof(double), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return KEVD_OK;
+    }
     if (!ctx || !Linv_host || !ctx->have_factor || ctx->pivots.size() != (size_t)ctx->rank)
         return kevd_fail(ctx, KEVD_EINVAL, "get_linv: needs a factor built by kevd_arpc");
     KEVD_HIP(ctx, hipSetDevice(ctx->device));

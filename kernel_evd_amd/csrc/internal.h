@@ -49,6 +49,11 @@ struct kevd_ctx {
     bool have_factor = false;
     std::vector<int64_t> pivots;  // acceptance order, when F came from kevd_arpc
     double factor_eps = 0.0;
+    // The last kevd_arpc ran in the LM form: no F, L^-1 kept instead
+    // (row-major lower, FP64, leading dimension lm_ldl; a named buffer).
+    bool lm = false;
+    double* lm_linv = nullptr;
+    int64_t lm_ldl = 0;
 
     // kEVD output held on the device.
     float* U = nullptr;
