@@ -7,8 +7,10 @@
 //             (--eps E | --eps-median [--median-pairs P])
 //             --rank R [--block 128] [--seed 0] [--tol 1e-10]
 //             [--precision mixed|fp64] [--max-rounds K] [--inject-pivots S.npy|spiv.h5]
-//             [--factor-out F.npy] [--pivots-out S.npy|spiv.h5] [--linv-out Linv.npy|linv.h5]
-//             [--device 0]
+//             [--regenerate] [--factor-out F.npy] [--pivots-out S.npy|spiv.h5]
+//             [--linv-out Linv.npy|linv.h5] [--device 0]
+// --regenerate runs the reference's own LM form (S and L^-1 kept, K(R,S)
+// regenerated every round; memory O(n + r^2)): no factor file then.
 // File arguments are .npy, or the reference's HDF5 files (ks_arpclm.cu:
 // 628-644): the snapshot file's /u (delay embedded with --delay), and /spiv
 // (unsigned 64-bit) and /linv (FP64, r x r column-major, flattened) out.
@@ -34,7 +36,7 @@ int main(int argc, char** argv)
     if (a.has("help")) {
         std::printf("kevd-arpc --points X.npy|u.h5 [--delay Nq] [--grid-points M] (--eps E | --eps-median)\n"
                     "          --rank R [--block 128] [--seed 0] [--tol 1e-10] [--precision mixed|fp64]\n"
-                    "          [--max-rounds K] [--inject-pivots S.npy|spiv.h5] [--factor-out F.npy]\n"
+                    "          [--max-rounds K] [--inject-pivots S.npy|spiv.h5] [--regenerate] [--factor-out F.npy]\n"
                     "          [--pivots-out S.npy|spiv.h5] [--linv-out Linv.npy|linv.h5]\n"
                     "          [--median-pairs 4194304] [--device 0]\n");
         return 0;
@@ -62,6 +64,12 @@ int main(int argc, char** argv)
     p.tol = a.num("tol", 1e-10);
     p.prec = a.str("precision", "mixed") == "fp64" ? KEVD_FP64 : KEVD_MIXED;
     p.max_rounds = (int32_t)a.integer("max-rounds", 0);
+    p.regenerate = a.has("regenerate") ? 1 : 0;
+    if (p.regenerate && a.has("factor-out")) {
+        std::fprintf(stderr, "--regenerate keeps no factor: drop --factor-out\n");
+        kevd_destroy(c);
+        return 1;
+    }
     std::vector<int64_t> inj;
     if (a.has("inject-pivots")) {
         try {

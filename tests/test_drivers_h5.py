@@ -4,7 +4,8 @@ the snapshot file /u,/t (ks_datagen.py:91-103) in, /spiv and /linv
 1930-1938, 2300-2306) out.  Files are read back with the HDF5 tools'
 h5dump (no Python HDF5 module here).
 
-Bars: the .h5 and .npy routes give bit-identical points, pivots and L^-1;
+Bars: the .h5 and .npy routes give bit-identical points, pivots and L^-1
+(and the LM form, --regenerate, the same pivots and L^-1 to 1e-9);
 kevd-evd on (/u, /spiv, /linv) -- the reference kEVD's inputs -- matches the
 same EVD run on the factor file to 1e-10 in the eigenvalues and reports a
 rebuilt L^-1 within 1e-10 of /linv."""
@@ -97,6 +98,15 @@ def test_arpc_and_evd_on_reference_files(h5_ok, tmp_path):
     assert np.array_equal(spiv.astype(np.int64), S)
     linv, _ = h5read(tmp_path / "linv.h5", "/linv")
     assert np.array_equal(linv, np.load(tmp_path / "L.npy"))
+    print("step kevd-arpc --regenerate", flush=True)
+    run("kevd-arpc", "--points", tmp_path / "u.h5", "--delay", nq, *common, "--regenerate",
+        "--pivots-out", tmp_path / "spiv_lm.h5", "--linv-out", tmp_path / "linv_lm.h5")
+    assert np.array_equal(h5read(tmp_path / "spiv_lm.h5", "/spiv")[0], spiv)
+    linv_lm, _ = h5read(tmp_path / "linv_lm.h5", "/linv")
+    assert np.linalg.norm(linv_lm - linv) <= 1e-9 * np.linalg.norm(linv)
+    r = run("kevd-arpc", "--points", tmp_path / "u.h5", "--delay", nq, *common, "--regenerate",
+            "--factor-out", tmp_path / "F_lm.npy", check=False)
+    assert r.returncode == 1
     print("step kevd-evd on /u /spiv /linv", flush=True)
     r3 = run("kevd-evd", "--points", tmp_path / "u.h5", "--delay", nq, "--pivots", tmp_path / "spiv.h5",
              "--linv", tmp_path / "linv.h5", "--eps-median", "--seed", 5, "--kout", 10, "--precision", "fp64",
