@@ -1330,8 +1330,9 @@ kevd_status orth_basis(kevd_ctx* ctx, double* Y, int64_t r, int l, bool househol
 // iteration on l = max(k + 64, min(4k, 256)) columns (rounded up to 32):
 // Q <- orth(W Q), each step reducing the unwanted
 // components of eigenvector i by lambda_{l+1} / lambda_i.  Rayleigh-Ritz
-// (H = Q^T W Q = S diag(theta) S^T) runs after the first few steps and
-// then after every step, until the leading k residuals
+// (H = Q^T W Q = S diag(theta) S^T) runs after the first few steps (where
+// the context's last solve converged, at least 4) and then after as many
+// steps as the slowest Ritz pair needs, until the leading k residuals
 // ||W x - theta x|| <= 1e-6 theta + 1e-14 theta_max.  The error of a Ritz
 // value is of the order of the squared relative residual, so the stopping
 // rule gives ~1e-12 relative eigenvalues.
@@ -1368,25 +1369,33 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
     double one = 1.0, zero = 0.0;
     std::vector<double> hres((size_t)k), hth((size_t)l);
     std::vector<int> hinfo(1 + 3 * max_orth);
-    const int maxit = 16, first_check = 4;
+    // A Rayleigh-Ritz check (an l x l syevd, latency-bound) costs several
+    // iterations, so the first one goes where the last solve on this context
+    // converged (at least step 4): repeated solves on similar matrices then
+    // check once, and a guess past convergence costs only a few iterations.
+    const int maxit = 16;
+    const int first_check = std::max(4, std::min(ctx->eig_converged_at, maxit - 1));
     for (int householder = 0; householder < 2; ++householder) {
         KEVD_HIP(ctx, hipMemsetAsync(info, 0, (1 + 3 * max_orth) * sizeof(int), ctx->stream));
         k_randn<<<(unsigned)((r * l + 255) / 256), 256, 0, ctx->stream>>>(Q, r * l, 0x6b657664ull);
         KEVD_HIP(ctx, hipGetLastError());
         int north = 0;
-        KEVD_TRY(orth_basis(ctx, Q, r, l, householder, G, tau, info + 1 + 3 * north++));
+        // a Gaussian block is well conditioned: one shifted pass
+        KEVD_TRY(orth_basis(ctx, Q, r, l, householder, G, tau, info + 1 + 3 * north++, 1));
         bool conv = false, broken = false;
         int it = 0, next_check = first_check;
         for (it = 1; it <= maxit; ++it) {
             BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, Lb, R, &one, W, R, Q, R,
                                &zero, Y, R));
             if (it < next_check && north < max_orth) {
-                // the next product only needs a well-conditioned basis of the
-                // span: one shifted and one plain Cholesky QR pass; the basis
-                // that Rayleigh-Ritz reads gets all three
+                // the iteration only follows span(Q), so the next product
+                // needs a well-conditioned basis, not an orthonormal one: one
+                // shifted Cholesky QR pass (cond(Q) = 1 + O(cond(Y)^2 u)); the
+                // basis that Rayleigh-Ritz reads gets a second, plain pass,
+                // which leaves it orthonormal to O(u)
                 bool before_check = it + 1 >= next_check || north + 1 >= max_orth;
                 KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++,
-                                    before_check ? 3 : 2));
+                                    before_check ? 2 : 1));
                 std::swap(Q, Y);
                 continue;
             }
@@ -1429,13 +1438,14 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
             next_check = it + need;
             KEVD_HIP(ctx, hipMemsetAsync(info, 0, (1 + 3 * max_orth) * sizeof(int), ctx->stream));
             north = 0;
-            KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++));
+            KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++, 1));
             std::swap(Q, Y);
         }
         *iters = it;
         ctx->tm.eig_iters = it;
         ctx->tm.eig_householder = householder;
         if (broken) continue;  // a Cholesky QR pass lost definiteness: redo with Householder
+        if (conv) ctx->eig_converged_at = it;
         // Ritz vectors of the leading k: X = Q S(:, l-k:l)
         BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, (rocblas_int)k, Lb, &one, Q, R,
                            H + (int64_t)(l - k) * l, Lb, &zero, X, R));

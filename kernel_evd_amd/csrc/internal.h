@@ -55,6 +55,11 @@ struct kevd_ctx {
     double* lm_linv = nullptr;
     int64_t lm_ldl = 0;
 
+    // Subspace-iteration step at which the last leading-eigenpair solve on
+    // this context converged (0: none yet); the next solve runs its first
+    // Rayleigh-Ritz check there.
+    int eig_converged_at = 0;
+
     // kEVD output held on the device.
     float* U = nullptr;
     size_t U_bytes = 0;
