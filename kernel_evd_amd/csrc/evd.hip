@@ -1170,13 +1170,22 @@ __global__ __launch_bounds__(256) void k_chol_shift(double* G, int l, double c)
 // (column-major, upper triangle read and overwritten with R, A = R^T R), in
 // one workgroup, for l <= CHOL_LMAX.  Right-looking over 32-row panels:
 //   the 32 x 32 diagonal block is factored by one wave in registers (lane c
-//   owns column c; rows of R travel by shuffles, no barriers);
+//   owns column c); the pivot and row k of R reach every lane as scalar
+//   operands (v_readlane), with no LDS round trip;
 //   R12 = R11^-T A12 by forward substitution, one thread per trailing
 //   column, also kept in LDS;
 //   A22 -= R12^T R12 on the upper triangle, read-modify-write through L2.
 // *info = 0, or the 1-based index of the first non-positive pivot (the
 // factorisation stops there), as LAPACK potrf.
 constexpr int CHOL_LMAX = 512;
+
+// Lane `lane` (wave-uniform) of a double, broadcast through scalar registers.
+__device__ inline double readlane_f64(double v, int lane)
+{
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
 __global__ __launch_bounds__(1024) void k_potrf_upper(double* A, int l, int* info)
 {
     __shared__ double D[32][33];
@@ -1192,7 +1201,8 @@ __global__ __launch_bounds__(1024) void k_potrf_upper(double* A, int l, int* inf
             D[r][c] = A[(j0 + r) + (int64_t)(j0 + c) * l];
         }
         __syncthreads();
-        if (tid < 32) {
+        if (tid < 64) {
+            // one wave; lanes 32..63 mirror lanes 0..31 (c >= 32 owns nothing)
             const int c = tid;
             double col[32];
 #pragma unroll
@@ -1201,16 +1211,16 @@ __global__ __launch_bounds__(1024) void k_potrf_upper(double* A, int l, int* inf
 #pragma unroll
             for (int k = 0; k < 32; ++k) {
                 if (k < nb && !fail) {
-                    double akk = __shfl(col[k], k, 32);
+                    double akk = readlane_f64(col[k], k);
                     if (!(akk > 0.0)) {
                         fail = j0 + k + 1;
                     } else {
                         double rkk = sqrt(akk);
-                        double rkc = c == k ? rkk : (c > k ? col[k] / rkk : 0.0);
+                        double rkc = c == k ? rkk : (c > k && c < 32 ? col[k] / rkk : 0.0);
                         col[k] = rkc;
 #pragma unroll
                         for (int i = k + 1; i < 32; ++i) {
-                            double rki = __shfl(rkc, i, 32);
+                            double rki = readlane_f64(rkc, i);
                             if (c >= i) col[i] -= rki * rkc;
                         }
                     }
