@@ -514,30 +514,56 @@ __global__ void k_weights(const double* x, int64_t n, int mode, double* w)
 // PROMOTE rows the MFMA accumulators are added into this block's FP64
 // partial tile (a private slice of `part`), which bounds the length of any
 // single FP32 accumulation.
-template <typename T> struct GramCfg;
+// NT threads per workgroup in WGM x WGN waves, WG_PER_CU workgroups resident
+// per CU.
+template <typename T, int NG> struct GramCfg;
 #ifndef KEVD_GRAM_PROMOTE
 #define KEVD_GRAM_PROMOTE 2048
 #endif
-template <> struct GramCfg<float> {
+#ifndef KEVD_GRAM1_CFG
+#define KEVD_GRAM1_CFG 0
+#endif
+template <int NG> struct GramCfg<float, NG> {
     static constexpr int BI = 128, BJ = 128, KC = 32, TM = 2, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
+    static constexpr int NT = 256, WGM = 2, WGN = 2, WG_PER_CU = 2;
 };
-template <> struct GramCfg<double> {
+#if KEVD_GRAM1_CFG == 1
+template <> struct GramCfg<float, 1> {
+    static constexpr int BI = 256, BJ = 128, KC = 32, TM = 2, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
+    static constexpr int NT = 512, WGM = 4, WGN = 2, WG_PER_CU = 1;
+};
+#elif KEVD_GRAM1_CFG == 2
+template <> struct GramCfg<float, 1> {
+    static constexpr int BI = 256, BJ = 256, KC = 32, TM = 4, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
+    static constexpr int NT = 512, WGM = 2, WGN = 4, WG_PER_CU = 1;
+};
+#elif KEVD_GRAM1_CFG == 3
+template <> struct GramCfg<float, 1> {
+    static constexpr int BI = 256, BJ = 256, KC = 16, TM = 4, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
+    static constexpr int NT = 512, WGM = 2, WGN = 4, WG_PER_CU = 1;
+};
+#endif
+template <int NG> struct GramCfg<double, NG> {
     static constexpr int BI = 128, BJ = 64, KC = 16, TM = 4, TN = 2, PROMOTE = 1 << 30;
+    static constexpr int NT = 256, WGM = 2, WGN = 2, WG_PER_CU = 2;
 };
 
 template <typename T, int NG>
 struct GramTile {
-    typedef GramCfg<T> C;
+    typedef GramCfg<T, NG> C;
     typedef Mfma<T> M;
-    static constexpr int BI = C::BI, BJ = C::BJ, KC = C::KC, TM = C::TM, TN = C::TN;
+    static constexpr int BI = C::BI, BJ = C::BJ, KC = C::KC, TM = C::TM, TN = C::TN, NT = C::NT;
     static constexpr int IL = 2;                            // interleaved tiles per load
     static constexpr int EPV = 16 / (int)sizeof(T);
     static constexpr int PI = BI + EPV, PJ = BJ + EPV;
     static constexpr int WOFF = KC * PI + KC * PJ;          // weights after the panels
     static constexpr int STAGE = WOFF + NG * KC;
     static constexpr int VI = BI / EPV, VJ = BJ / EPV;     // 16 B vectors per panel row
-    static constexpr int NVI = (KC * VI + 255) / 256, NVJ = (KC * VJ + 255) / 256;
+    static constexpr int NVI = (KC * VI + NT - 1) / NT, NVJ = (KC * VJ + NT - 1) / NT;
     static constexpr int WR = TM * M::MT, WC = TN * M::MT; // wave tile
+    static_assert(C::WGM * WR == BI && C::WGN * WC == BJ && C::WGM * C::WGN * 64 == NT, "Gram wave layout");
+    static_assert(NG * KC <= NT, "Gram weight staging");
+    static_assert((KC * VI) % NT == 0 && (KC * VJ) % NT == 0, "Gram panel staging");
     typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type pair_t;
 
     typename M::acc_t acc[NG][TM][TN];
@@ -571,32 +597,32 @@ struct GramTile {
         }
     }
 
-    __device__ inline void gload(Stage& sg, const T* __restrict__ F, int64_t ldf, int64_t r, int64_t ci,
-                                 int64_t cj, int64_t k0, int64_t k_end,
+    // F is read through a buffer resource over this split's rows [k_beg,
+    // k_end) (num_records = their bytes, below 2^31): a row at or past k_end
+    // reads as zero by the hardware's range check, so the panel loads carry
+    // no predicate and no branch.  kro = first row of the chunk - k_beg.
+    __device__ inline void gload(Stage& sg, __amdgpu_buffer_rsrc_t rs, int64_t ldf, int64_t ci, int64_t cj,
+                                 int64_t kro, int64_t k_beg, int64_t k_end,
                                  const T* __restrict__ w1, const T* __restrict__ w2)
     {
+        typedef __attribute__((ext_vector_type(4))) unsigned u4;
 #pragma unroll
         for (int j = 0; j < NVI; ++j) {
-            int lin = threadIdx.x + 256 * j;
+            int lin = threadIdx.x + NT * j;
             int kr = lin / VI, cv = lin % VI;
-            int64_t row = k0 + kr, col = ci + cv * EPV;
-            sg.ri[j] = make_uint4(0, 0, 0, 0);
-            if (lin < KC * VI && row < k_end && col < r) {
-                sg.ri[j] = *reinterpret_cast<const uint4*>(F + row * ldf + col);
-                mask(sg.ri[j], col, r);
-            }
+            uint32_t off = (uint32_t)(((kro + kr) * ldf + ci + cv * EPV) * (int64_t)sizeof(T));
+            u4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+            sg.ri[j] = make_uint4(v.x, v.y, v.z, v.w);
         }
 #pragma unroll
         for (int j = 0; j < NVJ; ++j) {
-            int lin = threadIdx.x + 256 * j;
+            int lin = threadIdx.x + NT * j;
             int kr = lin / VJ, cv = lin % VJ;
-            int64_t row = k0 + kr, col = cj + cv * EPV;
-            sg.rj[j] = make_uint4(0, 0, 0, 0);
-            if (lin < KC * VJ && row < k_end && col < r) {
-                sg.rj[j] = *reinterpret_cast<const uint4*>(F + row * ldf + col);
-                mask(sg.rj[j], col, r);
-            }
+            uint32_t off = (uint32_t)(((kro + kr) * ldf + cj + cv * EPV) * (int64_t)sizeof(T));
+            u4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+            sg.rj[j] = make_uint4(v.x, v.y, v.z, v.w);
         }
+        int64_t k0 = k_beg + kro;
         // threads [0, NG*KC) stage the chunk's row weights
         sg.wv = (T)0;
         if ((int)threadIdx.x < NG * KC) {
@@ -606,21 +632,26 @@ struct GramTile {
         }
     }
 
-    __device__ inline void sstore(const Stage& sg, T* st)
+    // Columns at or beyond r of a 16-byte vector that straddles r are zeroed
+    // here rather than after the load, so that no load is waited for before
+    // the chunk that needs it.
+    __device__ inline void sstore(Stage& sg, T* st, int64_t ci, int64_t cj, int64_t r)
     {
 #pragma unroll
         for (int j = 0; j < NVI; ++j) {
-            int lin = threadIdx.x + 256 * j;
+            int lin = threadIdx.x + NT * j;
             if (lin < KC * VI) {
                 int kr = lin / VI, cv = lin % VI;
+                mask(sg.ri[j], ci + cv * EPV, r);
                 *reinterpret_cast<uint4*>(st + kr * PI + cv * EPV) = sg.ri[j];
             }
         }
 #pragma unroll
         for (int j = 0; j < NVJ; ++j) {
-            int lin = threadIdx.x + 256 * j;
+            int lin = threadIdx.x + NT * j;
             if (lin < KC * VJ) {
                 int kr = lin / VJ, cv = lin % VJ;
+                mask(sg.rj[j], cj + cv * EPV, r);
                 *reinterpret_cast<uint4*>(st + KC * PI + kr * PJ + cv * EPV) = sg.rj[j];
             }
         }
@@ -701,6 +732,8 @@ struct GramTile {
                 x[q] = (double)acc[g][a][b][v];
             }
             if (!first) {
+                // all 16 loads issue before the first add uses one
+                double y[16];
 #pragma unroll
                 for (int q = 0; q < 16; ++q) {
                     int e = b0 + q;
@@ -709,8 +742,11 @@ struct GramTile {
                     int row = (a / IL) * M::MT * IL + IL * M::row_off(v) + (a % IL);
                     int col = (b / IL) * M::MT * IL + (b % IL);
                     int soff = (g * BI * BJ + row * BJ + col) * (int)sizeof(double);
-                    x[q] += __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+                    y[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
                 }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < 16; ++q) x[q] += y[q];
             }
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
@@ -728,7 +764,7 @@ struct GramTile {
 };
 
 template <typename T, int NG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ __launch_bounds__((GramCfg<T, NG>::NT)) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T* __restrict__ w1,
             const T* __restrict__ w2, const int* __restrict__ tileI, const int* __restrict__ tileJ,
             int ntiles, int nitems, int64_t rows_per_split, double* part)
@@ -746,11 +782,14 @@ void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T*
     int64_t k_beg = (int64_t)split * rows_per_split;
     int64_t k_end = k_beg + rows_per_split < n ? k_beg + rows_per_split : n;
     double* out = part + ((int64_t)split * ntiles + tile) * (int64_t)NG * G::BI * G::BJ;
+    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T*>(F + k_beg * ldf), (short)0,
+        (int)(k_end > k_beg ? (k_end - k_beg) * ldf * (int64_t)sizeof(T) : 0), 0x00020000);
 
     G g;
     int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    g.wm = w >> 1;
-    g.wn = w & 1;
+    g.wm = w / G::C::WGN;
+    g.wn = w % G::C::WGN;
     g.h = lane / Mfma<T>::MT;
     g.li = lane % Mfma<T>::MT;
     // on tiles that straddle the diagonal, a wave block with every row index
@@ -763,7 +802,7 @@ void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T*
     auto step = [&](const T* cur) {
         g.compute(cur);
         since += G::KC;
-        if (since >= GramCfg<T>::PROMOTE) {
+        if (since >= G::C::PROMOTE) {
             g.promote(out, first);
             first = false;
             since = 0;
@@ -771,25 +810,25 @@ void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T*
     };
     typename G::Stage s0;
     if (nk > 0) {
-        g.gload(s0, F, ldf, r, ci, cj, k_beg, k_end, w1, w2);
-        g.sstore(s0, lds);
+        g.gload(s0, rs, ldf, ci, cj, 0, k_beg, k_end, w1, w2);
+        g.sstore(s0, lds, ci, cj, r);
     }
     if constexpr (NG == 1) {
         // Loads run two chunks ahead (the single Gram has the registers):
         // chunk c multiplies from stage c & 1 while c + 1 waits in one
         // register set and c + 2 loads into the other.
         typename G::Stage s1;
-        if (nk > 1) g.gload(s1, F, ldf, r, ci, cj, k_beg + G::KC, k_end, w1, w2);
+        if (nk > 1) g.gload(s1, rs, ldf, ci, cj, G::KC, k_beg, k_end, w1, w2);
         __syncthreads();
         for (int64_t c = 0; c < nk; c += 2) {
-            if (c + 2 < nk) g.gload(s0, F, ldf, r, ci, cj, k_beg + (c + 2) * G::KC, k_end, w1, w2);
+            if (c + 2 < nk) g.gload(s0, rs, ldf, ci, cj, (c + 2) * G::KC, k_beg, k_end, w1, w2);
             step(lds);
-            if (c + 1 < nk) g.sstore(s1, lds + G::STAGE);
+            if (c + 1 < nk) g.sstore(s1, lds + G::STAGE, ci, cj, r);
             __syncthreads();
             if (c + 1 >= nk) break;
-            if (c + 3 < nk) g.gload(s1, F, ldf, r, ci, cj, k_beg + (c + 3) * G::KC, k_end, w1, w2);
+            if (c + 3 < nk) g.gload(s1, rs, ldf, ci, cj, (c + 3) * G::KC, k_beg, k_end, w1, w2);
             step(lds + G::STAGE);
-            if (c + 2 < nk) g.sstore(s0, lds);
+            if (c + 2 < nk) g.sstore(s0, lds, ci, cj, r);
             __syncthreads();
         }
     } else {
@@ -798,9 +837,9 @@ void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T*
             T* cur = lds + (kc & 1) * G::STAGE;
             T* nxt = lds + ((kc + 1) & 1) * G::STAGE;
             bool more = kc + 1 < nk;
-            if (more) g.gload(s0, F, ldf, r, ci, cj, k_beg + (kc + 1) * G::KC, k_end, w1, w2);
+            if (more) g.gload(s0, rs, ldf, ci, cj, (kc + 1) * G::KC, k_beg, k_end, w1, w2);
             step(cur);
-            if (more) g.sstore(s0, nxt);
+            if (more) g.sstore(s0, nxt, ci, cj, r);
             __syncthreads();
         }
     }
@@ -993,7 +1032,7 @@ kevd_status rowdot(kevd_ctx* ctx, const double* c, double* y, int mode, const do
 template <typename T, int NG>
 kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, double* G2)
 {
-    typedef GramCfg<T> C;
+    typedef GramCfg<T, NG> C;
     int64_t n = ctx->n, r = ctx->rank;
     int ti_n = (int)((r + C::BI - 1) / C::BI), tj_n = (int)((r + C::BJ - 1) / C::BJ);
     std::vector<int> TI, TJ;
@@ -1005,15 +1044,20 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
             }
     int ntiles = (int)TI.size();
     // At least ~2048 workgroups, with the split count chosen so that the
-    // grid fills whole waves of the device's resident slots (2 per CU).
-    const int64_t slots = 2 * (int64_t)ctx->num_cu;
-    int ns0 = (int)std::max<int64_t>(1, (2048 + ntiles - 1) / ntiles);
+    // grid fills whole waves of the device's resident slots.
+    const int64_t slots = C::WG_PER_CU * (int64_t)ctx->num_cu;
+    // A split's rows are addressed by 32-bit buffer offsets: at most 2^31
+    // bytes of F per split.
+    const int64_t rp_max = ((int64_t)INT32_MAX / (ctx->ldf * (int64_t)sizeof(T))) / C::KC * C::KC - C::KC;
+    if (rp_max < C::KC) return kevd_fail(ctx, KEVD_EINVAL, "Gram: factor rows too long for 32-bit offsets");
+    int ns0 = (int)std::max<int64_t>({(int64_t)1, (2048 + ntiles - 1) / ntiles, (n + rp_max - 1) / rp_max});
     int nsplit = 1;
     int64_t rps = n;
     double best = -1.0;
     for (int ns = ns0; ns <= 2 * ns0; ++ns) {
         int64_t rp = KEVD_ROUND_UP((n + ns - 1) / ns, C::KC);
         if (rp < 2048) rp = 2048;
+        if (rp > rp_max) rp = rp_max;
         int64_t nse = (n + rp - 1) / rp, w = nse * ntiles;
         double eff = (double)w / (double)(slots * ((w + slots - 1) / slots));
         if (eff > best + 1e-9) {
@@ -1039,7 +1083,7 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
     KEVD_HIP(ctx, hipMemcpyAsync(dTJ, TJ.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     {
         ProfScope ps(ctx, TC_GRAM);
-        k_gram<T, NG><<<nitems, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, wt1, wt2, dTI, dTJ,
+        k_gram<T, NG><<<nitems, C::NT, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, wt1, wt2, dTI, dTJ,
                                                       ntiles, nitems, rps, part);
     }
     k_gram_reduce<C::BI, C::BJ, NG><<<ntiles, 256, 0, ctx->stream>>>(part, nsplit, ntiles, dTI, dTJ, r,

@@ -89,6 +89,29 @@ def test_sinkhorn_mode(gpu_ctx):
     assert abs(res.lam[-1] - 1.0) < 1e-8
 
 
+@pytest.mark.parametrize("n,rank", [(1200, 150), (6000, 600)])
+def test_sinkhorn_mixed_vs_oracle(gpu_ctx, n, rank):
+    # FP32 factor: one Gram F^T S^2 F (lower-triangle tiles, ragged edge at
+    # rank 600) against the FP64 oracle on the same FP32-rounded factor
+    X, eps, out, F = _factor(gpu_ctx, n=n, rank=rank, seed=4)
+    F32 = F.astype(np.float32)
+    gpu_ctx.set_factor(F32)
+    k = 32
+    res = gpu_ctx.evd(out.rank, k_out=k, prec="mixed", norm="sinkhorn", sinkhorn_tol=1e-12,
+                      sinkhorn_max_iter=5000)
+    lam, U, s, it = orc.lowrank_sinkhorn_evd(F32.astype(np.float64), k, tol=1e-12, max_iter=5000)
+    rel = np.abs(res.lam[-k:] - lam[-k:]) / np.abs(lam[-k:])
+    print("mixed sinkhorn rank", rank, "iters", res.sinkhorn_iters, it, "top-32 max rel err", rel.max(),
+          flush=True)
+    assert res.rowsum_dev <= 1e-12
+    assert rel.max() < 1e-5
+    Ua = _align(res.U.astype(np.float64), U)
+    gaps = np.abs(np.diff(lam[-k - 1:]))
+    ok = (gaps[:-1] > 1e-4) & (gaps[1:] > 1e-4)
+    err = np.max(np.abs(Ua - U), axis=0)
+    assert np.all(err[-8:][ok[-8:]] < 1e-3), err[-8:]
+
+
 def test_nonpositive_degree_status(gpu_ctx):
     X = gauss_points(400, 3, seed=8)
     gpu_ctx.set_points(X)
