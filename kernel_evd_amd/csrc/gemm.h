@@ -168,18 +168,27 @@ struct NTGemm {
         }
     }
 
-    // Like gload, for chunks that lie wholly below kvalid: only the row
-    // bound is checked, so no load waits on a masking step.
-    template <int NV>
-    __device__ static inline void gload_full(uint4 (&r)[NV], const T* src, int64_t ld,
-                                             int64_t nvalid, int64_t k0)
+    // Like gload, for chunks that lie wholly below kvalid, through a buffer
+    // resource over the operand's valid rows (rows_rsrc): a row past them
+    // reads as zero by the hardware's range check, so the loads carry no
+    // predicate, no branch and no 64-bit address arithmetic.
+    __device__ static inline __amdgpu_buffer_rsrc_t rows_rsrc(const T* src, int64_t ld, int64_t nvalid, int nmax)
     {
+        int64_t rows = nvalid < nmax ? (nvalid > 0 ? nvalid : 0) : nmax;
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(src), (short)0,
+                                                 (int)(rows * ld * (int64_t)sizeof(T)), 0x00020000);
+    }
+    template <int NV>
+    __device__ static inline void gload_full(uint4 (&r)[NV], __amdgpu_buffer_rsrc_t rs, int64_t ld, int64_t k0)
+    {
+        typedef __attribute__((ext_vector_type(4))) unsigned u4;
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
             int lin = threadIdx.x + 256 * j;
             int row = lin / VPR, cv = lin % VPR;
-            r[j] = row < nvalid ? *reinterpret_cast<const uint4*>(src + (int64_t)row * ld + k0 + cv * EPV)
-                                : make_uint4(0, 0, 0, 0);
+            u4 v = __builtin_amdgcn_raw_buffer_load_b128(
+                rs, (uint32_t)(((int64_t)row * ld + k0 + cv * EPV) * (int64_t)sizeof(T)), 0, 0);
+            r[j] = make_uint4(v.x, v.y, v.z, v.w);
         }
     }
 
@@ -256,10 +265,12 @@ struct NTGemm {
     {
         int64_t nk = (K + KC - 1) / KC, nfull = K / KC;
         if (nk <= 0) return;
+        // A block is at most BM x lda (BN x ldb) elements: 32-bit offsets
+        const __amdgpu_buffer_rsrc_t rsA = rows_rsrc(A, lda, arows, BM), rsB = rows_rsrc(B, ldb, brows, BN);
         auto load = [&](uint4 (&ra)[NVA], uint4 (&rb)[NVB], int64_t c) {
             if (c < nfull) {
-                gload_full<NVA>(ra, A, lda, arows, c * KC);
-                gload_full<NVB>(rb, B, ldb, brows, c * KC);
+                gload_full<NVA>(ra, rsA, lda, c * KC);
+                gload_full<NVB>(rb, rsB, ldb, c * KC);
             } else {
                 gload<NVA>(ra, A, lda, arows, c * KC, K);
                 gload<NVB>(rb, B, ldb, brows, c * KC, K);
