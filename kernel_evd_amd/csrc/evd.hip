@@ -10,7 +10,8 @@
 //   W  = R B R^T;  W = V diag(lam) V^T             dense 2 (1764-1913)
 //   U  = D~^-1 F R^-1 V_k                          Z step + pass 4 (1973-2275)
 // plus P 1 - 1 (row-sum deviation) from one extra output column of the U GEMM.
-// Sinkhorn mode: s .* (K^ s) = 1 by s <- sqrt(s / K^ s), one Gram F^T S^2 F,
+// Sinkhorn mode: s .* (K^ s) = 1 by a Chebyshev-accelerated, scale-free
+// Sinkhorn-Knopp iteration (one pass over F each), one Gram F^T S^2 F,
 // U = S F V_k Lambda_k^-1/2.
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -169,16 +170,18 @@ __global__ __launch_bounds__(256) void k_rowdot(const T* __restrict__ F, int64_t
 }
 
 // One fused Sinkhorn pass over F (symmetric scaling of K^ = F F^T).
-// With the current iterate s = bscale * b and u = uscale * uraw = F^T s,
-// each row i gives y_i = F_i . u = (K^ s)_i, the scaled row sum
-// rs_i = s_i y_i and the next iterate
-//   undamped (Sinkhorn-Knopp):  b'_i = 1 / y_i
-//   damped:                     b'_i = sqrt(s_i / y_i),
-// and the same loaded row adds b'_i F_i to this block's partial of
-// F^T b' -- so one iteration streams F once instead of twice.  A block
-// owns SK_ROWS rows and walks them SKR at a time; each thread keeps VPT
-// 16-byte column vectors of u and of its F^T b' partial in registers.
-// Per block: bstat = [min y, max |rs - 1|, sum rs, min s].
+// With the iterate s and u = F^T s, each row i gives y_i = F_i . u =
+// (K^ s)_i, the scaled row sum rs_i = s_i y_i and the vector the next
+// iterate is built from,
+//   q_i = 1 / y_i            (Sinkhorn-Knopp / Chebyshev steps)
+//   q_i = sqrt(s_i / y_i)    (damped),
+// and the same loaded row adds q_i F_i to this block's partial of F^T q
+// -- so one iteration streams F once instead of twice.  A block owns
+// SK_ROWS rows and walks them SKR at a time; each thread keeps VPT 16-byte
+// column vectors of u and of its F^T q partial in registers.
+// Per block: bstat = [min y, max |rs - 1|, sum rs, min s, sum (rs - 1)^2].
+constexpr int SK_NSTAT = 5;
+
 constexpr int SK_ROWS = 512;
 
 template <typename T> struct SkRows;
@@ -187,9 +190,8 @@ template <> struct SkRows<double> { static constexpr int R = 2; };
 
 template <typename T, int VPT>
 __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, int64_t ldf, int64_t n,
-                                                       int64_t r, const double* __restrict__ uraw,
-                                                       double uscale, const double* __restrict__ b,
-                                                       double bscale, int damped, double* b_next,
+                                                       int64_t r, const double* __restrict__ u,
+                                                       const double* __restrict__ s, int damped, double* q_out,
                                                        double* part, int64_t rpad, double* bstat)
 {
     constexpr int V = 16 / (int)sizeof(T), R = SkRows<T>::R;
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             int64_t c = ((int64_t)threadIdx.x + 256 * q) * V + e;
-            uv[q][e] = c < r ? uscale * uraw[c] : 0.0;
+            uv[q][e] = c < r ? u[c] : 0.0;
             acc[q][e] = 0.0;
         }
     // two register sets, each indexed only by compile-time constants
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
                     f[rr][q] = vec_t{};
             }
     };
-    double mn = INFINITY, dv = 0.0, srs = 0.0, smin = INFINITY;
+    double mn = INFINITY, dv = 0.0, srs = 0.0, smin = INFINITY, se2 = 0.0;
     auto group = [&](const vec_t (&f)[R][VPT], int64_t i0, double (&rd)[4][R]) {
         double fv[R][VPT][V];
 #pragma unroll
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
         // the iterate's entries for these rows, read before the barrier
         double sv[R];
 #pragma unroll
-        for (int rr = 0; rr < R; ++rr) sv[rr] = i0 + rr < r1 ? bscale * b[i0 + rr] : 1.0;
+        for (int rr = 0; rr < R; ++rr) sv[rr] = i0 + rr < r1 ? s[i0 + rr] : 1.0;
         __syncthreads();
 #pragma unroll
         for (int rr = 0; rr < R; ++rr) {
@@ -257,11 +259,12 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
             double rs = si * y;
             double bn = damped ? sqrt(si / y) : 1.0 / y;
             if (threadIdx.x == 0) {
-                b_next[i] = bn;
+                q_out[i] = bn;
                 mn = fmin(mn, y);
                 dv = fmax(dv, fabs(rs - 1.0));
                 srs += rs;
                 smin = fmin(smin, si);
+                se2 += (rs - 1.0) * (rs - 1.0);
             }
 #pragma unroll
             for (int q = 0; q < VPT; ++q)
@@ -290,10 +293,11 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
             if (c < r) part[(int64_t)blockIdx.x * rpad + c] = acc[q][e];
         }
     if (threadIdx.x == 0) {
-        bstat[4 * blockIdx.x + 0] = mn;
-        bstat[4 * blockIdx.x + 1] = dv;
-        bstat[4 * blockIdx.x + 2] = srs;
-        bstat[4 * blockIdx.x + 3] = smin;
+        bstat[SK_NSTAT * blockIdx.x + 0] = mn;
+        bstat[SK_NSTAT * blockIdx.x + 1] = dv;
+        bstat[SK_NSTAT * blockIdx.x + 2] = srs;
+        bstat[SK_NSTAT * blockIdx.x + 3] = smin;
+        bstat[SK_NSTAT * blockIdx.x + 4] = se2;
     }
 }
 
@@ -309,14 +313,13 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
 // the FP64 factor uses k_sinkhorn_pass.
 template <typename T, int VPQ>
 __global__ __launch_bounds__(256) void k_sinkhorn_rows(const T* __restrict__ F, int64_t ldf, int64_t n,
-                                                       int64_t r, const double* __restrict__ uraw,
-                                                       double uscale, const double* __restrict__ b,
-                                                       double bscale, int damped, double* b_next,
+                                                       int64_t r, const double* __restrict__ u,
+                                                       const double* __restrict__ s, int damped, double* q_out,
                                                        double* part, int64_t rpad, double* bstat)
 {
     constexpr int V = 16 / (int)sizeof(T);
     typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type vec_t;
-    __shared__ double red[4][4];
+    __shared__ double red[4][SK_NSTAT];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t r0 = (int64_t)blockIdx.x * SK_ROWS;
     const int64_t r1 = r0 + SK_ROWS < n ? r0 + SK_ROWS : n;
@@ -326,7 +329,7 @@ __global__ __launch_bounds__(256) void k_sinkhorn_rows(const T* __restrict__ F, 
 #pragma unroll
         for (int e = 0; e < V; ++e) {
             int64_t c = ((int64_t)lane + 64 * q) * V + e;
-            uv[q][e] = c < r ? uscale * uraw[c] : 0.0;
+            uv[q][e] = c < r ? u[c] : 0.0;
             acc[q][e] = 0.0;
         }
     auto load = [&](vec_t (&f)[VPQ], double& sb, int64_t i) {
@@ -335,9 +338,9 @@ __global__ __launch_bounds__(256) void k_sinkhorn_rows(const T* __restrict__ F, 
             int64_t c = ((int64_t)lane + 64 * q) * V;
             f[q] = (i < r1 && c < r) ? *reinterpret_cast<const vec_t*>(F + i * ldf + c) : vec_t{};
         }
-        sb = i < r1 ? b[i] : 1.0;
+        sb = i < r1 ? s[i] : 1.0;
     };
-    double mn = INFINITY, dv = 0.0, srs = 0.0, smin = INFINITY;
+    double mn = INFINITY, dv = 0.0, srs = 0.0, smin = INFINITY, se2 = 0.0;
     // F values are widened twice (dot, then accumulation) rather than held
     // as doubles, which would cost VPQ * V * 2 more registers
     auto fval = [&](const vec_t& f, int q, int k) {
@@ -351,15 +354,16 @@ __global__ __launch_bounds__(256) void k_sinkhorn_rows(const T* __restrict__ F, 
 #pragma unroll
             for (int k = 0; k < V; ++k) d += fval(f[q], q, k) * uv[q][k];
         double y = wave_sum(d);
-        double si = bscale * sb;
+        double si = sb;
         double rs = si * y;
         double bn = damped ? sqrt(si / y) : 1.0 / y;
         if (lane == 0) {
-            b_next[i] = bn;
+            q_out[i] = bn;
             mn = fmin(mn, y);
             dv = fmax(dv, fabs(rs - 1.0));
             srs += rs;
             smin = fmin(smin, si);
+            se2 += (rs - 1.0) * (rs - 1.0);
         }
 #pragma unroll
         for (int q = 0; q < VPQ; ++q)
@@ -394,6 +398,7 @@ __global__ __launch_bounds__(256) void k_sinkhorn_rows(const T* __restrict__ F, 
                 red[w][1] = dv;
                 red[w][2] = srs;
                 red[w][3] = smin;
+                red[w][4] = se2;
             }
         }
         __syncthreads();
@@ -404,28 +409,32 @@ __global__ __launch_bounds__(256) void k_sinkhorn_rows(const T* __restrict__ F, 
         if (c < r) part[(int64_t)blockIdx.x * rpad + c] = cpart[idx];
     }
     if (threadIdx.x == 0) {
-        bstat[4 * blockIdx.x + 0] = fmin(fmin(red[0][0], red[1][0]), fmin(red[2][0], red[3][0]));
-        bstat[4 * blockIdx.x + 1] = fmax(fmax(red[0][1], red[1][1]), fmax(red[2][1], red[3][1]));
-        bstat[4 * blockIdx.x + 2] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
-        bstat[4 * blockIdx.x + 3] = fmin(fmin(red[0][3], red[1][3]), fmin(red[2][3], red[3][3]));
+        bstat[SK_NSTAT * blockIdx.x + 0] = fmin(fmin(red[0][0], red[1][0]), fmin(red[2][0], red[3][0]));
+        bstat[SK_NSTAT * blockIdx.x + 1] = fmax(fmax(red[0][1], red[1][1]), fmax(red[2][1], red[3][1]));
+        bstat[SK_NSTAT * blockIdx.x + 2] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
+        bstat[SK_NSTAT * blockIdx.x + 3] = fmin(fmin(red[0][3], red[1][3]), fmin(red[2][3], red[3][3]));
+        bstat[SK_NSTAT * blockIdx.x + 4] = (red[0][4] + red[1][4]) + (red[2][4] + red[3][4]);
     }
 }
 
-// [min, max, sum, min] over the nb per-block quadruples, in block order.
+// [min, max, sum, min, sum] over the nb per-block Sinkhorn statistics, in
+// block order.
 __global__ __launch_bounds__(256) void k_stat_reduce(const double* bstat, int nb, double* out)
 {
-    __shared__ double smn[256], sdv[256], ssm[256], ssn[256];
-    double mn = INFINITY, dv = 0.0, sm = 0.0, sn = INFINITY;
+    __shared__ double smn[256], sdv[256], ssm[256], ssn[256], se2[256];
+    double mn = INFINITY, dv = 0.0, sm = 0.0, sn = INFINITY, e2 = 0.0;
     for (int j = threadIdx.x; j < nb; j += 256) {
-        mn = fmin(mn, bstat[4 * j]);
-        dv = fmax(dv, bstat[4 * j + 1]);
-        sm += bstat[4 * j + 2];
-        sn = fmin(sn, bstat[4 * j + 3]);
+        mn = fmin(mn, bstat[SK_NSTAT * j]);
+        dv = fmax(dv, bstat[SK_NSTAT * j + 1]);
+        sm += bstat[SK_NSTAT * j + 2];
+        sn = fmin(sn, bstat[SK_NSTAT * j + 3]);
+        e2 += bstat[SK_NSTAT * j + 4];
     }
     smn[threadIdx.x] = mn;
     sdv[threadIdx.x] = dv;
     ssm[threadIdx.x] = sm;
     ssn[threadIdx.x] = sn;
+    se2[threadIdx.x] = e2;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
@@ -433,6 +442,7 @@ __global__ __launch_bounds__(256) void k_stat_reduce(const double* bstat, int nb
             sdv[threadIdx.x] = fmax(sdv[threadIdx.x], sdv[threadIdx.x + o]);
             ssm[threadIdx.x] += ssm[threadIdx.x + o];
             ssn[threadIdx.x] = fmin(ssn[threadIdx.x], ssn[threadIdx.x + o]);
+            se2[threadIdx.x] += se2[threadIdx.x + o];
         }
         __syncthreads();
     }
@@ -441,7 +451,22 @@ __global__ __launch_bounds__(256) void k_stat_reduce(const double* bstat, int nb
         out[1] = sdv[0];
         out[2] = ssm[0];
         out[3] = ssn[0];
+        out[4] = se2[0];
     }
+}
+
+// out = a x + b y + c z over r entries (a Sinkhorn iterate over the n
+// points, or its F^T image over the rank); a term with a zero coefficient
+// is not read (its vector may be unset).  out may alias x, y or z.
+__global__ void k_lincomb3(int64_t r, double a, const double* x, double b, const double* y, double c,
+                           const double* z, double* out)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= r) return;
+    double v = a != 0.0 ? a * x[j] : 0.0;
+    if (b != 0.0) v += b * y[j];
+    if (c != 0.0) v += c * z[j];
+    out[j] = v;
 }
 
 // Per-block minimum of x (grid-stride), for a second-level k_block_minmax.
@@ -1606,6 +1631,62 @@ kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* re
     return KEVD_OK;
 }
 
+// Step schedule of the fused Sinkhorn iteration: the coefficients of
+// s_{j+1} = a1 s_j + a2 s_{j-1} + a3 T(s_j) after the pass that measured
+// s_j.  In the error of s the map T acts as -P on the modes below the
+// scale, with the spectrum of P in [0, lam_2] (P is positive semidefinite).
+// The first NPLAIN steps are plain (s <- T(s)); then the Chebyshev
+// semi-iteration for I + P on [1, 1 + lam] (Golub and Varga, Numer. Math.
+// 3 (1961) 147; the recurrence of Saad, Iterative Methods for Sparse
+// Linear Systems, 2nd ed., Alg. 12.1) contracts the error by about
+// 1 / (sig + sqrt(sig^2 - 1)) per step, sig = (2 + lam) / lam.  lam starts
+// at 0.3; when the RMS deviation falls more slowly than that by a factor
+// 1.5 over two steps, the mode that decays at the observed rate sits at
+// lam' = lam (x + 1) / 2 with x from the Chebyshev growth outside the
+// interval, and the recurrence restarts with 1.1 lam' (at least 1.25 lam,
+// at most 0.995).
+struct SkCheb {
+    static constexpr int NPLAIN = 2;
+    double lam = 0.3;
+    int j = 0;  // steps decided
+    int k = 0;  // steps of the current Chebyshev run
+    double rho = 0.0;
+    std::vector<double> e2;  // RMS deviation of every measured iterate
+
+    void next(double err, double* a1, double* a2, double* a3)
+    {
+        e2.push_back(err);
+        ++j;
+        if (j <= NPLAIN) {
+            *a1 = 0.0, *a2 = 0.0, *a3 = 1.0;
+            return;
+        }
+        double th = 1.0 + 0.5 * lam, de = 0.5 * lam, sig = th / de;
+        size_t m = e2.size();
+        if (k >= 3 && e2[m - 3] > 0.0) {
+            double g = sig + std::sqrt(sig * sig - 1.0);
+            double q = std::sqrt(e2[m - 1] / e2[m - 3]);
+            if (q > 1.5 / g) {
+                double z = q * g, x = 0.5 * (z + 1.0 / z);
+                lam = std::min(0.995, std::max(1.25 * lam, 1.1 * lam * (x + 1.0) * 0.5));
+                k = 0;
+                th = 1.0 + 0.5 * lam, de = 0.5 * lam, sig = th / de;
+            }
+        }
+        double alpha, beta;
+        if (k == 0) {
+            rho = 1.0 / sig;
+            alpha = 0.0, beta = 1.0 / th;
+        } else {
+            double rn = 1.0 / (2.0 * sig - rho);
+            alpha = rn * rho, beta = 2.0 * rn / de;
+            rho = rn;
+        }
+        ++k;
+        *a1 = 1.0 - beta + alpha, *a2 = -alpha, *a3 = beta;
+    }
+};
+
 template <typename T>
 kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res)
 {
@@ -1641,36 +1722,53 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     constexpr int V = 16 / (int)sizeof(T), VPT_MAX = sizeof(T) == 4 ? 4 : 8;
     int vpt = (int)(((r + V - 1) / V + 255) / 256);
     if (vpt <= VPT_MAX) {
-        // One fused pass per iteration.  Undamped Sinkhorn-Knopp
-        // s <- 1 / (K^ s): near the fixed point the error in log s
-        // contracts by P = S K^ S, whose eigenvalue 1 (the common scale of
-        // s) does not decay but alternates; it is removed by rescaling
-        // with a = sqrt(mean(s .* K^ s)), so the rate is the second
-        // eigenvalue of P.  Three rises of the deviation in a row switch to
-        // the damped update s <- sqrt(s / K^ s) (rate <= 1/2).
+        // One fused pass per iteration.  Near the fixed point s* the error
+        // e in s = s* (1 + e) maps under s <- 1 / (K^ s) to -P e, with
+        // P = S* K^ S* bistochastic and positive semidefinite (K^ = F F^T).
+        // Its eigenvalue 1 (the common scale of s) is removed exactly by
+        // T(s) = sqrt(mean(s .* K^ s)) / (K^ s), which is invariant under
+        // s -> c s; every other mode of the map lies in [-lam_2, 0].  The
+        // steps are SkCheb's: two plain steps s <- T(s), then the Chebyshev
+        // semi-iteration for the spectrum [0, lam_2] (three-term recurrence
+        // on s and, by linearity, on u = F^T s).  Three rises of the
+        // deviation in a row switch to the damped update s <- sqrt(s / K^ s)
+        // (rate <= 1/2).
         int nb = (int)((n + SK_ROWS - 1) / SK_ROWS);
         int64_t rpad = KEVD_ROUND_UP(r, 4);
         double *part, *bstat, *st3;
         KEVD_TRY(ctx_buf_t(ctx, "e_skpart", (size_t)nb * rpad, &part));
-        KEVD_TRY(ctx_buf_t(ctx, "e_skstat", (size_t)4 * nb, &bstat));
-        KEVD_TRY(ctx_buf_t(ctx, "e_skst3", 4, &st3));
-        double* u2;
-        KEVD_TRY(ctx_buf_t(ctx, "e_u2", r, &u2));
-        // s_0 = c0 * 1 and u_0 = F^T s_0 = c0 * (F^T 1), which u holds
-        double c0 = 1.0 / std::sqrt(mean), bscale = c0, uscale = c0;
-        k_fill_d<<<gn, 256, 0, ctx->stream>>>(s, n, 1.0);
-        bool damped = false;
+        KEVD_TRY(ctx_buf_t(ctx, "e_skstat", (size_t)SK_NSTAT * nb, &bstat));
+        KEVD_TRY(ctx_buf_t(ctx, "e_skst3", SK_NSTAT, &st3));
+        // iterate s_j in Sb[j % 3] (built by k_lincomb3 before its pass),
+        // q_j in q; u_j = F^T s_j in ucur, u_{j-1} in uprev, v_j = F^T q_j
+        // in uv
+        double *s3, *q, *uprev, *uv, *un;
+        KEVD_TRY(ctx_buf_t(ctx, "e_s3", n, &s3));
+        KEVD_TRY(ctx_buf_t(ctx, "e_q0", n, &q));
+        KEVD_TRY(ctx_buf_t(ctx, "e_u2", r, &uprev));
+        KEVD_TRY(ctx_buf_t(ctx, "e_uv", r, &uv));
+        KEVD_TRY(ctx_buf_t(ctx, "e_un", r, &un));
+        double* Sb[3] = {s, s2, s3};
+        double* ucur = u;
+        unsigned gr = (unsigned)((r + 255) / 256);
+        // s_0 = c0 * 1 and u_0 = c0 F^T 1
+        double c0 = 1.0 / std::sqrt(mean);
+        k_fill_d<<<gn, 256, 0, ctx->stream>>>(Sb[0], n, c0);
+        k_lincomb3<<<gr, 256, 0, ctx->stream>>>(r, c0, u, 0.0, nullptr, 0.0, nullptr, ucur);
+        int damped = 0;
+        SkCheb cheb;
         int rises = 0;
-        double prev = INFINITY, st[4];
-        for (it = 1; it <= maxit; ++it) {
+        double prev = INFINITY, st[SK_NSTAT];
+        int j = 0;
+        for (it = 1; it <= maxit; ++it, ++j) {
             {
                 ProfScope ps(ctx, TC_MATVEC);
 #define KEVD_SK_ROWS(VQ)                                                                                     \
-    k_sinkhorn_rows<T, VQ><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, u, uscale, s,     \
-                                                       bscale, damped ? 1 : 0, s2, part, rpad, bstat)
+    k_sinkhorn_rows<T, VQ><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, ucur, Sb[j % 3],  \
+                                                       damped, q, part, rpad, bstat)
 #define KEVD_SK_PASS(VP)                                                                                     \
-    k_sinkhorn_pass<T, VP><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, u, uscale, s,     \
-                                                       bscale, damped ? 1 : 0, s2, part, rpad, bstat)
+    k_sinkhorn_pass<T, VP><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, ucur, Sb[j % 3],  \
+                                                       damped, q, part, rpad, bstat)
                 const int vpq = (int)(((r + V - 1) / V + 63) / 64);
                 bool rows_done = false;
                 if constexpr (sizeof(T) == 4) {
@@ -1695,11 +1793,11 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
                     KEVD_SK_PASS(VPT_MAX);
 #undef KEVD_SK_PASS
 #undef KEVD_SK_ROWS
-                k_colsum_reduce<<<(unsigned)((r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, r, rpad, u2);
+                k_colsum_reduce<<<(unsigned)((r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, r, rpad, uv);
                 k_stat_reduce<<<1, 256, 0, ctx->stream>>>(bstat, nb, st3);
             }
             KEVD_HIP(ctx, hipGetLastError());
-            KEVD_HIP(ctx, hipMemcpyAsync(st, st3, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KEVD_HIP(ctx, hipMemcpyAsync(st, st3, SK_NSTAT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
             KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
             if (!(st[0] > 0.0)) {
                 // the first pass sees K^ 1 up to a positive constant: a degree
@@ -1708,17 +1806,26 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
             }
             dev = st[1];
             smin_fused = st[3];
-            if (dev <= tol || it == maxit) break;  // s = bscale * b is the iterate measured
-            bool was_damped = damped;
+            s = Sb[j % 3];  // the iterate this pass measured
+            if (dev <= tol || it == maxit) break;
+            bool was_damped = damped != 0;
             rises = dev > prev ? rises + 1 : 0;
             prev = dev;
-            if (!damped && rises >= 3) damped = true;
-            bscale = was_damped ? 1.0 : std::sqrt(st[2] / (double)n);
-            uscale = bscale;  // u2 = F^T b_next
-            std::swap(s, s2);
-            std::swap(u, u2);
+            // s_{j+1} = a1 s_j + a2 s_{j-1} + a3 q_j
+            double a1, a2, a3;
+            if (was_damped) {
+                a1 = 0.0, a2 = 0.0, a3 = 1.0;
+            } else {
+                double m = st[2] / (double)n;
+                cheb.next(std::sqrt(st[4] / (double)n), &a1, &a2, &a3);
+                a3 *= std::sqrt(m);
+            }
+            k_lincomb3<<<gr, 256, 0, ctx->stream>>>(r, a1, ucur, a2, uprev, a3, uv, un);
+            k_lincomb3<<<gn, 256, 0, ctx->stream>>>(n, a1, Sb[j % 3], a2, Sb[(j + 2) % 3], a3, q, Sb[(j + 1) % 3]);
+            std::swap(uprev, ucur);  // uprev = u_j
+            std::swap(ucur, un);     // ucur = u_{j+1}; un free
+            damped = (was_damped || rises >= 3) ? 1 : 0;
         }
-        if (bscale != 1.0) k_scale_d<<<gn, 256, 0, ctx->stream>>>(s, n, bscale);
     } else {
         // rank beyond the register tiling: two passes per damped iteration
         k_fill_d<<<gn, 256, 0, ctx->stream>>>(s, n, 1.0 / std::sqrt(mean));
