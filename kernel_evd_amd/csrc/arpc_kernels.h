@@ -269,76 +269,134 @@ __device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, c
 // ------------------------------------------------------- rejection block
 
 // Algorithm 2.1 on H (rejection_sample, ks_arpclm.cu:300-408) in one
-// workgroup with H resident in LDS.  Every thread evaluates the acceptance
-// test itself (same Philox counter, same LDS values), so a rejected proposal
-// costs no barrier.  Column i of L is kept transposed in the unused upper
-// triangle of H, its diagonal in Ld.  Work inside a step is spread over a
-// 16 x 16 thread grid (no integer division).  After the loop the compacted
-// inverse X = L_c^-1 is formed in place in the lower triangle by a
-// right-looking triangular inversion (k sequential steps, each a parallel
-// rank-1 update) instead of one thread per column.
+// workgroup of 256 threads.  The Schur complement lives in registers: on a
+// 16 x 16 thread grid, thread (tx, ty) owns H(ty + 16a, tx + 16b) for
+// a, b < 8, so a step's rank-1 update is 64 register FMAs after 16 LDS
+// reads.  The current diagonal and the proposals' uniforms sit in LDS, and
+// every thread evaluates the acceptance test itself, so a rejected proposal
+// costs no barrier; an accepted one costs two (its column of L published,
+// then the updated diagonal).  Column c of L (the c-th acceptance) is kept
+// by proposal row in Lst[c][.].  The compacted inverse X = L_c^-1 is then
+// formed in registers on the same grid by a right-looking triangular
+// inversion: step c finalises row c (its owners divide by L_c(c, c) and
+// publish it) and every later row subtracts L_c(a, c) times it, one barrier
+// per step (the published row is double buffered).
 template <typename T>
-__device__ inline void rejchol_body(const double* __restrict__ H, ArpcState* st, uint64_t seed, double tol,
-                                    int inject, const int64_t* __restrict__ Sprime, int64_t* Si, T* LinvT,
-                                    double* Linv64, int32_t* Spos = nullptr)
+__device__ __attribute__((always_inline)) inline void rejchol_body(const double* __restrict__ H, ArpcState* st, uint64_t seed, double tol,
+                                                                   int inject, const int64_t* __restrict__ Sprime,
+                                                                   int64_t* Si, T* LinvT, double* Linv64,
+                                                                   int32_t* Spos = nullptr)
 {
+    static_assert(BMAX == 128, "the 16 x 16 grid of 8 x 8 register blocks covers 128 x 128");
     if (st->done) return;
     const int round = st->round;
-    __shared__ double Hs[BMAX][BMAX + 1];
-    __shared__ double u[BMAX], Lcol[BMAX], Ld[BMAX];
+    __shared__ double Lst[BMAX][BMAX];
+    __shared__ double dg[BMAX], u0[BMAX], rndv[BMAX], xrow[2][BMAX];
     __shared__ int accpos[BMAX];
-    int np = st->nprop;
-    int tid = threadIdx.x;
-    int tx = tid & 15, ty = tid >> 4;
-    for (int p = ty; p < np; p += 16)
-        for (int q = tx; q < np; q += 16) Hs[p][q] = H[p * BMAX + q];
-    __syncthreads();
-    if (tid < np) u[tid] = Hs[tid][tid];
+    const int np = st->nprop;
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    double h[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) h[a][b] = H[(ty + 16 * a) * BMAX + tx + 16 * b];
+    if (tx == ty) {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            dg[ty + 16 * a] = h[a][a];
+            u0[ty + 16 * a] = h[a][a];
+        }
+    }
+    if (tid < BMAX)
+        rndv[tid] = kevd_u64_to_unit(kevd_draw_u64(seed, KEVD_STREAM_REJECT, (uint32_t)round, (uint32_t)tid));
     __syncthreads();
     int k = 0;
     for (int i = 0; i < np; ++i) {
-        double rnd = kevd_u64_to_unit(kevd_draw_u64(seed, KEVD_STREAM_REJECT, (uint32_t)round, (uint32_t)i));
-        double hii = Hs[i][i];
-        bool acc = inject ? (hii > tol) : ((u[i] * rnd < hii) && (hii > tol));
+        const double hii = dg[i];
+        const bool acc = inject ? (hii > tol) : ((u0[i] * rndv[i] < hii) && (hii > tol));
         if (!acc) continue;
-        double f = 1.0 / sqrt(hii);
-        if (tid >= i && tid < np) Lcol[tid] = Hs[tid][i] * f;
-        __syncthreads();
-        for (int jj = i + 1 + ty; jj < np; jj += 16) {
-            double lj = Lcol[jj];
-            for (int mm = i + 1 + tx; mm <= jj; mm += 16) Hs[jj][mm] -= lj * Lcol[mm];
+        const double f = 1.0 / sqrt(hii);
+        if (tx == (i & 15)) {
+            const int bi = i >> 4;
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                double v = 0.0;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) v = b == bi ? h[a][b] : v;
+                int row = ty + 16 * a;
+                if (row >= i) Lst[k][row] = v * f;
+            }
         }
-        if (tid > i && tid < np) Hs[i][tid] = Lcol[tid];
-        if (tid == i) Ld[i] = Lcol[i];
         if (tid == 0) accpos[k] = i;
+        __syncthreads();
+        double lr[8], lc[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            int row = ty + 16 * a, col = tx + 16 * a;
+            lr[a] = (row > i && row < np) ? Lst[k][row] : 0.0;
+            lc[a] = (col > i && col < np) ? Lst[k][col] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int b = 0; b < 8; ++b) h[a][b] -= lr[a] * lc[b];
+        if (tx == ty) {
+#pragma unroll
+            for (int a = 0; a < 8; ++a) dg[ty + 16 * a] = h[a][a];
+        }
         ++k;
         __syncthreads();
     }
     int64_t room = st->rank - st->Nr;
     if ((int64_t)k > room) k = (int)room;
-    // L_c(a, c) = L(accpos[a], accpos[c]) = Hs[accpos[c]][accpos[a]] for a > c,
-    // L_c(a, a) = Ld[accpos[a]].  X = L_c^-1 lives at Hs[a][t], a >= t
-    // (compacted indices, strictly below the upper-triangle storage).
-    for (int a = ty; a < k; a += 16)
-        for (int t = tx; t <= a; t += 16) Hs[a][t] = (a == t) ? 1.0 : 0.0;
-    __syncthreads();
-    for (int a = 0; a < k; ++a) {
-        double inv = 1.0 / Ld[accpos[a]];
-        if (tid <= a) Hs[a][tid] *= inv;  // finalise row a (k <= 128 < 256 threads)
-        __syncthreads();
-        int pa = accpos[a];
-        for (int j = a + 1 + ty; j < k; j += 16) {
-            double l = Hs[pa][accpos[j]];
-            for (int t = tx; t <= a; t += 16) Hs[j][t] -= l * Hs[a][t];
+    // X = L_c^-1 with L_c(a, c) = Lst[c][accpos[a]] (a >= c); thread (tx, ty)
+    // owns X(ty + 16a, tx + 16b).
+    int apos[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) apos[a] = ty + 16 * a < k ? accpos[ty + 16 * a] : 0;
+    double x[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) x[a][b] = (ty + 16 * a == tx + 16 * b && ty + 16 * a < k) ? 1.0 : 0.0;
+    for (int c = 0; c < k; ++c) {
+        const int buf = c & 1;
+        if (ty == (c & 15)) {
+            const int ac = c >> 4;
+            const double inv = 1.0 / Lst[c][accpos[c]];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+                if (a == ac) {
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        x[a][b] *= inv;
+                        xrow[buf][tx + 16 * b] = x[a][b];
+                    }
+                }
         }
         __syncthreads();
+        double l[8], xr[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            int row = ty + 16 * a;
+            l[a] = (row > c && row < k) ? Lst[c][apos[a]] : 0.0;
+            xr[a] = xrow[buf][tx + 16 * a];
+        }
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int b = 0; b < 8; ++b) x[a][b] -= l[a] * xr[b];
     }
-    for (int e = tid; e < BMAX * BMAX; e += 256) {
-        int a = e / BMAX, c = e % BMAX;
-        double v = (a < k && c <= a) ? Hs[a][c] : 0.0;
-        LinvT[e] = (T)v;
-        Linv64[e] = v;
-    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            int row = ty + 16 * a, col = tx + 16 * b;
+            double v = (row < k && col <= row) ? x[a][b] : 0.0;
+            LinvT[row * BMAX + col] = (T)v;
+            Linv64[row * BMAX + col] = v;
+        }
     if (tid < BMAX) Si[tid] = tid < k ? Sprime[accpos[tid]] : -1;
     // positions in S' of the accepted proposals, when asked for
     if (Spos && tid < BMAX) Spos[tid] = tid < k ? accpos[tid] : 0;
