@@ -1365,6 +1365,157 @@ __global__ __launch_bounds__(1024) void k_potrf_upper(double* A, int l, int* inf
     if (tid == 0) *info = bad;
 }
 
+// Householder tridiagonalisation of the symmetric l x l FP64 matrix M
+// (column-major in A, lda = l, l <= SYTRD_LMAX; both triangles read) in one
+// workgroup, with LAPACK dsytrd's lower-storage output (Golub and Van Loan,
+// Matrix Computations, 4th ed., Alg. 8.3.1): D, E the tridiagonal, and
+// reflector j = I - tau_j v v^T with v(j+1) = 1 and v(j+2:l) stored in
+// A(j+2:l, j), so rocSOLVER's dstedc and dormtr finish the eigensolve.
+// Reflector j is dlarfg's: beta = -sign(alpha) ||(alpha, x)||,
+// tau = (beta - alpha) / beta, v = x / (alpha - beta).  Then
+// p = tau M v, w = p - (tau / 2)(p^T v) v and M -= v w^T + w v^T on the
+// trailing block.  The lower block-triangle of M lives in registers: on
+// 32 x 32 blocks (A, B), A >= B, thread (ty < 16, tx < 32) holds rows ty and
+// ty + 16 of column tx of every block (diagonal blocks in full).  The
+// symmetric product sums each stored entry into its row and, below the
+// diagonal blocks, into its column; both partial sets are reduced through
+// LDS in a fixed order.  Four barriers per column.
+constexpr int SYTRD_LMAX = 256;
+__global__ __launch_bounds__(512) void k_sytrd_lower(double* A, int l, double* D, double* E, double* tau)
+{
+    constexpr int NBK = SYTRD_LMAX / 32, NBLK = NBK * (NBK + 1) / 2;
+    __shared__ double colv[2][SYTRD_LMAX], pv[2][SYTRD_LMAX], sc[2][2];
+    __shared__ double rpart[32][SYTRD_LMAX], cpart[16][SYTRD_LMAX];
+    const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63;
+    double e[NBLK][2];
+#pragma unroll
+    for (int a = 0; a < NBK; ++a)
+#pragma unroll
+        for (int b = 0; b <= a; ++b)
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                int row = 32 * a + ty + 16 * hh, col = 32 * b + tx;
+                e[a * (a + 1) / 2 + b][hh] = (row < l && col < l) ? A[row + (int64_t)col * l] : 0.0;
+            }
+    __syncthreads();  // every entry is in registers before A is written
+    for (int i = 0; i + 1 < l; ++i) {
+        const int q = i & 1, bi = i >> 5;
+        // column i below the diagonal (and its diagonal entry) from its owners
+        if (tx == (i & 31)) {
+#pragma unroll
+            for (int a = 0; a < NBK; ++a)
+#pragma unroll
+                for (int b = 0; b <= a; ++b)
+                    if (b == bi) {
+#pragma unroll
+                        for (int hh = 0; hh < 2; ++hh) {
+                            int row = 32 * a + ty + 16 * hh;
+                            double v = e[a * (a + 1) / 2 + b][hh];
+                            if (row > i && row < l) colv[q][row] = v;
+                            if (row == i) D[i] = v;
+                        }
+                    }
+        }
+        __syncthreads();
+        if (tid < 64) {
+            double s2 = 0.0;
+            for (int row = i + 2 + lane; row < l; row += 64) s2 += colv[q][row] * colv[q][row];
+            s2 = wave_sum(s2);
+            if (lane == 0) {
+                double alpha = colv[q][i + 1], t = 0.0, scale = 0.0, beta = alpha;
+                if (s2 > 0.0) {
+                    beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+                    t = (beta - alpha) / beta;
+                    scale = 1.0 / (alpha - beta);
+                }
+                sc[q][0] = t;
+                sc[q][1] = scale;
+                E[i] = beta;
+                tau[i] = t;
+                A[(i + 1) + (int64_t)i * l] = beta;
+            }
+        }
+        __syncthreads();
+        const double t = sc[q][0], scale = sc[q][1];
+        auto vval = [&](int x) { return x == i + 1 ? 1.0 : ((x > i + 1 && x < l) ? colv[q][x] * scale : 0.0); };
+        for (int row = i + 2 + tid; row < l; row += 512) A[row + (int64_t)i * l] = colv[q][row] * scale;
+        // p = M v on rows > i: row sums over the blocks left of the diagonal
+        // (and the diagonal block), column sums over the blocks below it
+        double vc[NBK], csum[NBK];
+#pragma unroll
+        for (int c = 0; c < NBK; ++c) {
+            vc[c] = vval(32 * c + tx);
+            csum[c] = 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < NBK; ++a) {
+            double r0 = 0.0, r1 = 0.0;
+            if (32 * a + 31 > i) {
+                const double v0 = vval(32 * a + ty), v1 = vval(32 * a + ty + 16);
+#pragma unroll
+                for (int c = 0; c <= a; ++c) {
+                    const double e0 = e[a * (a + 1) / 2 + c][0], e1 = e[a * (a + 1) / 2 + c][1];
+                    r0 += e0 * vc[c];
+                    r1 += e1 * vc[c];
+                    if (c < a) csum[c] += e0 * v0 + e1 * v1;
+                }
+            }
+            rpart[tx][32 * a + ty] = r0;
+            rpart[tx][32 * a + ty + 16] = r1;
+        }
+#pragma unroll
+        for (int c = 0; c < NBK; ++c) cpart[ty][32 * c + tx] = csum[c];
+        __syncthreads();
+        if (tid < SYTRD_LMAX) {
+            const int x = tid;
+            double sr = 0.0, sq = 0.0;
+            if (x > i && x < l) {
+#pragma unroll 8
+                for (int j = 0; j < 32; ++j) sr += rpart[j][x];
+#pragma unroll 8
+                for (int j = 0; j < 16; ++j) sq += cpart[j][x];
+            }
+            pv[q][x] = (x > i && x < l) ? t * (sr + sq) : 0.0;
+        }
+        __syncthreads();
+        // w = p - (tau / 2)(p^T v) v; every wave forms p^T v itself
+        double dot = 0.0;
+        for (int x = i + 1 + lane; x < l; x += 64) dot += pv[q][x] * vval(x);
+        dot = wave_sum(dot);
+        const double kc = -0.5 * t * dot;
+        double wc[NBK];
+#pragma unroll
+        for (int c = 0; c < NBK; ++c) wc[c] = pv[q][32 * c + tx] + kc * vc[c];
+#pragma unroll
+        for (int a = 0; a < NBK; ++a) {
+            if (32 * a + 31 > i) {
+                double vr[2], wr[2];
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh) {
+                    vr[hh] = vval(32 * a + ty + 16 * hh);
+                    wr[hh] = pv[q][32 * a + ty + 16 * hh] + kc * vr[hh];
+                }
+#pragma unroll
+                for (int c = 0; c <= a; ++c)
+#pragma unroll
+                    for (int hh = 0; hh < 2; ++hh)
+                        e[a * (a + 1) / 2 + c][hh] -= vr[hh] * wc[c] + wr[hh] * vc[c];
+            }
+        }
+    }
+    // the last diagonal entry
+    const int il = l - 1, bl = il >> 5;
+    if (tx == (il & 31)) {
+#pragma unroll
+        for (int a = 0; a < NBK; ++a)
+            if (a == bl) {
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh)
+                    if (32 * a + ty + 16 * hh == il) D[il] = e[a * (a + 1) / 2 + a][hh];
+            }
+    }
+}
+
 // Orthonormal basis of the columns of the r x l block Y, in place.
 // Householder: rocSOLVER geqrf + orgqr.  Otherwise shifted Cholesky QR 3
 // (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput.
@@ -1483,7 +1634,19 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
             BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Y, R,
                                Y, R, &zero, G, Lb));
             k_symmetrize<<<dim3((l + 15) / 16, (l + 15) / 16), 256, 0, ctx->stream>>>(H, l);
-            BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, Lb, H, Lb, th, E, info));
+            if (l <= SYTRD_LMAX) {
+                // one-workgroup tridiagonalisation, then rocSOLVER's
+                // divide and conquer on T and the back-transformation;
+                // the eigenvectors land in GS, which becomes H
+                k_sytrd_lower<<<1, 512, 0, ctx->stream>>>(H, l, th, E, tau);
+                KEVD_HIP(ctx, hipGetLastError());
+                BLAS(rocsolver_dstedc(h, rocblas_evect_tridiagonal, Lb, th, E, GS, Lb, info));
+                BLAS(rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, Lb, Lb, H,
+                                      Lb, tau, GS, Lb));
+                std::swap(H, GS);
+            } else {
+                BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, Lb, H, Lb, th, E, info));
+            }
             BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, Lb, Lb, Lb, &one, G, Lb, H,
                                Lb, &zero, GS, Lb));
             k_ritz_res2<<<k, 256, 0, ctx->stream>>>(H, GS, th, l, k, res2);
