@@ -1379,13 +1379,15 @@ __global__ __launch_bounds__(1024) void k_potrf_upper(double* A, int l, int* inf
 // ty + 16 of column tx of every block (diagonal blocks in full).  The
 // symmetric product sums each stored entry into its row and, below the
 // diagonal blocks, into its column; both partial sets are reduced through
-// LDS in a fixed order.  Four barriers per column.
+// LDS in a fixed order (the row partials padded against bank conflicts).
+// Every wave forms the reflector itself from the published column, so a
+// column costs three barriers.
 constexpr int SYTRD_LMAX = 256;
 __global__ __launch_bounds__(512) void k_sytrd_lower(double* A, int l, double* D, double* E, double* tau)
 {
     constexpr int NBK = SYTRD_LMAX / 32, NBLK = NBK * (NBK + 1) / 2;
-    __shared__ double colv[2][SYTRD_LMAX], pv[2][SYTRD_LMAX], sc[2][2];
-    __shared__ double rpart[32][SYTRD_LMAX], cpart[16][SYTRD_LMAX];
+    __shared__ double colv[2][SYTRD_LMAX], pv[2][SYTRD_LMAX];
+    __shared__ double rpart[32][SYTRD_LMAX + 1], cpart[16][SYTRD_LMAX];
     const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5, lane = tid & 63;
     double e[NBLK][2];
 #pragma unroll
@@ -1417,26 +1419,22 @@ __global__ __launch_bounds__(512) void k_sytrd_lower(double* A, int l, double* D
                     }
         }
         __syncthreads();
-        if (tid < 64) {
-            double s2 = 0.0;
-            for (int row = i + 2 + lane; row < l; row += 64) s2 += colv[q][row] * colv[q][row];
-            s2 = wave_sum(s2);
-            if (lane == 0) {
-                double alpha = colv[q][i + 1], t = 0.0, scale = 0.0, beta = alpha;
-                if (s2 > 0.0) {
-                    beta = -copysign(sqrt(alpha * alpha + s2), alpha);
-                    t = (beta - alpha) / beta;
-                    scale = 1.0 / (alpha - beta);
-                }
-                sc[q][0] = t;
-                sc[q][1] = scale;
-                E[i] = beta;
-                tau[i] = t;
-                A[(i + 1) + (int64_t)i * l] = beta;
-            }
+        // the reflector, formed identically by every wave
+        double s2 = 0.0;
+        for (int row = i + 2 + lane; row < l; row += 64) s2 += colv[q][row] * colv[q][row];
+        s2 = wave_sum(s2);
+        const double alpha = colv[q][i + 1];
+        double t = 0.0, scale = 0.0, beta = alpha;
+        if (s2 > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+            t = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
         }
-        __syncthreads();
-        const double t = sc[q][0], scale = sc[q][1];
+        if (tid == 0) {
+            E[i] = beta;
+            tau[i] = t;
+            A[(i + 1) + (int64_t)i * l] = beta;
+        }
         auto vval = [&](int x) { return x == i + 1 ? 1.0 : ((x > i + 1 && x < l) ? colv[q][x] * scale : 0.0); };
         for (int row = i + 2 + tid; row < l; row += 512) A[row + (int64_t)i * l] = colv[q][row] * scale;
         // p = M v on rows > i: row sums over the blocks left of the diagonal
