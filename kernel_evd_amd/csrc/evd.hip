@@ -548,6 +548,12 @@ template <typename T, int NG> struct GramCfg;
 #ifndef KEVD_GRAM1_CFG
 #define KEVD_GRAM1_CFG 0
 #endif
+// rows of F per split (0: chosen from the tile count and the CU count)
+#ifndef KEVD_GRAM_SPLIT_ROWS
+#define KEVD_GRAM_SPLIT_ROWS 0
+#endif
+// workgroups per tile in the split reduction
+constexpr int GRAM_REDUCE_SLICES = 8;
 template <int NG> struct GramCfg<float, NG> {
     static constexpr int BI = 128, BJ = 128, KC = 32, TM = 2, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
     static constexpr int NT = 256, WGM = 2, WGN = 2, WG_PER_CU = 2;
@@ -887,7 +893,7 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const double* part, int nsp
 {
     int tile = blockIdx.x;
     int64_t ci = (int64_t)tileI[tile] * BI, cj = (int64_t)tileJ[tile] * BJ;
-    for (int e = threadIdx.x; e < BI * BJ; e += 256) {
+    for (int e = blockIdx.y * 256 + threadIdx.x; e < BI * BJ; e += 256 * gridDim.y) {
         int row = e / BJ, col = e % BJ;
         int64_t i = ci + row, j = cj + col;
         if (i >= r || j >= r || i < j) continue;
@@ -1081,6 +1087,7 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
     double best = -1.0;
     for (int ns = ns0; ns <= 2 * ns0; ++ns) {
         int64_t rp = KEVD_ROUND_UP((n + ns - 1) / ns, C::KC);
+        if (KEVD_GRAM_SPLIT_ROWS > 0) rp = KEVD_ROUND_UP(KEVD_GRAM_SPLIT_ROWS, C::KC);
         if (rp < 2048) rp = 2048;
         if (rp > rp_max) rp = rp_max;
         int64_t nse = (n + rp - 1) / rp, w = nse * ntiles;
@@ -1111,8 +1118,8 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
         k_gram<T, NG><<<nitems, C::NT, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, wt1, wt2, dTI, dTJ,
                                                       ntiles, nitems, rps, part);
     }
-    k_gram_reduce<C::BI, C::BJ, NG><<<ntiles, 256, 0, ctx->stream>>>(part, nsplit, ntiles, dTI, dTJ, r,
-                                                                    G1, G2);
+    k_gram_reduce<C::BI, C::BJ, NG><<<dim3(ntiles, GRAM_REDUCE_SLICES), 256, 0, ctx->stream>>>(
+        part, nsplit, ntiles, dTI, dTJ, r, G1, G2);
     KEVD_HIP(ctx, hipGetLastError());
     ctx->tm.gram_flop += (double)NG * (double)n * (double)r * (double)(r + 1);
     return KEVD_OK;
