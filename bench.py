@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per launch at this workload (rocprofv3 FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, the factors a cold 2 GiB copy gives), from
 # profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md.
-TRAFFIC_BYTES = {"k_downdate<float>": 4.285e9, "k_gram<float,1>": 123.27e9,
+TRAFFIC_BYTES = {"k_downdate<float>": 4.291e9, "k_gram<float,1>": 111.20e9,
                  "k_sinkhorn_rows<float,8>": 8.507e9}
 TRAFFIC_SOURCE = "profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md"
 FP32_MFMA_PEAK_TF = 157.3
@@ -362,7 +362,7 @@ def cpu_baseline(X, eps, rank, block, kout, n_sample, norm):
     r = orc.arpc(Xs, eps, rank, block=block, seed=0)
     t1 = time.perf_counter()
     if norm == "sinkhorn":
-        orc.lowrank_sinkhorn_evd(r.F, kout, tol=SINKHORN_TOL, max_iter=1000)
+        orc.lowrank_sinkhorn_evd(r.F, kout, tol=SINKHORN_TOL, max_iter=1000, method="chebyshev")
     else:
         orc.lowrank_evd(r.F, kout)
     t2 = time.perf_counter()
@@ -610,9 +610,10 @@ def main():
         scale = args.n / ns
         out["cpu_baseline"] = {"value": round((ta + te) * scale, 3), "unit": "s", "cores": int(nt),
                                "kind": "port",
-                               "sample": "first %d of %d points, same rank/block/seed/eps/normalisation; ARPC %.2f s "
-                                         "+ kEVD %.2f s measured, scaled x%.0f (both stages linear in N)"
-                                         % (ns, args.n, ta, te, scale)}
+                               "sample": "first %d of %d points, same rank/block/seed/eps/normalisation%s; ARPC "
+                                         "%.2f s + kEVD %.2f s measured, scaled x%.0f (both stages linear in N)"
+                                         % (ns, args.n, " (Sinkhorn on the library's Chebyshev schedule)"
+                                            if norm == "sinkhorn" else "", ta, te, scale)}
         log("cpu baseline: %.2f s measured on %d points (%d threads)" % (ta + te, ns, nt))
         out["c0_small"] = c0_leg()
     if ctx is not None:

@@ -229,20 +229,70 @@ def lowrank_evd(F, k_out):
     return lam, U, P1 - 1.0, s
 
 
-def lowrank_sinkhorn_evd(F, k_out, tol=1e-12, max_iter=10000):
+def _cheb_schedule():
+    """Step coefficients of the accelerated Sinkhorn iteration (the same
+    schedule the library runs): after the pass that measured s_j with RMS
+    deviation err, s_{j+1} = a1 s_j + a2 s_{j-1} + a3 T(s_j), where
+    T(s) = sqrt(mean(s * K^ s)) / (K^ s).  Two plain steps, then the
+    Chebyshev semi-iteration for I + P with the spectrum of P taken in
+    [0, lam] (lam = 0.3 at first; raised and restarted when the deviation
+    falls more slowly than predicted by a factor 1.5 over two steps)."""
+    st = {"lam": 0.3, "j": 0, "k": 0, "rho": 0.0, "e2": []}
+
+    def nxt(err):
+        st["e2"].append(err)
+        st["j"] += 1
+        if st["j"] <= 2:
+            return 0.0, 0.0, 1.0
+        lam = st["lam"]
+        th, de = 1.0 + 0.5 * lam, 0.5 * lam
+        sig = th / de
+        e2 = st["e2"]
+        if st["k"] >= 3 and e2[-3] > 0.0:
+            g = sig + np.sqrt(sig * sig - 1.0)
+            q = np.sqrt(e2[-1] / e2[-3])
+            if q > 1.5 / g:
+                z = q * g
+                x = 0.5 * (z + 1.0 / z)
+                lam = min(0.995, max(1.25 * lam, 1.1 * lam * (x + 1.0) * 0.5))
+                st["lam"], st["k"] = lam, 0
+                th, de = 1.0 + 0.5 * lam, 0.5 * lam
+                sig = th / de
+        if st["k"] == 0:
+            st["rho"] = 1.0 / sig
+            alpha, beta = 0.0, 1.0 / th
+        else:
+            rn = 1.0 / (2.0 * sig - st["rho"])
+            alpha, beta = rn * st["rho"], 2.0 * rn / de
+            st["rho"] = rn
+        st["k"] += 1
+        return 1.0 - beta + alpha, -alpha, beta
+    return nxt
+
+
+def lowrank_sinkhorn_evd(F, k_out, tol=1e-12, max_iter=10000, method="damped"):
     """Symmetric Sinkhorn on K^ = F F^T: s .* (K^ s) = 1, P = S K^ S; the
-    nonzero spectrum is that of F^T S^2 F.  Damped fixed point
-    s <- sqrt(s / (K^ s))."""
+    nonzero spectrum is that of F^T S^2 F.  method "damped": the fixed point
+    iteration s <- sqrt(s / (K^ s)); "chebyshev": the accelerated schedule
+    of _cheb_schedule (same fixed point, fewer passes)."""
     F = np.asarray(F, dtype=np.float64)
     n, r = F.shape
     s = np.ones(n) / np.sqrt(max(1e-300, np.mean(F @ (F.T @ np.ones(n)))))
+    sprev = s
+    step = _cheb_schedule() if method == "chebyshev" else None
     it = 0
     for it in range(1, max_iter + 1):
         Ks = F @ (F.T @ s)
-        dev = np.max(np.abs(s * Ks - 1.0))
+        rs = s * Ks
+        dev = np.max(np.abs(rs - 1.0))
         if dev <= tol:
             break
-        s = np.sqrt(s / Ks)
+        if step is None:
+            s = np.sqrt(s / Ks)
+        else:
+            a1, a2, a3 = step(np.sqrt(np.mean((rs - 1.0) ** 2)))
+            snew = a1 * s + a2 * sprev + a3 * np.sqrt(np.mean(rs)) / Ks
+            sprev, s = s, snew
     G = F.T @ (F * (s * s)[:, None])
     G = 0.5 * (G + G.T)
     lam, V = np.linalg.eigh(G)

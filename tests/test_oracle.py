@@ -176,6 +176,25 @@ def test_sinkhorn_fixed_point():
     assert abs(lam[-1] - 1.0) < 1e-10
 
 
+@pytest.mark.parametrize("eps_scale", [1.0, 0.25])
+def test_sinkhorn_chebyshev_schedule_reaches_the_damped_fixed_point(eps_scale):
+    """The accelerated schedule (the library's) and the damped iteration
+    converge to the same scaling; the fixed point s .* (K^ s) = 1 is unique,
+    so any error in the schedule's coefficients either stalls it or lands
+    elsewhere, and both would fail here."""
+    X = clustered_points(1500, 6, seed=11)
+    eps = orc.median_sqdist(X) * eps_scale
+    r = orc.arpc(X, eps, 150, block=32, seed=5)
+    lam_d, _, s_d, it_d = orc.lowrank_sinkhorn_evd(r.F, 8, tol=1e-12, max_iter=5000)
+    lam_c, _, s_c, it_c = orc.lowrank_sinkhorn_evd(r.F, 8, tol=1e-12, max_iter=5000, method="chebyshev")
+    Ks = r.F @ (r.F.T @ s_c)
+    assert np.max(np.abs(s_c * Ks - 1.0)) <= 1e-12
+    assert np.max(np.abs(s_c - s_d) / s_d) < 1e-10
+    assert np.max(np.abs(lam_c[-8:] - lam_d[-8:])) < 1e-10
+    print("damped", it_d, "chebyshev", it_c)
+    assert it_c < it_d
+
+
 def _ks_lambda(L, M):
     q = 2 * np.pi / L * np.arange(M // 2 + 1)
     return q ** 2 - q ** 4
