@@ -228,26 +228,53 @@ __device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, c
     }
     __shared__ double Fp[16][65], Fq[16][65];
     int64_t sp = Sprime[p < np ? p : 0], sq = Sprime[q < np ? q : 0];
-    double acc = 0.0;
-    for (int64_t m0 = 0; m0 < Nr; m0 += 64) {
-        // stage 16 rows x 64 columns of F(S'_p) and F(S'_q)
-        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-            int rr = e >> 6, cc = e & 63;
-            int pp = by * 16 + rr, qq = bx * 16 + rr;
-            int64_t m = m0 + cc;
-            double vp = 0.0, vq = 0.0;
-            if (m < Nr) {
-                // F(S'_p, :) is row S'_p of F, or row p of the proposals' rows
-                if (pp < np) vp = (double)F[(frow_by_position ? pp : Sprime[pp]) * ldf + m];
-                if (qq < np) vq = (double)F[(frow_by_position ? qq : Sprime[qq]) * ldf + m];
-            }
-            Fp[rr][cc] = vp;
-            Fq[rr][cc] = vq;
+    // Thread t stages elements e = t + 256 j (j < 4) of each 16 x 64 chunk:
+    // rows (t >> 6) + 4 j, column t & 63.  The rows' F addresses are
+    // resolved once, and the next chunk is loaded into registers while the
+    // current one is multiplied from LDS.  The sum runs over m in order, as
+    // a plain dot product does.
+    const int cc = threadIdx.x & 63;
+    const T* rowp[4];
+    const T* rowq[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int rr = (threadIdx.x >> 6) + 4 * j, pp = by * 16 + rr, qq = bx * 16 + rr;
+        rowp[j] = pp < np ? F + (frow_by_position ? (int64_t)pp : Sprime[pp]) * ldf : nullptr;
+        rowq[j] = qq < np ? F + (frow_by_position ? (int64_t)qq : Sprime[qq]) * ldf : nullptr;
+    }
+    double vp[4], vq[4];
+    auto load = [&](int64_t m0) {
+        int64_t m = m0 + cc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            vp[j] = (m < Nr && rowp[j]) ? (double)rowp[j][m] : 0.0;
+            vq[j] = (m < Nr && rowq[j]) ? (double)rowq[j][m] : 0.0;
         }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            int rr = (threadIdx.x >> 6) + 4 * j;
+            Fp[rr][cc] = vp[j];
+            Fq[rr][cc] = vq[j];
+        }
+    };
+    double acc = 0.0;
+    if (Nr > 0) {
+        load(0);
+        store();
         __syncthreads();
+    }
+    for (int64_t m0 = 0; m0 < Nr; m0 += 64) {
+        const bool more = m0 + 64 < Nr;
+        if (more) load(m0 + 64);
 #pragma unroll 8
         for (int c = 0; c < 64; ++c) acc += Fp[ty][c] * Fq[tx][c];
         __syncthreads();
+        if (more) {
+            store();
+            __syncthreads();
+        }
     }
     double kv;
     if (sp == sq) {
