@@ -280,6 +280,9 @@ def lowrank_sinkhorn_evd(F, k_out, tol=1e-12, max_iter=10000, method="damped"):
     s = np.ones(n) / np.sqrt(max(1e-300, np.mean(F @ (F.T @ np.ones(n)))))
     sprev = s
     step = _cheb_schedule() if method == "chebyshev" else None
+    # the schedule falls back to the damped update after three rises of the
+    # deviation in a row, as the library does
+    damped, rises, prev = step is None, 0, np.inf
     it = 0
     for it in range(1, max_iter + 1):
         Ks = F @ (F.T @ s)
@@ -287,12 +290,16 @@ def lowrank_sinkhorn_evd(F, k_out, tol=1e-12, max_iter=10000, method="damped"):
         dev = np.max(np.abs(rs - 1.0))
         if dev <= tol:
             break
-        if step is None:
-            s = np.sqrt(s / Ks)
+        was_damped = damped
+        rises = rises + 1 if dev > prev else 0
+        prev = dev
+        if was_damped:
+            snew = np.sqrt(s / Ks)
         else:
             a1, a2, a3 = step(np.sqrt(np.mean((rs - 1.0) ** 2)))
             snew = a1 * s + a2 * sprev + a3 * np.sqrt(np.mean(rs)) / Ks
-            sprev, s = s, snew
+        damped = was_damped or rises >= 3
+        sprev, s = s, snew
     G = F.T @ (F * (s * s)[:, None])
     G = 0.5 * (G + G.T)
     lam, V = np.linalg.eigh(G)

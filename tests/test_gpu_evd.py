@@ -89,6 +89,31 @@ def test_sinkhorn_mode(gpu_ctx):
     assert abs(res.lam[-1] - 1.0) < 1e-8
 
 
+@pytest.mark.parametrize("eps_scale", [0.25, 0.1])
+def test_sinkhorn_small_bandwidth_schedule(gpu_ctx, eps_scale):
+    """Narrow kernels (lambda_2 of P at 0.95 and 0.9995 here) push the
+    Chebyshev schedule through its restarts.  The FP64 run must reach the
+    damped oracle's fixed point and take the oracle schedule's pass count
+    (the same coefficients from the same deviations; one pass of slack for
+    a restart test decided differently by rounding)."""
+    X = clustered_points(1500, 8, seed=7)
+    gpu_ctx.set_points(X)
+    eps = orc.median_sqdist(X) * eps_scale
+    out = gpu_ctx.arpc(eps, 200, block=32, seed=7, prec="fp64")
+    F = gpu_ctx.get_factor(out.rank, "fp64")
+    k = 16
+    res = gpu_ctx.evd(out.rank, k_out=k, prec="fp64", norm="sinkhorn", sinkhorn_tol=1e-12,
+                      sinkhorn_max_iter=5000)
+    lam_d, _, _, it_d = orc.lowrank_sinkhorn_evd(F, k, tol=1e-12, max_iter=20000)
+    _, _, _, it_c = orc.lowrank_sinkhorn_evd(F, k, tol=1e-12, max_iter=20000, method="chebyshev")
+    print("eps x", eps_scale, "gpu passes", res.sinkhorn_iters, "oracle chebyshev", it_c, "damped", it_d,
+          flush=True)
+    assert res.rowsum_dev <= 1e-12
+    np.testing.assert_allclose(res.lam[-k:], lam_d[-k:], rtol=1e-8, atol=1e-12)
+    assert abs(res.sinkhorn_iters - it_c) <= 1
+    assert res.sinkhorn_iters < it_d
+
+
 @pytest.mark.parametrize("n,rank", [(1200, 150), (6000, 600)])
 def test_sinkhorn_mixed_vs_oracle(gpu_ctx, n, rank):
     # FP32 factor: one Gram F^T S^2 F (lower-triangle tiles, ragged edge at
