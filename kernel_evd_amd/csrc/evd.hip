@@ -556,34 +556,41 @@ template <typename T, int NG> struct GramCfg;
 constexpr int GRAM_REDUCE_SLICES = 8;
 template <int NG> struct GramCfg<float, NG> {
     static constexpr int BI = 128, BJ = 128, KC = 32, TM = 2, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
-    static constexpr int NT = 256, WGM = 2, WGN = 2, WG_PER_CU = 2;
+    static constexpr int NT = 256, WGM = 2, WGN = 2, WG_PER_CU = 2, WPE = 2, PB = 16;
 };
 #if KEVD_GRAM1_CFG == 1
 template <> struct GramCfg<float, 1> {
     static constexpr int BI = 256, BJ = 128, KC = 32, TM = 2, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
-    static constexpr int NT = 512, WGM = 4, WGN = 2, WG_PER_CU = 1;
+    static constexpr int NT = 512, WGM = 4, WGN = 2, WG_PER_CU = 1, WPE = 2, PB = 16;
 };
 #elif KEVD_GRAM1_CFG == 2
 template <> struct GramCfg<float, 1> {
     static constexpr int BI = 256, BJ = 256, KC = 32, TM = 4, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
-    static constexpr int NT = 512, WGM = 2, WGN = 4, WG_PER_CU = 1;
+    static constexpr int NT = 512, WGM = 2, WGN = 4, WG_PER_CU = 1, WPE = 2, PB = 16;
 };
 #elif KEVD_GRAM1_CFG == 3
 template <> struct GramCfg<float, 1> {
     static constexpr int BI = 256, BJ = 256, KC = 16, TM = 4, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
-    static constexpr int NT = 512, WGM = 2, WGN = 4, WG_PER_CU = 1;
+    static constexpr int NT = 512, WGM = 2, WGN = 4, WG_PER_CU = 1, WPE = 2, PB = 16;
+};
+#elif KEVD_GRAM1_CFG == 4
+// three workgroups per CU: 16-row chunks and 8-value promotion batches keep
+// a workgroup within 168 VGPRs and 54 KiB of LDS
+template <> struct GramCfg<float, 1> {
+    static constexpr int BI = 128, BJ = 128, KC = 16, TM = 2, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
+    static constexpr int NT = 256, WGM = 2, WGN = 2, WG_PER_CU = 3, WPE = 3, PB = 8;
 };
 #endif
 template <int NG> struct GramCfg<double, NG> {
     static constexpr int BI = 128, BJ = 64, KC = 16, TM = 4, TN = 2, PROMOTE = 1 << 30;
-    static constexpr int NT = 256, WGM = 2, WGN = 2, WG_PER_CU = 2;
+    static constexpr int NT = 256, WGM = 2, WGN = 2, WG_PER_CU = 2, WPE = 2, PB = 16;
 };
 
 template <typename T, int NG>
 struct GramTile {
     typedef GramCfg<T, NG> C;
     typedef Mfma<T> M;
-    static constexpr int BI = C::BI, BJ = C::BJ, KC = C::KC, TM = C::TM, TN = C::TN, NT = C::NT;
+    static constexpr int BI = C::BI, BJ = C::BJ, KC = C::KC, TM = C::TM, TN = C::TN, NT = C::NT, PB = C::PB;
     static constexpr int IL = 2;                            // interleaved tiles per load
     static constexpr int EPV = 16 / (int)sizeof(T);
     static constexpr int PI = BI + EPV, PJ = BJ + EPV;
@@ -744,7 +751,7 @@ struct GramTile {
     // MFMA output row), likewise columns.  The partial tile is addressed
     // through a buffer resource: the lane part of the offset is one VGPR, the
     // compile-time part of each element's offset an SGPR constant, and the
-    // read-modify-write runs 16 values per batch so the loads overlap.
+    // read-modify-write runs PB values per batch so the loads overlap.
     __device__ inline void promote(double* out, bool first)
     {
         constexpr int NVAL = NG * TM * TN * M::ACC;
@@ -753,20 +760,20 @@ struct GramTile {
         int voff = ((wm * WR + IL * M::row_lane(h)) * BJ + wn * WC + IL * li) * (int)sizeof(double);
         typedef __attribute__((ext_vector_type(2))) unsigned u2;
 #pragma unroll
-        for (int b0 = 0; b0 < NVAL; b0 += 16) {
-            double x[16];
+        for (int b0 = 0; b0 < NVAL; b0 += PB) {
+            double x[PB];
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
+            for (int q = 0; q < PB; ++q) {
                 int e = b0 + q;
                 int v = e % M::ACC, b = (e / M::ACC) % TN, a = (e / (M::ACC * TN)) % TM,
                     g = e / (M::ACC * TN * TM);
                 x[q] = (double)acc[g][a][b][v];
             }
             if (!first) {
-                // all 16 loads issue before the first add uses one
-                double y[16];
+                // all PB loads issue before the first add uses one
+                double y[PB];
 #pragma unroll
-                for (int q = 0; q < 16; ++q) {
+                for (int q = 0; q < PB; ++q) {
                     int e = b0 + q;
                     int v = e % M::ACC, b = (e / M::ACC) % TN, a = (e / (M::ACC * TN)) % TM,
                         g = e / (M::ACC * TN * TM);
@@ -777,10 +784,10 @@ struct GramTile {
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int q = 0; q < 16; ++q) x[q] += y[q];
+                for (int q = 0; q < PB; ++q) x[q] += y[q];
             }
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
+            for (int q = 0; q < PB; ++q) {
                 int e = b0 + q;
                 int v = e % M::ACC, b = (e / M::ACC) % TN, a = (e / (M::ACC * TN)) % TM,
                     g = e / (M::ACC * TN * TM);
@@ -795,7 +802,8 @@ struct GramTile {
 };
 
 template <typename T, int NG>
-__global__ __launch_bounds__((GramCfg<T, NG>::NT)) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ __launch_bounds__((GramCfg<T, NG>::NT))
+__attribute__((amdgpu_waves_per_eu(GramCfg<T, NG>::WPE, GramCfg<T, NG>::WPE)))
 void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T* __restrict__ w1,
             const T* __restrict__ w2, const int* __restrict__ tileI, const int* __restrict__ tileJ,
             int ntiles, int nitems, int64_t rows_per_split, double* part)
