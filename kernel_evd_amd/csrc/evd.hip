@@ -485,12 +485,6 @@ __global__ __launch_bounds__(256) void k_min_partial(const double* x, int64_t n,
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
-__global__ void k_scale_d(double* p, int64_t n, double a)
-{
-    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] *= a;
-}
-
 __global__ void k_block_minmax(const double* bmin, const double* bdev, int nb, double* out)
 {
     __shared__ double smn[256], sdv[256];
