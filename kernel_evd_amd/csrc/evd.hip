@@ -1523,6 +1523,46 @@ __global__ __launch_bounds__(512) void k_sytrd_lower(double* A, int l, double* D
     }
 }
 
+// C = A^T B for column-major r x l blocks A, B (lda = ldb = r; C is l x l,
+// ldc = l), FP64.  With l x l outputs and K = r the plain GEMM runs on a
+// handful of workgroups; here K is split into GRAM_TN_SPLIT strided-batched
+// GEMMs (the remainder rows added into the first partial) and the partials
+// are summed in a fixed order.
+constexpr int GRAM_TN_SPLIT = 8;
+__global__ void k_sum_parts(const double* __restrict__ part, int ns, int64_t count, double* __restrict__ out)
+{
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    double s = 0.0;
+    for (int b = 0; b < ns; ++b) s += part[(int64_t)b * count + e];
+    out[e] = s;
+}
+
+kevd_status gram_tn(kevd_ctx* ctx, const double* A, const double* B, int64_t r, int l, double* C)
+{
+    rocblas_handle h = blas_handle(ctx);
+    const rocblas_int R = (rocblas_int)r, Lb = (rocblas_int)l;
+    double one = 1.0, zero = 0.0;
+    const int64_t kc = r / GRAM_TN_SPLIT, rem = r - GRAM_TN_SPLIT * kc;
+    if (kc < 64) {
+        BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, A, R, B, R,
+                           &zero, C, Lb));
+        return KEVD_OK;
+    }
+    const int64_t cnt = (int64_t)l * l;
+    double* part;
+    KEVD_TRY(ctx_buf_t(ctx, "q_gram_part", (size_t)GRAM_TN_SPLIT * cnt, &part));
+    BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb,
+                                       (rocblas_int)kc, &one, A, R, kc, B, R, kc, &zero, part, Lb, cnt,
+                                       GRAM_TN_SPLIT));
+    if (rem > 0)
+        BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, (rocblas_int)rem, &one,
+                           A + GRAM_TN_SPLIT * kc, R, B + GRAM_TN_SPLIT * kc, R, &one, part, Lb));
+    k_sum_parts<<<(unsigned)((cnt + 255) / 256), 256, 0, ctx->stream>>>(part, GRAM_TN_SPLIT, cnt, C);
+    KEVD_HIP(ctx, hipGetLastError());
+    return KEVD_OK;
+}
+
 // Orthonormal basis of the columns of the r x l block Y, in place.
 // Householder: rocSOLVER geqrf + orgqr.  Otherwise shifted Cholesky QR 3
 // (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput.
@@ -1545,8 +1585,7 @@ kevd_status orth_basis(kevd_ctx* ctx, double* Y, int64_t r, int l, bool househol
     double one = 1.0, zero = 0.0;
     const double c = 11.0 * ((double)r * l + (double)l * (l + 1)) * 0x1.0p-53;
     for (int pass = 0; pass < passes; ++pass) {
-        BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Y, R, Y,
-                           R, &zero, G, Lb));
+        KEVD_TRY(gram_tn(ctx, Y, Y, r, l, G));
         if (pass == 0) k_chol_shift<<<1, 256, 0, ctx->stream>>>(G, l, c);
         if (l <= CHOL_LMAX)
             k_potrf_upper<<<1, 1024, 0, ctx->stream>>>(G, l, info + pass);
@@ -1636,10 +1675,8 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
                 std::swap(Q, Y);
                 continue;
             }
-            BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Q, R,
-                               Y, R, &zero, H, Lb));
-            BLAS(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, R, &one, Y, R,
-                               Y, R, &zero, G, Lb));
+            KEVD_TRY(gram_tn(ctx, Q, Y, r, l, H));
+            KEVD_TRY(gram_tn(ctx, Y, Y, r, l, G));
             k_symmetrize<<<dim3((l + 15) / 16, (l + 15) / 16), 256, 0, ctx->stream>>>(H, l);
             if (l <= SYTRD_LMAX) {
                 // one-workgroup tridiagonalisation, then rocSOLVER's
