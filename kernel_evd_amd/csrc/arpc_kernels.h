@@ -299,11 +299,12 @@ __device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, c
 // workgroup of 256 threads.  The Schur complement lives in registers: on a
 // 16 x 16 thread grid, thread (tx, ty) owns H(ty + 16a, tx + 16b) for
 // a, b < 8, so a step's rank-1 update is 64 register FMAs after 16 LDS
-// reads.  The current diagonal and the proposals' uniforms sit in LDS, and
-// every thread evaluates the acceptance test itself, so a rejected proposal
-// costs no barrier; an accepted one costs two (its column of L published,
-// then the updated diagonal).  Column c of L (the c-th acceptance) is kept
-// by proposal row in Lst[c][.].  The compacted inverse X = L_c^-1 is then
+// reads.  Every wave keeps its own copy of the current diagonal in LDS and
+// updates it with the same FMA the owners apply to their registers, and the
+// proposals' uniforms sit in LDS, so every thread evaluates the acceptance
+// test itself: a rejected proposal costs no barrier and an accepted one
+// costs one (its column of L published).  Column c of L (the c-th
+// acceptance) is kept by proposal row in Lst[c][.].  The compacted inverse X = L_c^-1 is then
 // formed in registers on the same grid by a right-looking triangular
 // inversion: step c finalises row c (its owners divide by L_c(c, c) and
 // publish it) and every later row subtracts L_c(a, c) times it, one barrier
@@ -318,7 +319,7 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
     if (st->done) return;
     const int round = st->round;
     __shared__ double Lst[BMAX][BMAX];
-    __shared__ double dg[BMAX], u0[BMAX], rndv[BMAX], xrow[2][BMAX];
+    __shared__ double dgw[4][BMAX], u0[BMAX], rndv[BMAX], xrow[2][BMAX];
     __shared__ int accpos[BMAX];
     const int np = st->nprop;
     const int tid = threadIdx.x;
@@ -328,19 +329,20 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
     for (int a = 0; a < 8; ++a)
 #pragma unroll
         for (int b = 0; b < 8; ++b) h[a][b] = H[(ty + 16 * a) * BMAX + tx + 16 * b];
+    const int wv = tid >> 6, lane = tid & 63;
+    // this wave's diagonal copy (H is zero past the proposals)
+    dgw[wv][lane] = H[lane * BMAX + lane];
+    dgw[wv][lane + 64] = H[(lane + 64) * BMAX + lane + 64];
     if (tx == ty) {
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            dg[ty + 16 * a] = h[a][a];
-            u0[ty + 16 * a] = h[a][a];
-        }
+        for (int a = 0; a < 8; ++a) u0[ty + 16 * a] = h[a][a];
     }
     if (tid < BMAX)
         rndv[tid] = kevd_u64_to_unit(kevd_draw_u64(seed, KEVD_STREAM_REJECT, (uint32_t)round, (uint32_t)tid));
     __syncthreads();
     int k = 0;
     for (int i = 0; i < np; ++i) {
-        const double hii = dg[i];
+        const double hii = dgw[wv][i];
         const bool acc = inject ? (hii > tol) : ((u0[i] * rndv[i] < hii) && (hii > tol));
         if (!acc) continue;
         const double f = 1.0 / sqrt(hii);
@@ -368,12 +370,16 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
         for (int a = 0; a < 8; ++a)
 #pragma unroll
             for (int b = 0; b < 8; ++b) h[a][b] -= lr[a] * lc[b];
-        if (tx == ty) {
+        // the wave's diagonal copy, as the owners' h(j, j) -= L(j) L(j)
 #pragma unroll
-            for (int a = 0; a < 8; ++a) dg[ty + 16 * a] = h[a][a];
+        for (int q = 0; q < 2; ++q) {
+            const int j = lane + 64 * q;
+            if (j > i && j < np) {
+                const double l = Lst[k][j];
+                dgw[wv][j] -= l * l;
+            }
         }
         ++k;
-        __syncthreads();
     }
     int64_t room = st->rank - st->Nr;
     if ((int64_t)k > room) k = (int)room;
