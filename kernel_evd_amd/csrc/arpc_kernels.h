@@ -212,70 +212,69 @@ __device__ inline void draw_body(const double* __restrict__ psum, const uint64_t
 // 16 x 16 tile per block.  K from the FP64 centred points, direct
 // differences.  Rows/columns past the proposal count are zero.  With
 // frow_by_position the rows of F are the proposals' own rows (row p).
+// Each wave takes a quarter of m (64-column chunks, staged in its own LDS
+// slice, so no block barrier per chunk) and each lane a 2 x 2 block of the
+// tile; the four partial sums are added in wave order at the end.
 template <typename T>
 __device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, const T* __restrict__ F,
                                    int64_t ldf, const int64_t* __restrict__ Sprime, const ArpcState* st,
                                    double neg_inv_eps, double* H, int bx, int by, bool frow_by_position = false)
 {
     if (st->done) return;
-    int np = st->nprop;
-    int64_t Nr = st->Nr;
-    int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
-    int p = by * 16 + ty, q = bx * 16 + tx;
+    const int np = st->nprop;
+    const int64_t Nr = st->Nr;
+    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    const int p = by * 16 + ty, q = bx * 16 + tx;
     if (by * 16 >= np || bx * 16 >= np) {
         if (p < BMAX && q < BMAX) H[p * BMAX + q] = 0.0;
         return;
     }
-    __shared__ double Fp[16][65], Fq[16][65];
-    int64_t sp = Sprime[p < np ? p : 0], sq = Sprime[q < np ? q : 0];
-    // Thread t stages elements e = t + 256 j (j < 4) of each 16 x 64 chunk:
-    // rows (t >> 6) + 4 j, column t & 63.  The rows' F addresses are
-    // resolved once, and the next chunk is loaded into registers while the
-    // current one is multiplied from LDS.  The sum runs over m in order, as
-    // a plain dot product does.
-    const int cc = threadIdx.x & 63;
-    const T* rowp[4];
-    const T* rowq[4];
+    __shared__ double Fp[4][16][65], Fq[4][16][65];
+    __shared__ double part[4][16][17];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // this wave's m range, in whole 64-column chunks
+    const int64_t nch = (Nr + 63) / 64, per = (nch + 3) / 4;
+    const int64_t c_beg = wv * per, c_end = c_beg + per < nch ? c_beg + per : nch;
+    // lane -> rows 2 rp, 2 rp + 1 of the tile, columns 2 cp, 2 cp + 1
+    const int rp = lane >> 3, cp = lane & 7;
+    // staging: lane holds column `lane` of the chunk for all 16 rows
+    const T* rowp[16];
+    const T* rowq[16];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        int rr = (threadIdx.x >> 6) + 4 * j, pp = by * 16 + rr, qq = bx * 16 + rr;
-        rowp[j] = pp < np ? F + (frow_by_position ? (int64_t)pp : Sprime[pp]) * ldf : nullptr;
-        rowq[j] = qq < np ? F + (frow_by_position ? (int64_t)qq : Sprime[qq]) * ldf : nullptr;
+    for (int r = 0; r < 16; ++r) {
+        int pp = by * 16 + r, qq = bx * 16 + r;
+        rowp[r] = pp < np ? F + (frow_by_position ? (int64_t)pp : Sprime[pp]) * ldf : nullptr;
+        rowq[r] = qq < np ? F + (frow_by_position ? (int64_t)qq : Sprime[qq]) * ldf : nullptr;
     }
-    double vp[4], vq[4];
-    auto load = [&](int64_t m0) {
-        int64_t m = m0 + cc;
+    double acc00 = 0.0, acc01 = 0.0, acc10 = 0.0, acc11 = 0.0;
+    for (int64_t c = c_beg; c < c_end; ++c) {
+        const int64_t m = c * 64 + lane;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            vp[j] = (m < Nr && rowp[j]) ? (double)rowp[j][m] : 0.0;
-            vq[j] = (m < Nr && rowq[j]) ? (double)rowq[j][m] : 0.0;
+        for (int r = 0; r < 16; ++r) {
+            Fp[wv][r][lane] = (m < Nr && rowp[r]) ? (double)rowp[r][m] : 0.0;
+            Fq[wv][r][lane] = (m < Nr && rowq[r]) ? (double)rowq[r][m] : 0.0;
         }
-    };
-    auto store = [&]() {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            int rr = (threadIdx.x >> 6) + 4 * j;
-            Fp[rr][cc] = vp[j];
-            Fq[rr][cc] = vq[j];
-        }
-    };
-    double acc = 0.0;
-    if (Nr > 0) {
-        load(0);
-        store();
-        __syncthreads();
-    }
-    for (int64_t m0 = 0; m0 < Nr; m0 += 64) {
-        const bool more = m0 + 64 < Nr;
-        if (more) load(m0 + 64);
+        // the slice is this wave's own: its writes are visible to its reads
+        // in order, with no block barrier
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll 8
-        for (int c = 0; c < 64; ++c) acc += Fp[ty][c] * Fq[tx][c];
-        __syncthreads();
-        if (more) {
-            store();
-            __syncthreads();
+        for (int cc = 0; cc < 64; ++cc) {
+            const double a0 = Fp[wv][2 * rp][cc], a1 = Fp[wv][2 * rp + 1][cc];
+            const double b0 = Fq[wv][2 * cp][cc], b1 = Fq[wv][2 * cp + 1][cc];
+            acc00 += a0 * b0;
+            acc01 += a0 * b1;
+            acc10 += a1 * b0;
+            acc11 += a1 * b1;
         }
+        __builtin_amdgcn_wave_barrier();
     }
+    part[wv][2 * rp][2 * cp] = acc00;
+    part[wv][2 * rp][2 * cp + 1] = acc01;
+    part[wv][2 * rp + 1][2 * cp] = acc10;
+    part[wv][2 * rp + 1][2 * cp + 1] = acc11;
+    __syncthreads();
+    const double acc = ((part[0][ty][tx] + part[1][ty][tx]) + part[2][ty][tx]) + part[3][ty][tx];
+    const int64_t sp = Sprime[p < np ? p : 0], sq = Sprime[q < np ? q : 0];
     double kv;
     if (sp == sq) {
         kv = 1.0;
