@@ -466,9 +466,10 @@ template <typename T> struct DowndateLds {
 
 // Epilogue constants of one downdate row block, staged in LDS before the
 // distance product so their global loads overlap it: the block's row norms
-// and the pivots' norms and indices.
+// and residual diagonal, and the pivots' norms and indices.
 template <typename T> struct DowndateEp {
     double nrm[Cfg<T>::BM];
+    double d[Cfg<T>::BM];
     double nrmS[Cfg<T>::BN];
     int64_t Si[Cfg<T>::BN];
 };
@@ -478,14 +479,18 @@ template <typename T> struct DowndateEp {
 // and the compensated exp; exactly 1 on a row's own pivot).
 template <typename T>
 __device__ inline void dd_prologue(typename Cfg<T>::G& g, const T* __restrict__ X, int32_t dp,
-                                   const double* __restrict__ nrm, int64_t n, const T* __restrict__ XS,
-                                   const double* __restrict__ nrmS, const int64_t* __restrict__ Si, int k,
-                                   GaussScale gs, int64_t r0, T* lds, DowndateEp<T>* ep)
+                                   const double* __restrict__ nrm, const double* __restrict__ d, int64_t n,
+                                   const T* __restrict__ XS, const double* __restrict__ nrmS,
+                                   const int64_t* __restrict__ Si, int k, GaussScale gs, int64_t r0, T* lds,
+                                   DowndateEp<T>* ep)
 {
     typedef typename Cfg<T>::G G;
     constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
     int64_t rows = n - r0;
-    for (int i = threadIdx.x; i < BM; i += 256) ep->nrm[i] = r0 + i < n ? nrm[r0 + i] : 0.0;
+    for (int i = threadIdx.x; i < BM; i += 256) {
+        ep->nrm[i] = r0 + i < n ? nrm[r0 + i] : 0.0;
+        ep->d[i] = r0 + i < n ? d[r0 + i] : 0.0;
+    }
     for (int i = threadIdx.x; i < BN; i += 256) {
         ep->nrmS[i] = nrmS[i];
         ep->Si[i] = Si[i];
@@ -494,32 +499,50 @@ __device__ inline void dd_prologue(typename Cfg<T>::G& g, const T* __restrict__ 
     g.set_live_cols(k);
     // distance product (its barriers also publish ep)
     g.mainloop(lds, X + r0 * dp, dp, rows, XS, dp, BN, dp);
+    // The lane's row and column constants are read once; the element loop
+    // is then branch-free (out-of-range rows and columns are masked to 0).
+    double nr[G::NTM][G::M::ACC], ns[G::NTN];
+    int rl[G::NTM][G::M::ACC], sp[G::NTN];
+    bool rv[G::NTM][G::M::ACC], cv[G::NTN];
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int v = 0; v < G::M::ACC; ++v) {
+            rl[a][v] = g.row_of(a, v);
+            rv[a][v] = rl[a][v] < rows;
+            nr[a][v] = ep->nrm[rl[a][v]];
+        }
+#pragma unroll
+    for (int b = 0; b < G::NTN; ++b) {
+        int col = g.col_of(b);
+        cv[b] = col < k;
+        ns[b] = ep->nrmS[col];
+        int64_t s = ep->Si[col] - r0;  // the pivot's row within this block, if it lies here
+        sp[b] = s >= 0 && s < BM ? (int)s : -1;
+    }
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
 #pragma unroll
         for (int b = 0; b < G::NTN; ++b)
 #pragma unroll
             for (int v = 0; v < G::M::ACC; ++v) {
-                int64_t row = r0 + g.row_of(a, v);
-                int col = g.col_of(b);
-                double kv = 0.0;
-                if (row < n && col < k) {
-                    double D = ep->nrm[row - r0] + ep->nrmS[col] - 2.0 * (double)g.acc[a][b][v];
-                    D = D > 0.0 ? D : 0.0;
-                    kv = (row == ep->Si[col]) ? 1.0 : (double)gauss_val(D, gs, (T*)nullptr);
-                }
-                g.acc[a][b][v] = (T)(-kv);
+                double D = nr[a][v] + ns[b] - 2.0 * (double)g.acc[a][b][v];
+                D = D > 0.0 ? D : 0.0;
+                T kv = gauss_val(D, gs, (T*)nullptr);
+                kv = rl[a][v] == sp[b] ? (T)1 : kv;
+                kv = cv[b] && rv[a][v] ? kv : (T)0;
+                g.acc[a][b][v] = -kv;
             }
 }
 
 // Last stage: with acc = -(K(R,S_i) - F_R F_{S_i}^T) (the overlap added by
 // the caller, which leaves the LDS free behind a barrier), T goes to LDS,
 // G = T L_c^-T (L_c^-T is triangular), G is written to F(R, Nr:Nr+k) when F
-// is given, and d(R) = max(d(R) - rowsq(G), 0).
+// is given, and d(R) = max(d(R) - rowsq(G), 0) (the old d(R) from ep).
 template <typename T>
 __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64_t ldf, double* __restrict__ d,
                                  int64_t n, const T* __restrict__ LinvT, int k, int64_t Nr, int64_t r0, T* lds,
-                                 double (*rs)[Cfg<T>::BM])
+                                 double (*rs)[Cfg<T>::BM], const DowndateEp<T>* ep)
 {
     typedef typename Cfg<T>::G G;
     typedef DowndateLds<T> DL;
@@ -573,7 +596,7 @@ __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64
             double sq = 0.0;
 #pragma unroll
             for (int q = 0; q < WN; ++q) sq += rs[q][r];
-            double dd = d[row] - sq;
+            double dd = ep->d[r] - sq;
             d[row] = dd >= 0.0 ? dd : 0.0;
         }
     }
@@ -606,10 +629,10 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
     int64_t Nr = st->Nr;
     int64_t r0 = blk * BM;
     G g;
-    dd_prologue<T>(g, X, dp, nrm, n, XS, nrmS, Si, k, gs, r0, lds, ep);
+    dd_prologue<T>(g, X, dp, nrm, d, n, XS, nrmS, Si, k, gs, r0, lds, ep);
     // overlap with the current factor (ends with a barrier)
     if (Nr > 0) g.mainloop_live(lds, F + r0 * ldf, ldf, n - r0, FS, ldf, BN, Nr);
-    dd_finish<T>(g, F, ldf, d, n, LinvT, k, Nr, r0, lds, rs);
+    dd_finish<T>(g, F, ldf, d, n, LinvT, k, Nr, r0, lds, rs, ep);
 }
 
 // ------------------------------------------- regenerating (LM) variant
@@ -728,16 +751,34 @@ __device__ inline void downdate_lm_body(const T* __restrict__ X, int32_t dp, con
     int64_t Nr = st->Nr;
     int64_t r0 = blk * BM;
     G g;
-    dd_prologue<T>(g, X, dp, nrm, n, XS, nrmS, Si, k, gs, r0, lds, ep);
+    dd_prologue<T>(g, X, dp, nrm, d, n, XS, nrmS, Si, k, gs, r0, lds, ep);
     T* ldsD = lds;
     T* Tc = lds + GD::LDS_ELEMS;
     for (int64_t c0 = 0; c0 < Nr; c0 += LM_CH) {
         const int cw = Nr - c0 < LM_CH ? (int)(Nr - c0) : LM_CH;
         GD gd;
         gd.init();
+        // the lane's pivot norms and pivot rows (within this block, else -1)
+        // are loaded ahead of the distance product, which covers their latency
+        double ns[GD::NTN];
+        int sp[GD::NTN];
+#pragma unroll
+        for (int b = 0; b < GD::NTN; ++b) {
+            int col = gd.col_of(b);
+            bool live = col < cw;
+            ns[b] = live ? nrmP[c0 + col] : 0.0;
+            int64_t s = live ? piv[c0 + col] - r0 : -1;
+            sp[b] = s >= 0 && s < BM ? (int)s : -1;
+        }
         // the previous chunk's reads of ldsD ended behind its barrier; those
         // of Tc end before any wave passes this mainloop's barriers
         gd.mainloop(ldsD, X + r0 * dp, dp, n - r0, XP + c0 * dp, dp, cw, dp);
+        double nr[GD::NTM][GD::M::ACC];
+#pragma unroll
+        for (int a = 0; a < GD::NTM; ++a)
+#pragma unroll
+            for (int v = 0; v < GD::M::ACC; ++v) nr[a][v] = ep->nrm[gd.row_of(a, v)];
+        // branch-free element loop: out-of-range rows and columns masked to 0
 #pragma unroll
         for (int a = 0; a < GD::NTM; ++a)
 #pragma unroll
@@ -746,19 +787,18 @@ __device__ inline void downdate_lm_body(const T* __restrict__ X, int32_t dp, con
                 for (int v = 0; v < GD::M::ACC; ++v) {
                     int rl = gd.row_of(a, v);
                     int col = gd.col_of(b);
-                    double kv = 0.0;
-                    if (r0 + rl < n && col < cw) {
-                        double D = ep->nrm[rl] + nrmP[c0 + col] - 2.0 * (double)gd.acc[a][b][v];
-                        D = D > 0.0 ? D : 0.0;
-                        kv = (r0 + rl == piv[c0 + col]) ? 1.0 : (double)gauss_val(D, gs, (T*)nullptr);
-                    }
-                    Tc[rl * TPC + col] = (T)kv;
+                    double D = nr[a][v] + ns[b] - 2.0 * (double)gd.acc[a][b][v];
+                    D = D > 0.0 ? D : 0.0;
+                    T kv = gauss_val(D, gs, (T*)nullptr);
+                    kv = rl == sp[b] ? (T)1 : kv;
+                    kv = r0 + rl < n && col < cw ? kv : (T)0;
+                    Tc[rl * TPC + col] = kv;
                 }
         __syncthreads();
         g.lds_global(Tc, TPC, MT + c0, ldm, LM_CH);
     }
     __syncthreads();  // every wave is done with Tc before T overwrites the LDS
-    dd_finish<T>(g, nullptr, 0, d, n, LinvT, k, Nr, r0, lds, rs);
+    dd_finish<T>(g, nullptr, 0, d, n, LinvT, k, Nr, r0, lds, rs, ep);
 }
 
 // Block of BMAX threads.

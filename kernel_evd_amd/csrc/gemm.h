@@ -448,7 +448,10 @@ template <> struct Cfg<double> {
 // the argument a = -D log2(e) / eps is formed in FP64 and split into an
 // integer part and a fraction f in [-1/2, 1/2]; 2^f runs on v_exp_f32 and
 // the integer part is an exact exponent shift, so a large |a| costs no
-// accuracy (the FP32 rounding of a itself would cost |a| ulps).
+// accuracy (the FP32 rounding of a itself would cost |a| ulps).  Below
+// a = -160 the value is 0, chosen by a select (no branch, so a caller's
+// unrolled loop stays one basic block); the clamp keeps the integer
+// conversion in range there.
 struct GaussScale {
     double neg_inv_eps;        // -1 / eps
     double neg_log2e_inv_eps;  // -log2(e) / eps
@@ -460,8 +463,8 @@ __device__ inline double gauss_val(double D, const GaussScale& g, double*)
 __device__ inline float gauss_val(double D, const GaussScale& g, float*)
 {
     double a = D * g.neg_log2e_inv_eps;
-    if (a < -160.0) return 0.0f;
-    double ni = rint(a);
-    float f = (float)(a - ni);
-    return ldexpf(__builtin_amdgcn_exp2f(f), (int)ni);
+    double ni = rint(a > -200.0 ? a : -200.0);
+    float f = (float)fma(D, g.neg_log2e_inv_eps, -ni);  // the fraction from the unrounded product
+    float e = ldexpf(__builtin_amdgcn_exp2f(f), (int)ni);
+    return a < -160.0 ? 0.0f : e;
 }
