@@ -560,23 +560,39 @@ __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64
     g.lds_global_tri_live(lds, TP, LinvT, BMAX, (k + G::KSUB - 1) / G::KSUB * G::KSUB);  // L_c^-T is triangular
     // write the new columns and the squared row norms
     double part[G::NTM][G::M::ACC];
+    int ro[G::NTM][G::M::ACC];  // element offset of the lane's rows from F(r0, Nr)
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
 #pragma unroll
-        for (int v = 0; v < G::M::ACC; ++v) part[a][v] = 0.0;
+        for (int v = 0; v < G::M::ACC; ++v) {
+            part[a][v] = 0.0;
+            ro[a][v] = g.row_of(a, v) * (int)ldf;
+        }
+    if (F) {
+        // buffer stores over F(r0 : r0 + valid rows, Nr : ldf): a row past n
+        // falls outside the range and is dropped by the hardware, and the
+        // offsets are 32-bit (BM ldf elements per block)
+        int64_t vrows = n - r0 < BM ? n - r0 : BM;
+        __amdgpu_buffer_rsrc_t rsF = __builtin_amdgcn_make_buffer_rsrc(
+            F + r0 * ldf + Nr, (short)0, (int)((vrows * ldf - Nr) * (int64_t)sizeof(T)), 0x00020000);
+#pragma unroll
+        for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+            for (int b = 0; b < G::NTN; ++b)
+#pragma unroll
+                for (int v = 0; v < G::M::ACC; ++v) {
+                    int col = g.col_of(b);
+                    if (col < k) buffer_store(g.acc[a][b][v], rsF, (uint32_t)((ro[a][v] + col) * (int)sizeof(T)));
+                }
+    }
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
 #pragma unroll
         for (int b = 0; b < G::NTN; ++b)
 #pragma unroll
             for (int v = 0; v < G::M::ACC; ++v) {
-                int64_t row = r0 + g.row_of(a, v);
-                int col = g.col_of(b);
-                T gv = g.acc[a][b][v];
-                if (col < k) {
-                    if (F && row < n) F[row * ldf + Nr + col] = gv;
-                    part[a][v] += (double)gv * (double)gv;
-                }
+                double gv = (double)g.acc[a][b][v];
+                part[a][v] += g.col_of(b) < k ? gv * gv : 0.0;
             }
     // reduce over the MT lanes that share a row (lane % MT = column)
     constexpr int MT = G::M::MT;

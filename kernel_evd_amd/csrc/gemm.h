@@ -12,6 +12,18 @@
 
 #include "common.h"
 
+// One element stored through a buffer resource (byte offset off); a store
+// outside the resource's range is dropped by the hardware.
+__device__ inline void buffer_store(float v, __amdgpu_buffer_rsrc_t rs, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, off, 0, 0);
+}
+__device__ inline void buffer_store(double v, __amdgpu_buffer_rsrc_t rs, uint32_t off)
+{
+    typedef __attribute__((ext_vector_type(2))) unsigned u2;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rs, off, 0, 0);
+}
+
 template <typename T, int BM, int BN, int KC, int WM, int WN, int TM, int TN, typename MF = Mfma<T>>
 struct NTGemm {
     typedef MF M;
