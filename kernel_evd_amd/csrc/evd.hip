@@ -923,25 +923,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                                                float* U, double* P1)
 {
     typedef typename Cfg<T>::G G;
+    constexpr int BM = Cfg<T>::BM;
     __shared__ __attribute__((aligned(16))) T lds[G::LDS_ELEMS];
+    __shared__ double scs[BM];  // the block's row scales
     G g;
     g.init();
-    int64_t r0 = (int64_t)blockIdx.x * Cfg<T>::BM;
+    int64_t r0 = (int64_t)blockIdx.x * BM;
     int c0 = blockIdx.y * Cfg<T>::BN;
+    for (int i = threadIdx.x; i < BM; i += 256) scs[i] = r0 + i < n ? scale[r0 + i] : 0.0;
     g.set_live_cols(kc - c0);
     g.mainloop_live(lds, F + r0 * ldf, ldf, n - r0, Y + (int64_t)c0 * ldy, ldy, kc - c0, r);
+    __syncthreads();  // scs (the main loop's barriers publish it too, for r > 0)
+    // U rows of this block through a buffer resource: a row past n falls
+    // outside its range and is dropped, offsets are 32-bit (BM kout values)
+    int64_t vrows = n - r0 < BM ? n - r0 : BM;
+    __amdgpu_buffer_rsrc_t rsU =
+        __builtin_amdgcn_make_buffer_rsrc(U + r0 * kout, (short)0, (int)(vrows * kout * 4), 0x00020000);
+    double sc[G::NTM][G::M::ACC];
+    int rl[G::NTM][G::M::ACC];
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int v = 0; v < G::M::ACC; ++v) {
+            rl[a][v] = g.row_of(a, v);
+            sc[a][v] = scs[rl[a][v]];
+        }
 #pragma unroll
     for (int a = 0; a < G::NTM; ++a)
 #pragma unroll
         for (int b = 0; b < G::NTN; ++b)
 #pragma unroll
             for (int v = 0; v < G::M::ACC; ++v) {
-                int64_t row = r0 + g.row_of(a, v);
                 int col = c0 + g.col_of(b);
-                if (row >= n || col >= kc) continue;
-                double x = (double)g.acc[a][b][v] * scale[row];
-                if (col < kout) U[row * kout + col] = (float)x;
-                else if (P1) P1[row] = x;
+                double x = (double)g.acc[a][b][v] * sc[a][v];
+                if (col < kout)
+                    buffer_store((float)x, rsU, (uint32_t)((rl[a][v] * kout + col) * 4));
+                else if (P1 && col < kc && r0 + rl[a][v] < n)
+                    P1[r0 + rl[a][v]] = x;
             }
 }
 
