@@ -340,45 +340,48 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
         rndv[tid] = kevd_u64_to_unit(kevd_draw_u64(seed, KEVD_STREAM_REJECT, (uint32_t)round, (uint32_t)tid));
     __syncthreads();
     int k = 0;
-    for (int i = 0; i < np; ++i) {
-        const double hii = dgw[wv][i];
-        const bool acc = inject ? (hii > tol) : ((u0[i] * rndv[i] < hii) && (hii > tol));
-        if (!acc) continue;
-        const double f = 1.0 / sqrt(hii);
-        if (tx == (i & 15)) {
-            const int bi = i >> 4;
+    // Proposal i lives in register column bi = i / 16 of its owners; the
+    // outer loop over bi is unrolled, so the owners read h[a][bi] from a
+    // fixed register (a runtime column index costs a select chain over all
+    // eight).  Column k of L is written whole: its rows below i are never
+    // read (the update, the diagonal copies and the inverse below read rows
+    // past the pivot only).
+#pragma unroll
+    for (int bi = 0; bi < 8; ++bi) {
+        const int i_end = np < 16 * (bi + 1) ? np : 16 * (bi + 1);
+        for (int i = 16 * bi; i < i_end; ++i) {
+            const double hii = dgw[wv][i];
+            const bool acc = inject ? (hii > tol) : ((u0[i] * rndv[i] < hii) && (hii > tol));
+            if (!acc) continue;
+            const double f = 1.0 / sqrt(hii);
+            if (tx == (i & 15)) {
+#pragma unroll
+                for (int a = 0; a < 8; ++a) Lst[k][ty + 16 * a] = h[a][bi] * f;
+            }
+            if (tid == 0) accpos[k] = i;
+            __syncthreads();
+            double lr[8], lc[8];
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
-                double v = 0.0;
-#pragma unroll
-                for (int b = 0; b < 8; ++b) v = b == bi ? h[a][b] : v;
-                int row = ty + 16 * a;
-                if (row >= i) Lst[k][row] = v * f;
+                int row = ty + 16 * a, col = tx + 16 * a;
+                lr[a] = (row > i && row < np) ? Lst[k][row] : 0.0;
+                lc[a] = (col > i && col < np) ? Lst[k][col] : 0.0;
             }
-        }
-        if (tid == 0) accpos[k] = i;
-        __syncthreads();
-        double lr[8], lc[8];
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            int row = ty + 16 * a, col = tx + 16 * a;
-            lr[a] = (row > i && row < np) ? Lst[k][row] : 0.0;
-            lc[a] = (col > i && col < np) ? Lst[k][col] : 0.0;
-        }
+            for (int a = 0; a < 8; ++a)
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
+                for (int b = 0; b < 8; ++b) h[a][b] -= lr[a] * lc[b];
+            // the wave's diagonal copy, as the owners' h(j, j) -= L(j) L(j)
 #pragma unroll
-            for (int b = 0; b < 8; ++b) h[a][b] -= lr[a] * lc[b];
-        // the wave's diagonal copy, as the owners' h(j, j) -= L(j) L(j)
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int j = lane + 64 * q;
-            if (j > i && j < np) {
-                const double l = Lst[k][j];
-                dgw[wv][j] -= l * l;
+            for (int q = 0; q < 2; ++q) {
+                const int j = lane + 64 * q;
+                if (j > i && j < np) {
+                    const double l = Lst[k][j];
+                    dgw[wv][j] -= l * l;
+                }
             }
+            ++k;
         }
-        ++k;
     }
     int64_t room = st->rank - st->Nr;
     if ((int64_t)k > room) k = (int)room;
@@ -392,33 +395,34 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
     for (int a = 0; a < 8; ++a)
 #pragma unroll
         for (int b = 0; b < 8; ++b) x[a][b] = (ty + 16 * a == tx + 16 * b && ty + 16 * a < k) ? 1.0 : 0.0;
-    for (int c = 0; c < k; ++c) {
-        const int buf = c & 1;
-        if (ty == (c & 15)) {
-            const int ac = c >> 4;
-            const double inv = 1.0 / Lst[c][accpos[c]];
+    // row c lives in register row ac = c / 16 of its owners: unrolled as in
+    // the acceptance loop
+#pragma unroll
+    for (int ac = 0; ac < 8; ++ac) {
+        const int c_end = k < 16 * (ac + 1) ? k : 16 * (ac + 1);
+        for (int c = 16 * ac; c < c_end; ++c) {
+            const int buf = c & 1;
+            if (ty == (c & 15)) {
+                const double inv = 1.0 / Lst[c][accpos[c]];
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    x[ac][b] *= inv;
+                    xrow[buf][tx + 16 * b] = x[ac][b];
+                }
+            }
+            __syncthreads();
+            double l[8], xr[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                int row = ty + 16 * a;
+                l[a] = (row > c && row < k) ? Lst[c][apos[a]] : 0.0;
+                xr[a] = xrow[buf][tx + 16 * a];
+            }
 #pragma unroll
             for (int a = 0; a < 8; ++a)
-                if (a == ac) {
 #pragma unroll
-                    for (int b = 0; b < 8; ++b) {
-                        x[a][b] *= inv;
-                        xrow[buf][tx + 16 * b] = x[a][b];
-                    }
-                }
+                for (int b = 0; b < 8; ++b) x[a][b] -= l[a] * xr[b];
         }
-        __syncthreads();
-        double l[8], xr[8];
-#pragma unroll
-        for (int a = 0; a < 8; ++a) {
-            int row = ty + 16 * a;
-            l[a] = (row > c && row < k) ? Lst[c][apos[a]] : 0.0;
-            xr[a] = xrow[buf][tx + 16 * a];
-        }
-#pragma unroll
-        for (int a = 0; a < 8; ++a)
-#pragma unroll
-            for (int b = 0; b < 8; ++b) x[a][b] -= l[a] * xr[b];
     }
 #pragma unroll
     for (int a = 0; a < 8; ++a)
