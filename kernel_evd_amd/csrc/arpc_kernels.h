@@ -318,7 +318,7 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
     if (st->done) return;
     const int round = st->round;
     __shared__ double Lst[BMAX][BMAX];
-    __shared__ double dgw[4][BMAX], u0[BMAX], rndv[BMAX], xrow[2][BMAX];
+    __shared__ double dgw[4][BMAX], u0[BMAX], rndv[BMAX], xrow[2][BMAX], dinv[BMAX];
     __shared__ int accpos[BMAX];
     const int np = st->nprop;
     const int tid = threadIdx.x;
@@ -360,12 +360,14 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
             }
             if (tid == 0) accpos[k] = i;
             __syncthreads();
+            // every row of column k was written: plain reads, then selects
             double lr[8], lc[8];
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 int row = ty + 16 * a, col = tx + 16 * a;
-                lr[a] = (row > i && row < np) ? Lst[k][row] : 0.0;
-                lc[a] = (col > i && col < np) ? Lst[k][col] : 0.0;
+                const double vr = Lst[k][row], vc = Lst[k][col];
+                lr[a] = (row > i && row < np) ? vr : 0.0;
+                lc[a] = (col > i && col < np) ? vc : 0.0;
             }
 #pragma unroll
             for (int a = 0; a < 8; ++a)
@@ -390,6 +392,9 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
     int apos[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) apos[a] = ty + 16 * a < k ? accpos[ty + 16 * a] : 0;
+    // the diagonal's reciprocals, all at once (one division per thread)
+    if (tid < k) dinv[tid] = 1.0 / Lst[tid][accpos[tid]];
+    __syncthreads();
     double x[8][8];
 #pragma unroll
     for (int a = 0; a < 8; ++a)
@@ -403,7 +408,7 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
         for (int c = 16 * ac; c < c_end; ++c) {
             const int buf = c & 1;
             if (ty == (c & 15)) {
-                const double inv = 1.0 / Lst[c][accpos[c]];
+                const double inv = dinv[c];
 #pragma unroll
                 for (int b = 0; b < 8; ++b) {
                     x[ac][b] *= inv;
@@ -415,7 +420,8 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 int row = ty + 16 * a;
-                l[a] = (row > c && row < k) ? Lst[c][apos[a]] : 0.0;
+                const double v = Lst[c][apos[a]];
+                l[a] = (row > c && row < k) ? v : 0.0;
                 xr[a] = xrow[buf][tx + 16 * a];
             }
 #pragma unroll
