@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # Measured HBM bytes per launch at this workload (rocprofv3 FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, the factors a cold 2 GiB copy gives), from
 # profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md.
-TRAFFIC_BYTES = {"k_downdate<float>": 4.291e9, "k_gram<float,1>": 111.20e9,
+TRAFFIC_BYTES = {"k_downdate<float>": 4.276e9, "k_gram<float,1>": 110.89e9,
                  "k_sinkhorn_rows<float,8>": 8.507e9}
 TRAFFIC_SOURCE = "profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md"
 FP32_MFMA_PEAK_TF = 157.3
