@@ -109,22 +109,35 @@ __global__ __launch_bounds__(256) void k_ens_gather(const EnsProb<T>* P)
     gather_pivots_body<T>(q.X, q.nrm, q.dp, q.F, q.ldf, q.Si, q.st, q.XS, q.nrmS, q.FS, blockIdx.x);
 }
 
-// One workgroup per (problem, row block) item; the hardware hands the
-// workgroups to free CU slots in item order, so long and short items
-// balance as they would through a work queue.  Items of finished problems
-// (or of a round that accepted nothing) return at once.
+__global__ void k_ens_zero(int* ctr)
+{
+    if (threadIdx.x == 0) *ctr = 0;
+}
+
+// Persistent downdate: each workgroup takes the next (problem, row block)
+// item until the list is exhausted.  Items of finished problems (or of a
+// round that accepted nothing) return at once.  The loop's two barriers keep
+// the item slot consistent whether or not the body itself synchronised.
 template <typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ens_downdate(
-    const EnsProb<T>* P, const int2* items)
+    const EnsProb<T>* P, const int2* items, int nitems, int* ctr)
 {
     typedef DowndateLds<T> DL;
     __shared__ __attribute__((aligned(16))) T lds[DL::ELEMS];
     __shared__ double rs[DL::WN][DL::BM];
     __shared__ DowndateEp<T> ep;
-    int2 w = items[blockIdx.x];
-    const EnsProb<T>& q = P[w.x];
-    downdate_body<T>(q.X, q.dp, q.nrm, q.F, q.ldf, q.d, q.n, q.XS, q.nrmS, q.Si, q.FS, q.LinvT, q.st, q.gs, w.y,
-                     lds, rs, &ep);
+    __shared__ int item_s;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(ctr, 1);
+        __syncthreads();
+        int it = item_s;
+        __syncthreads();
+        if (it >= nitems) break;
+        int2 w = items[it];
+        const EnsProb<T>& q = P[w.x];
+        downdate_body<T>(q.X, q.dp, q.nrm, q.F, q.ldf, q.d, q.n, q.XS, q.nrmS, q.Si, q.FS, q.LinvT, q.st, q.gs,
+                         w.y, lds, rs, &ep);
+    }
 }
 
 template <typename T>
@@ -215,10 +228,11 @@ kevd_status ens_rounds(kevd_ctx* m, kevd_ctx** ctxs, int nprob, const kevd_arpc_
         return kevd_fail(m, KEVD_EINVAL, "ensemble: too many work items");
     EnsProb<T>* dP;
     int2 *dchunks, *ditems;
-    int* flag;
+    int *ctr, *flag;
     KEVD_TRY(ctx_buf(m, "ens_desc", hp.size() * sizeof(EnsProb<T>), (void**)&dP));
     KEVD_TRY(ctx_buf_t(m, "ens_chunks", chunks.size(), &dchunks));
     KEVD_TRY(ctx_buf_t(m, "ens_items", items.size(), &ditems));
+    KEVD_TRY(ctx_buf_t(m, "ens_ctr", 1, &ctr));
     KEVD_TRY(ctx_buf_t(m, "ens_flag", 1, &flag));
     KEVD_HIP(m, hipMemcpyAsync(dP, hp.data(), hp.size() * sizeof(EnsProb<T>), hipMemcpyHostToDevice, s));
     KEVD_HIP(m, hipMemcpyAsync(dchunks, chunks.data(), chunks.size() * sizeof(int2), hipMemcpyHostToDevice, s));
@@ -227,6 +241,7 @@ kevd_status ens_rounds(kevd_ctx* m, kevd_ctx** ctxs, int nprob, const kevd_arpc_
 
     const unsigned nch = (unsigned)chunks.size();
     const int nit = (int)items.size();
+    const unsigned gdown = (unsigned)std::min<int64_t>(nit, 2 * (int64_t)m->num_cu);
     auto round = [&]() {
         k_ens_reduce_d<T><<<nch, 256, 0, s>>>(dP, dchunks, 0);
         k_ens_weights_scan<T><<<nch, 256, 0, s>>>(dP, dchunks);
@@ -234,7 +249,8 @@ kevd_status ens_rounds(kevd_ctx* m, kevd_ctx** ctxs, int nprob, const kevd_arpc_
         k_ens_form_H<T><<<dim3(BMAX / 16, BMAX / 16, nprob), 256, 0, s>>>(dP);
         k_ens_rejchol<T><<<nprob, 256, 0, s>>>(dP);
         k_ens_gather<T><<<dim3(BMAX, nprob), 256, 0, s>>>(dP);
-        k_ens_downdate<T><<<(unsigned)nit, 256, 0, s>>>(dP, ditems);
+        k_ens_zero<<<1, 64, 0, s>>>(ctr);
+        k_ens_downdate<T><<<gdown, 256, 0, s>>>(dP, ditems, nit, ctr);
         k_ens_commit<T><<<nprob, BMAX, 0, s>>>(dP);
         k_ens_all_done<T><<<1, 256, 0, s>>>(dP, nprob, flag);
     };
