@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_draw(const double* __restrict__ psum, c
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_form_H(const double* __restrict__ X64, int32_t dp, const T* __restrict__ F,
+__global__ __launch_bounds__(FORMH_THREADS) void k_form_H(const double* __restrict__ X64, int32_t dp, const T* __restrict__ F,
                                                 int64_t ldf, const int64_t* __restrict__ Sprime,
                                                 const ArpcState* st, double neg_inv_eps, double* H)
 {
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void k_lm_kprop(const double* __restrict__ X64
                   (int64_t)blockIdx.x * 256 + threadIdx.x);
 }
 
-__global__ __launch_bounds__(256) void k_lm_form_H(const double* __restrict__ X64, int32_t dp,
+__global__ __launch_bounds__(FORMH_THREADS) void k_lm_form_H(const double* __restrict__ X64, int32_t dp,
                                                    const double* __restrict__ P2, int64_t ldp,
                                                    const int64_t* __restrict__ Sprime, const ArpcState* st,
                                                    double neg_inv_eps, double* H)
@@ -224,7 +224,7 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
         }
         {
             ProfScope ps(ctx, TC_REJECT);
-            k_form_H<T><<<dim3(BMAX / 16, BMAX / 16), 256, 0, s>>>(ctx->X64, ctx->dp, F, ctx->ldf,
+            k_form_H<T><<<dim3(BMAX / 16, BMAX / 16), FORMH_THREADS, 0, s>>>(ctx->X64, ctx->dp, F, ctx->ldf,
                                                                   Sprime, st, gs.neg_inv_eps, H);
             k_rejchol<T><<<1, 256, 0, s>>>(H, st, p->seed, p->tol, inject_dev != nullptr, Sprime, Si, LinvT,
                                            Linv64);
@@ -346,7 +346,7 @@ kevd_status run_rounds_lm(kevd_ctx* ctx, const kevd_arpc_params* p, int max_roun
                                       rocblas_diagonal_non_unit, (rocblas_int)Nr, BMAX, &one, Linv,
                                       (rocblas_int)ldl, P, (rocblas_int)ldp, P2, (rocblas_int)ldp));
             }
-            k_lm_form_H<<<dim3(BMAX / 16, BMAX / 16), 256, 0, s>>>(ctx->X64, dp, P2, ldp, B.Sprime, B.st,
+            k_lm_form_H<<<dim3(BMAX / 16, BMAX / 16), FORMH_THREADS, 0, s>>>(ctx->X64, dp, P2, ldp, B.Sprime, B.st,
                                                                  gs.neg_inv_eps, B.H);
             k_lm_rejchol<T><<<1, 256, 0, s>>>(B.H, B.st, p->seed, p->tol, B.inj != nullptr, B.Sprime, B.Si,
                                               (T*)B.LinvT, B.Linv64, Spos);

@@ -209,31 +209,37 @@ __device__ inline void draw_body(const double* __restrict__ psum, const uint64_t
 // ------------------------------------------------------------ H = residual
 
 // H(p,q) = K(S'_p, S'_q) - sum_{m<Nr} F(S'_p, m) F(S'_q, m) in FP64, one
-// 16 x 16 tile per block.  K from the FP64 centred points, direct
-// differences.  Rows/columns past the proposal count are zero.  With
+// 16 x 16 tile per block of FORMH_THREADS.  K from the FP64 centred points,
+// direct differences.  Rows/columns past the proposal count are zero.  With
 // frow_by_position the rows of F are the proposals' own rows (row p).
-// Each wave takes a quarter of m (64-column chunks, staged in its own LDS
-// slice, so no block barrier per chunk) and each lane a 2 x 2 block of the
-// tile; the four partial sums are added in wave order at the end.
+// Each of the 8 waves takes an eighth of m (64-column chunks, staged in its
+// own LDS slice, so no block barrier per chunk; the next chunk's loads are
+// in flight while the current one is multiplied) and each lane a 2 x 2
+// block of the tile; the partial sums are added in wave order at the end.
+constexpr int FORMH_THREADS = 512;
 template <typename T>
 __device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, const T* __restrict__ F,
                                    int64_t ldf, const int64_t* __restrict__ Sprime, const ArpcState* st,
                                    double neg_inv_eps, double* H, int bx, int by, bool frow_by_position = false)
 {
+    constexpr int NW = FORMH_THREADS / 64;
     if (st->done) return;
     const int np = st->nprop;
     const int64_t Nr = st->Nr;
-    const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+    const int tid = threadIdx.x;
+    // threads < 256 own one entry (ty, tx) of the tile each
+    const bool owner = tid < 256;
+    const int ty = (tid >> 4) & 15, tx = tid & 15;
     const int p = by * 16 + ty, q = bx * 16 + tx;
     if (by * 16 >= np || bx * 16 >= np) {
-        if (p < BMAX && q < BMAX) H[p * BMAX + q] = 0.0;
+        if (owner && p < BMAX && q < BMAX) H[p * BMAX + q] = 0.0;
         return;
     }
-    __shared__ double Fp[4][16][65], Fq[4][16][65];
-    __shared__ double part[4][16][17];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ double Fp[NW][16][65], Fq[NW][16][65];
+    __shared__ double part[NW][16][17];
+    const int wv = tid >> 6, lane = tid & 63;
     // this wave's m range, in whole 64-column chunks
-    const int64_t nch = (Nr + 63) / 64, per = (nch + 3) / 4;
+    const int64_t nch = (Nr + 63) / 64, per = (nch + NW - 1) / NW;
     const int64_t c_beg = wv * per, c_end = c_beg + per < nch ? c_beg + per : nch;
     // lane -> rows 2 rp, 2 rp + 1 of the tile, columns 2 cp, 2 cp + 1
     const int rp = lane >> 3, cp = lane & 7;
@@ -246,14 +252,24 @@ __device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, c
         rowp[r] = pp < np ? F + (frow_by_position ? (int64_t)pp : Sprime[pp]) * ldf : nullptr;
         rowq[r] = qq < np ? F + (frow_by_position ? (int64_t)qq : Sprime[qq]) * ldf : nullptr;
     }
-    double acc00 = 0.0, acc01 = 0.0, acc10 = 0.0, acc11 = 0.0;
-    for (int64_t c = c_beg; c < c_end; ++c) {
+    T vp[16], vq[16];
+    auto gload = [&](int64_t c) {
         const int64_t m = c * 64 + lane;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            Fp[wv][r][lane] = (m < Nr && rowp[r]) ? (double)rowp[r][m] : 0.0;
-            Fq[wv][r][lane] = (m < Nr && rowq[r]) ? (double)rowq[r][m] : 0.0;
+            vp[r] = (m < Nr && rowp[r]) ? rowp[r][m] : (T)0;
+            vq[r] = (m < Nr && rowq[r]) ? rowq[r][m] : (T)0;
         }
+    };
+    if (c_beg < c_end) gload(c_beg);
+    double acc00 = 0.0, acc01 = 0.0, acc10 = 0.0, acc11 = 0.0;
+    for (int64_t c = c_beg; c < c_end; ++c) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            Fp[wv][r][lane] = (double)vp[r];
+            Fq[wv][r][lane] = (double)vq[r];
+        }
+        if (c + 1 < c_end) gload(c + 1);
         // the slice is this wave's own: its writes are visible to its reads
         // in order, with no block barrier
         __builtin_amdgcn_wave_barrier();
@@ -273,7 +289,10 @@ __device__ inline void form_H_body(const double* __restrict__ X64, int32_t dp, c
     part[wv][2 * rp + 1][2 * cp] = acc10;
     part[wv][2 * rp + 1][2 * cp + 1] = acc11;
     __syncthreads();
-    const double acc = ((part[0][ty][tx] + part[1][ty][tx]) + part[2][ty][tx]) + part[3][ty][tx];
+    if (!owner) return;
+    double acc = part[0][ty][tx];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) acc += part[w][ty][tx];
     const int64_t sp = Sprime[p < np ? p : 0], sq = Sprime[q < np ? q : 0];
     double kv;
     if (sp == sq) {

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_ens_draw(const EnsProb<T>* P)
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_ens_form_H(const EnsProb<T>* P)
+__global__ __launch_bounds__(FORMH_THREADS) void k_ens_form_H(const EnsProb<T>* P)
 {
     const EnsProb<T>& q = P[blockIdx.z];
     form_H_body<T>(q.X64, q.dp, q.F, q.ldf, q.Sprime, q.st, q.gs.neg_inv_eps, q.H, blockIdx.x, blockIdx.y);
@@ -246,7 +246,7 @@ kevd_status ens_rounds(kevd_ctx* m, kevd_ctx** ctxs, int nprob, const kevd_arpc_
         k_ens_reduce_d<T><<<nch, 256, 0, s>>>(dP, dchunks, 0);
         k_ens_weights_scan<T><<<nch, 256, 0, s>>>(dP, dchunks);
         k_ens_draw<T><<<nprob, 256, 0, s>>>(dP);
-        k_ens_form_H<T><<<dim3(BMAX / 16, BMAX / 16, nprob), 256, 0, s>>>(dP);
+        k_ens_form_H<T><<<dim3(BMAX / 16, BMAX / 16, nprob), FORMH_THREADS, 0, s>>>(dP);
         k_ens_rejchol<T><<<nprob, 256, 0, s>>>(dP);
         k_ens_gather<T><<<dim3(BMAX, nprob), 256, 0, s>>>(dP);
         k_ens_zero<<<1, 64, 0, s>>>(ctr);
