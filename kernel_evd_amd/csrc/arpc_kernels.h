@@ -149,13 +149,21 @@ __device__ inline void draw_body(const double* __restrict__ psum, const uint64_t
     if (st->done) return;
     const int round = st->round;
     __shared__ uint64_t bpre[MAX_SCAN_BLOCKS];
+    __shared__ double spsum[MAX_SCAN_BLOCKS];
     __shared__ int nprop_s;
+    // the block partials are loaded by every thread, then summed in block
+    // order by one (the same order, so the sums are unchanged)
+    for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+        bpre[j] = btot[j];
+        spsum[j] = psum[j];
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0.0;
         uint64_t run = 0;
         for (int j = 0; j < nb; ++j) {
-            s += psum[j];
-            run += btot[j];
+            s += spsum[j];
+            run += bpre[j];
             bpre[j] = run;
         }
         if (round > 0) err[round - 1] = s / trK;
