@@ -63,7 +63,9 @@ def test_median(gpu_ctx):
     assert abs(a - b) <= 1e-12 * b
 
 
-@pytest.mark.parametrize("block,rank", [(1, 40), (16, 100), (128, 300)])
+# block sizes 40 and 100 leave a partial 16-proposal group in the rejection
+# block's register layout (16 + 16 + 8 and 6 x 16 + 4 proposals)
+@pytest.mark.parametrize("block,rank", [(1, 40), (16, 100), (40, 150), (100, 250), (128, 300)])
 def test_arpc_fp64_matches_oracle(gpu_ctx, block, rank):
     X = clustered_points(2000, 12, seed=block)
     _setup(gpu_ctx, X)
