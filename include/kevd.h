@@ -245,6 +245,9 @@ typedef struct {
     int32_t eig_iters;        /* subspace iterations of the leading-eigenpair
                                  solve (0: full divide-and-conquer)         */
     int32_t eig_householder;  /* 1 if that solve fell back to Householder QR */
+    int32_t eig_full_fallback; /* 1 if subspace iteration had not converged
+                                  after its step limit and the full
+                                  divide-and-conquer solve gave the pairs  */
 } kevd_timings;
 
 /* Enable per-kernel HIP event timing on the context's stream (adds one
@@ -277,7 +280,8 @@ kevd_status kevd_ensemble_evd(kevd_ctx** ctxs, int32_t nprob, const kevd_evd_par
  * highest factor-columns-per-second of kevd_arpc on this context's points
  * (base supplies eps, rank, seed, tol, precision; two timed runs per
  * candidate, the faster one counts).  Without cols_per_s the choice is
- * cached per (device, n, padded dim, rank, precision, tol) for the process
+ * cached per (device, n, padded dim, rank, precision, tol, eps, candidate
+ * set) for the process
  * and a cached answer is returned without measuring; with it, every
  * candidate is measured and its rate written.  The run leaves the context
  * holding the last candidate's factor. */

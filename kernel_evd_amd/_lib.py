@@ -49,7 +49,8 @@ class Timings(C.Structure):
                 ("downdate_launches", i64), ("downdate_flop", dbl), ("gram_ms", dbl),
                 ("gram_launches", i64), ("gram_flop", dbl), ("eig_ms", dbl),
                 ("sample_ms", dbl), ("rejection_ms", dbl), ("matvec_ms", dbl),
-                ("dense_ms", dbl), ("u_ms", dbl), ("eig_iters", i32), ("eig_householder", i32)]
+                ("dense_ms", dbl), ("u_ms", dbl), ("eig_iters", i32), ("eig_householder", i32),
+                ("eig_full_fallback", i32)]
 
 
 class Peaks(C.Structure):

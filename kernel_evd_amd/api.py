@@ -51,6 +51,7 @@ class Context:
         self.n = 0
         self.dim = 0
         self._k_out = 0
+        self._rank = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -85,6 +86,7 @@ class Context:
             X = np.ascontiguousarray(X, dtype=np.float64)
             self._check(self._lib.kevd_set_points(self._h, _p(X, C.c_double), X.shape[0], X.shape[1]))
         self.n, self.dim = X.shape
+        self._rank = 0
 
     def median_sqdist(self, seed=0, nsample=0):
         out = C.c_double()
@@ -129,10 +131,16 @@ class Context:
         r.trace_err = _p(err, C.c_double)
         r.rounds_cap = mr
         self._check(self._lib.kevd_arpc(self._h, C.byref(p), C.byref(r)))
+        self._rank = 0 if regenerate else r.rank
         nr = r.rounds
         return ArpcOutput(r.rank, nr, piv[:r.rank].copy(), acc[:nr].copy(), err[:nr].copy())
 
-    def get_factor(self, rank, as_prec="fp64"):
+    def get_factor(self, rank=None, as_prec="fp64"):
+        """The factor held by the context (n x rank); rank defaults to the
+        realised rank and may not exceed it."""
+        rank = self._rank if rank is None else rank
+        if rank != self._rank:
+            raise KevdError(1, "get_factor: the context holds a factor of rank %d, not %d" % (self._rank, rank))
         dt = np.float32 if PREC[as_prec] == KEVD_MIXED else np.float64
         F = np.empty((self.n, rank), dtype=dt)
         self._check(self._lib.kevd_get_factor(self._h, F.ctypes.data_as(C.c_void_p), rank, PREC[as_prec]))
@@ -144,6 +152,7 @@ class Context:
         F = np.ascontiguousarray(F, dtype=np.float32 if prec == KEVD_MIXED else np.float64)
         self._check(self._lib.kevd_set_factor(self._h, F.ctypes.data_as(C.c_void_p), F.shape[0],
                                               F.shape[1], F.shape[1], prec))
+        self._rank = F.shape[1]
 
     def get_linv(self, rank):
         out = np.empty(rank * rank, dtype=np.float64)
@@ -205,6 +214,7 @@ class Context:
         rates = np.zeros(cand.size) if measure else None
         self._check(self._lib.kevd_autotune_block(self._h, C.byref(p), _p(cand, C.c_int32), cand.size,
                                                   C.byref(best), _p(rates, C.c_double) if measure else None))
+        self._rank = rank  # the context holds the last timed run's factor
         return (best.value, rates) if measure else best.value
 
     def measure_peaks(self, copy_bytes=4 << 30):
@@ -258,6 +268,7 @@ def ensemble_arpc(contexts, eps, ranks, blocks, seeds=None, tol=1e-10, prec="mix
         raise KevdError(st, lib.kevd_last_error(contexts[0]._h).decode())
     outs = []
     for i in range(P):
+        contexts[i]._rank = rs[i].rank
         piv, acc, err = keep[i]
         nr = rs[i].rounds
         outs.append(ArpcOutput(rs[i].rank, nr, piv[:rs[i].rank].copy(), acc[:nr].copy(), err[:nr].copy()))

@@ -47,12 +47,13 @@ __global__ __launch_bounds__(256) void k_weights_scan(const double* __restrict__
     weights_scan_body(d, n, pmax, nb, shift_base, prefix, btot, st, blockIdx.x);
 }
 
-__global__ __launch_bounds__(256) void k_draw(const double* __restrict__ psum, const uint64_t* __restrict__ btot,
-                                              int nb, const uint64_t* __restrict__ prefix, int64_t n, double trK,
-                                              uint64_t seed, int b, int64_t* Sprime, ArpcState* st, double* err,
-                                              const int64_t* inject, int64_t n_inject)
+__global__ __launch_bounds__(256) void k_draw(const double* __restrict__ psum, const double* __restrict__ pmax,
+                                              const uint64_t* __restrict__ btot, int nb,
+                                              const uint64_t* __restrict__ prefix, int64_t n, double trK,
+                                              uint64_t seed, int b, double tol, int64_t* Sprime, ArpcState* st,
+                                              double* err, const int64_t* inject, int64_t n_inject)
 {
-    draw_body(psum, btot, nb, prefix, n, trK, seed, b, Sprime, st, err, inject, n_inject);
+    draw_body(psum, pmax, btot, nb, prefix, n, trK, seed, b, tol, Sprime, st, err, inject, n_inject);
 }
 
 template <typename T>
@@ -219,8 +220,8 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
             ProfScope ps(ctx, TC_SAMPLE);
             k_reduce_d<<<nb, 256, 0, s>>>(d, n, pmax, psum, st, 0);
             k_weights_scan<<<nb, 256, 0, s>>>(d, n, pmax, nb, shift_base, prefix, btot, st);
-            k_draw<<<1, 256, 0, s>>>(psum, btot, nb, prefix, n, trK, p->seed, p->block, Sprime, st, err_dev,
-                                     inject_dev, n_inject);
+            k_draw<<<1, 256, 0, s>>>(psum, pmax, btot, nb, prefix, n, trK, p->seed, p->block, p->tol, Sprime, st,
+                                     err_dev, inject_dev, n_inject);
         }
         {
             ProfScope ps(ctx, TC_REJECT);
@@ -333,8 +334,8 @@ kevd_status run_rounds_lm(kevd_ctx* ctx, const kevd_arpc_params* p, int max_roun
             ProfScope ps(ctx, TC_SAMPLE);
             k_reduce_d<<<nb, 256, 0, s>>>(B.d, n, B.pmax, B.psum, B.st, 0);
             k_weights_scan<<<nb, 256, 0, s>>>(B.d, n, B.pmax, nb, shift_base, B.prefix, B.btot, B.st);
-            k_draw<<<1, 256, 0, s>>>(B.psum, B.btot, nb, B.prefix, n, trK, p->seed, p->block, B.Sprime, B.st,
-                                     B.err, B.inj, n_inject);
+            k_draw<<<1, 256, 0, s>>>(B.psum, B.pmax, B.btot, nb, B.prefix, n, trK, p->seed, p->block, p->tol,
+                                     B.Sprime, B.st, B.err, B.inj, n_inject);
         }
         {
             ProfScope ps(ctx, TC_REJECT);

@@ -29,7 +29,8 @@ constexpr int BMAX = 128;              // largest block size b
 
 struct ArpcState {
     int32_t done;
-    int32_t stop_reason;  // 1 rank reached, 2 zero residual mass, 4 pivot list exhausted
+    int32_t stop_reason;  // 1 rank reached, 2 zero residual mass, 3 max residual <= tol,
+                          // 4 pivot list exhausted, 8 round budget used (ensemble)
     int32_t k;            // accepted this round, after truncation to the target rank
     int32_t nprop;        // proposals this round
     int64_t Nr;           // rank so far
@@ -138,27 +139,37 @@ __device__ inline void weights_scan_body(const double* __restrict__ d, int64_t n
     if (threadIdx.x == 0) btot[blk] = wtot[0] + wtot[1] + wtot[2] + wtot[3];
 }
 
-// Record the previous round's trace error, stop on zero mass, then draw the
-// b proposals by integer inverse CDF (or take them from the injected list).
-// The round index is the device state's, so a captured round replays as is.
-__device__ inline void draw_body(const double* __restrict__ psum, const uint64_t* __restrict__ btot, int nb,
+// Record the previous round's trace error, stop on zero mass or when no
+// residual entry exceeds tol (Algorithm 2.1 accepts only H_ii > tol, and
+// H_ii is at most the residual diagonal, so no later round could accept),
+// then draw the b proposals by integer inverse CDF (or take them from the
+// injected list).  The round index is the device state's, so a captured
+// round replays as is.
+__device__ inline void draw_body(const double* __restrict__ psum, const double* __restrict__ pmax,
+                                 const uint64_t* __restrict__ btot, int nb,
                                  const uint64_t* __restrict__ prefix, int64_t n, double trK, uint64_t seed,
-                                 int b, int64_t* Sprime, ArpcState* st, double* err,
+                                 int b, double tol, int64_t* Sprime, ArpcState* st, double* err,
                                  const int64_t* inject, int64_t n_inject)
 {
     if (st->done) return;
     const int round = st->round;
     __shared__ uint64_t bpre[MAX_SCAN_BLOCKS];
     __shared__ double spsum[MAX_SCAN_BLOCKS];
+    __shared__ double wmax[4];
     __shared__ int nprop_s;
     // the block partials are loaded by every thread, then summed in block
     // order by one (the same order, so the sums are unchanged)
+    double mx = 0.0;
     for (int j = threadIdx.x; j < nb; j += blockDim.x) {
         bpre[j] = btot[j];
         spsum[j] = psum[j];
+        mx = fmax(mx, pmax[j]);
     }
+    mx = wave_max(mx);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
     __syncthreads();
     if (threadIdx.x == 0) {
+        const double dmax = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
         double s = 0.0;
         uint64_t run = 0;
         for (int j = 0; j < nb; ++j) {
@@ -176,6 +187,10 @@ __device__ inline void draw_body(const double* __restrict__ psum, const uint64_t
         if (run == 0) {
             st->done = 1;
             st->stop_reason = 2;
+            np = 0;
+        } else if (!(dmax > tol)) {
+            st->done = 1;
+            st->stop_reason = 3;
             np = 0;
         } else if (np <= 0) {
             st->done = 1;

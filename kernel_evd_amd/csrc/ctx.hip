@@ -70,7 +70,8 @@ static hipEvent_t pool_event(kevd_ctx* ctx)
 
 ProfScope::ProfScope(kevd_ctx* c, int category) : ctx(c), cat(category)
 {
-    if (!ctx->profiling) return;
+    // a negative category times nothing (a call nested in a timed scope)
+    if (!ctx->profiling || cat < 0) return;
     a = pool_event(ctx);
     b = pool_event(ctx);
     if (a) (void)hipEventRecord(a, ctx->stream);

@@ -85,7 +85,8 @@ __global__ __launch_bounds__(256) void k_ens_draw(const EnsProb<T>* P)
         q.st->stop_reason = 8;
     }
     __syncthreads();
-    draw_body(q.psum, q.btot, q.nb, q.prefix, q.n, q.trK, q.seed, q.block, q.Sprime, q.st, q.err, nullptr, 0);
+    draw_body(q.psum, q.pmax, q.btot, q.nb, q.prefix, q.n, q.trK, q.seed, q.block, q.tol, q.Sprime, q.st, q.err,
+              nullptr, 0);
 }
 
 template <typename T>
@@ -312,7 +313,9 @@ kevd_status ens_rounds(kevd_ctx* m, kevd_ctx** ctxs, int nprob, const kevd_arpc_
 
 // Autotuner cache: (device, n, dp, rank, precision, tol) -> block size.
 std::mutex tune_mu;
-std::map<std::tuple<int, int64_t, int32_t, int64_t, int, double>, int32_t> tune_cache;
+// key: device, n, padded dim, rank, precision, tol, eps, sorted candidate set
+typedef std::tuple<int, int64_t, int32_t, int64_t, int, double, double, std::vector<int32_t>> TuneKey;
+std::map<TuneKey, int32_t> tune_cache;
 
 }  // namespace
 
@@ -400,7 +403,10 @@ extern "C" kevd_status kevd_autotune_block(kevd_ctx* ctx, const kevd_arpc_params
 {
     if (!ctx || !base || !candidates || ncand < 1 || !best_block)
         return kevd_fail(ctx, KEVD_EINVAL, "autotune: null argument or no candidates");
-    auto key = std::make_tuple(ctx->device, ctx->n, ctx->dp, base->rank, (int)base->prec, base->tol);
+    std::vector<int32_t> cand_set(candidates, candidates + ncand);
+    std::sort(cand_set.begin(), cand_set.end());
+    cand_set.erase(std::unique(cand_set.begin(), cand_set.end()), cand_set.end());
+    TuneKey key(ctx->device, ctx->n, ctx->dp, base->rank, (int)base->prec, base->tol, base->eps, cand_set);
     if (!cols_per_s) {
         std::lock_guard<std::mutex> g(tune_mu);
         auto it = tune_cache.find(key);

@@ -1176,7 +1176,7 @@ kevd_status ugemm(kevd_ctx* ctx, const double* Y, int k, const double* colscale,
 
 // Symmetric eigensolve of the column-major r x r FP64 matrix W (lower
 // triangle used); V overwrites W, lam ascending.
-kevd_status eigh(kevd_ctx* ctx, double* W, int64_t r, double* lam)
+kevd_status eigh(kevd_ctx* ctx, double* W, int64_t r, double* lam, bool timed = true)
 {
     rocblas_handle h = blas_handle(ctx);
     double* E;
@@ -1184,7 +1184,7 @@ kevd_status eigh(kevd_ctx* ctx, double* W, int64_t r, double* lam)
     KEVD_TRY(ctx_buf_t(ctx, "e_E", r, &E));
     KEVD_TRY(ctx_buf_t(ctx, "e_info", 1, &info));
     {
-        ProfScope ps(ctx, TC_EIG);
+        ProfScope ps(ctx, timed ? TC_EIG : -1);
         BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)r, W,
                               (rocblas_int)r, lam, E, info));
     }
@@ -1751,7 +1751,17 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
         ctx->tm.eig_iters = it;
         ctx->tm.eig_householder = householder;
         if (broken) continue;  // a Cholesky QR pass lost definiteness: redo with Householder
-        if (conv) ctx->eig_converged_at = it;
+        if (!conv) {
+            // The step limit came first (a slowly decaying spectrum: the
+            // rate lambda_{l+1} / lambda_k close to 1).  The full solve gives
+            // every pair to working accuracy; W is unchanged by the
+            // iteration, which only read it.
+            ctx->tm.eig_full_fallback = 1;
+            KEVD_TRY(eigh(ctx, W, r, lam, false));
+            *Vk = W + (r - k) * r;
+            return KEVD_OK;
+        }
+        ctx->eig_converged_at = it;
         // Ritz vectors of the leading k: X = Q S(:, l-k:l)
         BLAS(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, R, (rocblas_int)k, Lb, &one, Q, R,
                            H + (int64_t)(l - k) * l, Lb, &zero, X, R));
@@ -2126,6 +2136,7 @@ extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_ev
     ctx->tm.u_ms = 0;
     ctx->tm.eig_iters = 0;
     ctx->tm.eig_householder = 0;
+    ctx->tm.eig_full_fallback = 0;
     prof_reset(ctx);
     size_t ubytes = (size_t)ctx->n * p->k_out * sizeof(float);
     if (ctx->U_bytes < ubytes) {

@@ -1,8 +1,8 @@
 // kevd-bandwidth -- the ks_bandwidth.cu stage: kernel sums over a grid of
 // bandwidths from one fused launch, then the reference's post-processing
 // (ks_bandwidth.cu:609-658) into a bw0.out-style text file:
-//   "%.5e %.5e %.5e" (bandwidth, log sum, slope) for i = 1..n-2,
-//   a blank line, eopt, mdim.
+//   "%.5e %.5e %.5e" (bandwidth, log of the mean row sum, slope) for
+//   i = 1..n-2, a blank line, eopt, mdim.
 //
 //   kevd-bandwidth --points X.npy|u.h5 [--delay Nq] [--grid-points M]
 //                  [--rows 16384 [--row-seed 0] | --all-pairs]
@@ -130,7 +130,13 @@ int main(int argc, char** argv)
         std::fprintf(stderr, "cannot write %s\n", out.c_str());
         return 1;
     }
-    for (int i = 1; i < ne - 1; ++i) std::fprintf(f, "%.5e %.5e %.5e\n", label[i], logS[i], slope[i - 1]);
+    // Column 2 is the log of the mean row sum, log(S / rows): with every row
+    // (--all-pairs) that is the reference's log(2 S_lower / NM + 1)
+    // (ks_bandwidth.cu:615-618, whose summands carry a 1/NM), the full sum
+    // being 2 S_lower + NM.  The constant does not change the slope.
+    const double log_rows = std::log((double)rows.size());
+    for (int i = 1; i < ne - 1; ++i)
+        std::fprintf(f, "%.5e %.5e %.5e\n", label[i], logS[i] - log_rows, slope[i - 1]);
     std::fprintf(f, "\n%.5e\n%.5e\n", eopt_label, mdim);
     std::fclose(f);
     if (a.has("sums-out")) npy_save(a.str("sums-out"), "<f8", {(int64_t)ne}, S.data());

@@ -93,6 +93,15 @@ static uint64_t mulhi64(uint64_t a, uint64_t b)
 /* K(i,j) = exp(-sum_k (x_ik - x_jk)^2 / eps), direct differences in FP64:
  * compute_kblock / kexp_t (ks_arpclm.cu:226-280) and compute_kernel_mat
  * (ks_reference.py:51-58) with eps = Nq * bw. */
+static double wtime(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
 static double kval(const double* X, int32_t dim, int64_t i, int64_t j, double eps)
 {
     if (i == j) return 1.0;
@@ -289,14 +298,17 @@ int32_t orc_rejchol(double* H, int32_t b, uint64_t seed, uint32_t round,
  *   err = sum(d) / n                                       (536-564)
  * The reference keeps L^-1 and regenerates K(R,S) (its "LM" form); with
  * F = K(:,S) L^-T both give the same G.  Rounds stop at the target rank
- * (the last block truncated), zero residual mass, or max_rounds.
- * F is n x rank row-major (caller-zeroed).  Returns 0, or 2 on allocation
- * failure. */
+ * (the last block truncated), zero residual mass, a maximum residual
+ * entry <= tol (nothing could be accepted any more), or max_rounds.
+ * F is n x rank row-major (caller-zeroed).  phase_s (optional, 3 values)
+ * receives the wall seconds of the sampling, of the pivot block (H and the
+ * rejection Cholesky) and of the downdate (the n-row update).  Returns 0,
+ * or 2 on allocation failure. */
 int orc_arpc(const double* X, int64_t n, int32_t dim, double eps, int64_t rank,
              int32_t b, uint64_t seed, double tol, int32_t max_rounds,
              const int64_t* inject, int64_t n_inject, double* F,
              int64_t* pivots, int32_t* accepted, double* err,
-             int32_t* rounds_out, int64_t* rank_out, double* d_out)
+             int32_t* rounds_out, int64_t* rank_out, double* d_out, double* phase_s)
 {
     double* d = (double*)malloc((size_t)n * sizeof(double));
     int64_t* Sp = (int64_t*)malloc((size_t)b * sizeof(int64_t));
@@ -311,7 +323,14 @@ int orc_arpc(const double* X, int64_t n, int32_t dim, double eps, int64_t rank,
 
     int64_t Nr = 0, inj_pos = 0;
     int32_t round = 0;
+    double t_sample = 0.0, t_block = 0.0, t_down = 0.0, t0;
     for (; round < max_rounds && Nr < rank; ++round) {
+        t0 = wtime();
+        /* no residual entry above tol: Algorithm 2.1 accepts only
+         * H_ii > tol and H_ii <= d_i, so no later round could accept */
+        double dmax = 0.0;
+        for (int64_t i = 0; i < n; ++i) dmax = d[i] > dmax ? d[i] : dmax;
+        if (!(dmax > tol)) break;
         int32_t bb = b;
         if (inject) {
             if (inj_pos >= n_inject) break;
@@ -322,6 +341,8 @@ int orc_arpc(const double* X, int64_t n, int32_t dim, double eps, int64_t rank,
             uint64_t W = 0;
             if (orc_propose(d, n, seed, (uint32_t)round, b, Sp, &W) != 0) break;
         }
+        t_sample += wtime() - t0;
+        t0 = wtime();
         /* H = K(S',S') - F(S',:Nr) F(S',:Nr)^T */
 #pragma omp parallel for schedule(static)
         for (int32_t p = 0; p < bb; ++p)
@@ -337,40 +358,75 @@ int orc_arpc(const double* X, int64_t n, int32_t dim, double eps, int64_t rank,
                                 accpos, Lc, Lcinv);
         if (k > rank - Nr) k = (int32_t)(rank - Nr);
         accepted[round] = k;
+        t_block += wtime() - t0;
+        t0 = wtime();
         if (k > 0) {
             int64_t Si[1024];
             for (int32_t a = 0; a < k; ++a) Si[a] = Sp[accpos[a]];
             for (int32_t a = 0; a < k; ++a)
                 memcpy(FS + (int64_t)a * rank, F + Si[a] * rank, (size_t)Nr * sizeof(double));
+            /* rows in blocks of RB: each pivot row of F(S_i,:) is loaded once
+             * per block and used for RB rows (register blocking) */
+            enum { RB = 4 };
 #pragma omp parallel
             {
-                double T[1024], G[1024];
+                double T[RB][1024], G[1024];
 #pragma omp for schedule(static)
-                for (int64_t i = 0; i < n; ++i) {
-                    const double* fi = F + i * rank;
-                    for (int32_t c = 0; c < k; ++c) {
-                        const double* fs = FS + (int64_t)c * rank;
-                        double t = 0.0;
+                for (int64_t i0 = 0; i0 < n; i0 += RB) {
+                    const int nrb = n - i0 < RB ? (int)(n - i0) : RB;
+                    if (nrb == RB) {
+                        const double* f0 = F + i0 * rank;
+                        const double* f1 = f0 + rank;
+                        const double* f2 = f1 + rank;
+                        const double* f3 = f2 + rank;
+                        for (int32_t c = 0; c < k; ++c) {
+                            const double* fs = FS + (int64_t)c * rank;
+                            double t0r = 0.0, t1r = 0.0, t2r = 0.0, t3r = 0.0;
+#pragma omp simd reduction(+ : t0r, t1r, t2r, t3r)
+                            for (int64_t m = 0; m < Nr; ++m) {
+                                const double v = fs[m];
+                                t0r += f0[m] * v;
+                                t1r += f1[m] * v;
+                                t2r += f2[m] * v;
+                                t3r += f3[m] * v;
+                            }
+                            T[0][c] = kval(X, dim, i0, Si[c], eps) - t0r;
+                            T[1][c] = kval(X, dim, i0 + 1, Si[c], eps) - t1r;
+                            T[2][c] = kval(X, dim, i0 + 2, Si[c], eps) - t2r;
+                            T[3][c] = kval(X, dim, i0 + 3, Si[c], eps) - t3r;
+                        }
+                    } else {
+                        for (int r = 0; r < nrb; ++r) {
+                            const double* fi = F + (i0 + r) * rank;
+                            for (int32_t c = 0; c < k; ++c) {
+                                const double* fs = FS + (int64_t)c * rank;
+                                double t = 0.0;
 #pragma omp simd reduction(+ : t)
-                        for (int64_t m = 0; m < Nr; ++m) t += fi[m] * fs[m];
-                        T[c] = kval(X, dim, i, Si[c], eps) - t;
+                                for (int64_t m = 0; m < Nr; ++m) t += fi[m] * fs[m];
+                                T[r][c] = kval(X, dim, i0 + r, Si[c], eps) - t;
+                            }
+                        }
                     }
-                    double sq = 0.0;
-                    for (int32_t a = 0; a < k; ++a) {
-                        double g = 0.0;
-                        for (int32_t c = 0; c <= a; ++c) g += Lcinv[a * bb + c] * T[c];
-                        G[a] = g;
-                        sq += g * g;
+                    for (int r = 0; r < nrb; ++r) {
+                        const int64_t i = i0 + r;
+                        double sq = 0.0;
+                        for (int32_t a = 0; a < k; ++a) {
+                            double g = 0.0;
+                            for (int32_t c = 0; c <= a; ++c) g += Lcinv[a * bb + c] * T[r][c];
+                            G[a] = g;
+                            sq += g * g;
+                        }
+                        double* fo = F + i * rank + Nr;
+                        for (int32_t a = 0; a < k; ++a) fo[a] = G[a];
+                        double dd = d[i] - sq;
+                        d[i] = dd >= 0.0 ? dd : 0.0;
                     }
-                    double* fo = F + i * rank + Nr;
-                    for (int32_t a = 0; a < k; ++a) fo[a] = G[a];
-                    double dd = d[i] - sq;
-                    d[i] = dd >= 0.0 ? dd : 0.0;
                 }
             }
             for (int32_t a = 0; a < k; ++a) pivots[Nr + a] = Si[a];
             Nr += k;
         }
+        t_down += wtime() - t0;
         double s = 0.0;
         for (int64_t i = 0; i < n; ++i) s += d[i];
         err[round] = s / (double)n;
@@ -378,6 +434,11 @@ int orc_arpc(const double* X, int64_t n, int32_t dim, double eps, int64_t rank,
     *rounds_out = round;
     *rank_out = Nr;
     if (d_out) memcpy(d_out, d, (size_t)n * sizeof(double));
+    if (phase_s) {
+        phase_s[0] = t_sample;
+        phase_s[1] = t_block;
+        phase_s[2] = t_down;
+    }
     free(d); free(Sp); free(H); free(Lc); free(Lcinv); free(accpos); free(FS);
     return 0;
 }

@@ -51,7 +51,7 @@ def so():
         L.orc_rejchol.argtypes = [P(dbl), i32, u64, u32, dbl, C.c_int, P(i32), P(dbl), P(dbl)]
         L.orc_arpc.restype = C.c_int
         L.orc_arpc.argtypes = [P(dbl), i64, i32, dbl, i64, i32, u64, dbl, i32, P(i64), i64,
-                               P(dbl), P(i64), P(i32), P(dbl), P(i32), P(i64), P(dbl)]
+                               P(dbl), P(i64), P(i32), P(dbl), P(i32), P(i64), P(dbl), P(dbl)]
         L.orc_bandwidth.argtypes = [P(dbl), i64, i32, P(i64), i64, P(dbl), i32, P(dbl)]
         L.orc_num_threads.restype = C.c_int
         _so = L
@@ -148,9 +148,10 @@ def arpc(X, eps, rank, block=128, seed=0, tol=1e-10, max_rounds=0, inject=None):
         inject = np.ascontiguousarray(inject, dtype=np.int64)
         injp = _p(inject, i64)
         ninj = inject.size
+    phase = np.zeros(3)
     rc = so().orc_arpc(_p(X, dbl), n, dim, eps, rank, block, seed, tol, mr, injp, ninj,
                        _p(F, dbl), _p(piv, i64), _p(acc, i32), _p(err, dbl), C.byref(rounds),
-                       C.byref(rk), _p(d, dbl))
+                       C.byref(rk), _p(d, dbl), _p(phase, dbl))
     if rc:
         raise RuntimeError("orc_arpc failed: %d" % rc)
     r = ArpcRef()
@@ -161,6 +162,8 @@ def arpc(X, eps, rank, block=128, seed=0, tol=1e-10, max_rounds=0, inject=None):
     r.accepted = acc[:r.rounds]
     r.trace_err = err[:r.rounds]
     r.d = d
+    # wall seconds: sampling, pivot block (H + rejection), n-row downdate
+    r.t_sample, r.t_block, r.t_downdate = (float(v) for v in phase)
     return r
 
 
@@ -188,16 +191,21 @@ def dense_kernel(X, eps):
     return np.exp(-cdist(X, X, "sqeuclidean") / eps)
 
 
-def dense_reference_evd(K):
+def dense_reference_evd(K, k=None):
     """ks_reference.py:21-40: d = K1/N, q = K D^-1 1 / N, Kt = D^-1 K Q^-1/2,
-    P = Kt Kt^T / N^2; eigenpairs descending."""
+    P = Kt Kt^T / N^2; eigenpairs descending.  With k, only the leading k
+    pairs (LAPACK's subset solver; the same eigenvalues, less time)."""
     NM = K.shape[0]
     one = np.ones(NM)
     dvec = (1.0 / NM) * (K @ one)
     qvec = (1.0 / NM) * (K @ (one / dvec))
     Kt = (K / dvec[:, None]) / np.sqrt(qvec)[None, :]
     P = (1.0 / NM ** 2) * (Kt @ Kt.T)
-    w, V = np.linalg.eigh(P)
+    if k is None:
+        w, V = np.linalg.eigh(P)
+    else:
+        from scipy.linalg import eigh
+        w, V = eigh(P, subset_by_index=[NM - k, NM - 1], driver="evr")
     return w[::-1].copy(), V[:, ::-1].copy(), P
 
 

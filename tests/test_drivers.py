@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
+from conftest import clustered_points
 from kernel_evd_amd import delay_embed, ks_generate
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -81,3 +82,34 @@ def test_pipeline_end_to_end(tmp_path):
     run("kevd-bandwidth", "--points", X, "--rows", 500, "--grid", "ref36", "--out", bw)
     lines = open(bw).read().strip().split("\n")
     assert len(lines) == 34 + 1 + 2 and lines[34] == ""
+
+
+@pytest.mark.gpu
+def test_bw0_out_matches_reference_columns(tmp_path):
+    """--all-pairs --grid ref36: column 2 of bw0.out is the reference's
+    log(2 S_lower / NM + 1) (ks_bandwidth.cu:615-618) = log(S_full / NM),
+    column 3 the centred slope, then eopt and mdim."""
+    X = tmp_path / "X.npy"
+    pts = clustered_points(700, 6, seed=3)
+    np.save(X, pts)
+    bw, S_f = tmp_path / "bw0.out", tmp_path / "S.npy"
+    run("kevd-bandwidth", "--points", X, "--all-pairs", "--grid", "ref36", "--precision", "fp64",
+        "--out", bw, "--sums-out", S_f)
+    S = np.load(S_f)
+    n, dim = pts.shape
+    bws = np.array([m * 10.0 ** e for e in range(-3, 1) for m in range(1, 10)])
+    # the reference's sums: strictly lower triangle of exp(-|dx|^2 / (dim bw)) / NM
+    from scipy.spatial.distance import pdist
+    D = pdist(pts, "sqeuclidean")
+    S_lower = np.array([np.exp(-D / (dim * b)).sum() for b in bws]) / n
+    ref_col2 = np.log(2.0 * S_lower + 1.0)
+    np.testing.assert_allclose(S, 2.0 * S_lower * n + n, rtol=1e-11)
+    lines = open(bw).read().strip().split("\n")
+    vals = np.array([[float(v) for v in l.split()] for l in lines[:34]])
+    np.testing.assert_allclose(vals[:, 0], bws[1:-1], rtol=1e-5)
+    np.testing.assert_allclose(vals[:, 1], ref_col2[1:-1], rtol=2e-5, atol=1e-9)
+    slope = (ref_col2[2:] - ref_col2[:-2]) / (np.log(bws[2:]) - np.log(bws[:-2]))
+    np.testing.assert_allclose(vals[:, 2], slope, rtol=2e-5, atol=1e-9)
+    idmax = int(np.argmax(slope))
+    assert abs(float(lines[35]) - bws[idmax + 1]) <= 1e-5 * bws[idmax + 1]
+    assert abs(float(lines[36]) - 2.0 * slope[idmax]) <= 2e-5 * abs(slope[idmax]) + 1e-9

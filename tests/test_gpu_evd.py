@@ -165,3 +165,34 @@ def test_leading_solver_matches_full_eigensolve(gpu_ctx):
     assert np.all(np.abs(lead.lam[-k:] - lam[-k:]) <= 1e-8 * np.abs(lam[-k:]) + 1e-14)
     Ua = _align(lead.U.astype(np.float64), U)
     assert np.max(np.abs(Ua[:, -16:] - U[:, -16:])) < 1e-5
+
+
+def test_leading_solve_falls_back_on_slowly_decaying_spectrum(gpu_ctx):
+    """A factor whose bistochastic spectrum is known: F = [1, Q diag(sqrt(n
+    lam))] with Q orthonormal and orthogonal to 1 gives K^ 1 = n 1, so the
+    Sinkhorn scaling is 1/sqrt(n) and P = 11^T/n + Q diag(lam) Q^T, with
+    eigenvalues 1 and lam_j = 0.5 * 0.9997^j.  For k = 100 the block
+    subspace iteration (l = 256 columns) contracts by lam_257/lam_100 ~ 0.954
+    per step and cannot converge within its step limit; the full solve must
+    supply the leading pairs."""
+    from kernel_evd_amd import Context
+    n, r, k = 4096, 600, 100
+    rng = np.random.default_rng(21)
+    G = rng.standard_normal((n, r - 1))
+    G -= G.mean(axis=0)
+    Q, _ = np.linalg.qr(G)
+    lam = 0.5 * 0.9997 ** np.arange(r - 1)
+    F = np.empty((n, r))
+    F[:, 0] = 1.0
+    F[:, 1:] = Q * np.sqrt(lam * n)
+    exact = np.concatenate([[1.0], lam])[:k]
+    with Context(0) as c:
+        c.set_points(gauss_points(n, 3, seed=22))
+        c.set_factor(F)
+        e = c.evd(r, k_out=k, norm="sinkhorn", prec="fp64", want_U=False)
+        tm = c.timings()
+    top = e.lam[::-1][:k]
+    print("fallback", tm["eig_full_fallback"], "iters", tm["eig_iters"], "max rel err",
+          np.max(np.abs(top - exact) / exact), flush=True)
+    assert tm["eig_full_fallback"] == 1
+    np.testing.assert_allclose(top, exact, rtol=1e-10)

@@ -231,3 +231,54 @@ def test_ks_mean_conserved_and_fourth_order():
 def test_gauss_ic_is_standard_normal():
     u = np.concatenate([orc.gauss_ic(64, s, scale=1.0) for s in range(60)])
     assert abs(u.mean()) < 0.08 and abs(u.std() - 1.0) < 0.06
+
+
+def test_dense_reference_leading_subset_equals_full():
+    X = clustered_points(300, 4, seed=13)
+    K = orc.dense_kernel(X, orc.median_sqdist(X))
+    w, _, _ = orc.dense_reference_evd(K)
+    wk, Vk, _ = orc.dense_reference_evd(K, k=12)
+    assert np.max(np.abs(wk - w[:12])) < 1e-13 and abs(wk[0] - 1.0) < 1e-12
+    assert Vk.shape == (300, 12)
+
+
+def test_arpc_stops_when_no_residual_exceeds_tol():
+    X = clustered_points(500, 3, seed=14)
+    eps = orc.median_sqdist(X)
+    # tol above every residual entry (all start at K_ii = 1): nothing to accept
+    r = orc.arpc(X, eps, 50, block=16, seed=1, tol=2.0)
+    assert r.rank == 0 and r.rounds == 0
+    # five distinct points, forty copies each: once one copy of each is a
+    # pivot every residual entry is at rounding level (below tol) and the
+    # run stops at rank 5 instead of running out its round budget
+    base = gauss_points(5, 4, seed=15)
+    Xd = np.repeat(base, 40, axis=0)
+    r = orc.arpc(Xd, orc.median_sqdist(base), 50, block=16, seed=2)
+    assert r.rank == 5 and r.rounds < 10
+    assert len(set(map(tuple, Xd[r.pivots]))) == 5
+    assert np.all(np.isfinite(r.F)) and r.trace_err[-1] < 1e-12
+
+
+def _post(eps, S):
+    from kernel_evd_amd import bandwidth_post
+    return bandwidth_post(eps, S)
+
+
+def test_bandwidth_post_closed_forms():
+    """ks_bandwidth.cu:609-642 on sums with a known slope.  S = c eps^(m/2):
+    every centred slope is m/2, so mdim = m (the location of the maximum is
+    decided by rounding there).
+    log S = (sqrt(pi)/2) erf(log eps - x0): the slope peaks at x0, where the
+    centred difference over a grid step h is sqrt(pi) erf(h) / (2 h)."""
+    eps = np.logspace(-2, 2, 41)
+    logS, slope, eopt, mdim = _post(eps, 3.0 * eps ** 2.5)
+    np.testing.assert_allclose(slope, 2.5, rtol=1e-12)
+    assert abs(mdim - 5.0) < 1e-11
+    np.testing.assert_allclose(logS, np.log(3.0) + 2.5 * np.log(eps), rtol=1e-13)
+    from scipy.special import erf
+    x = np.log(eps)
+    x0 = x[23]
+    h = x[1] - x[0]
+    logS, slope, eopt, mdim = _post(eps, np.exp(0.5 * np.sqrt(np.pi) * erf(x - x0)))
+    assert eopt == eps[23]
+    assert abs(mdim - np.sqrt(np.pi) * erf(h) / h) < 1e-10
