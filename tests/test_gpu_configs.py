@@ -80,7 +80,9 @@ def test_c2_c3_rank2000_factor_and_evd():
               % (e.sinkhorn_iters, e.rowsum_dev, _rel_top(e.lam, e64.lam, 32), _rel_top(e.lam, e64.lam, 100)),
               flush=True)
         assert e.rowsum_dev <= 1e-12 and e64.rowsum_dev <= 1e-12
-        assert abs(e.lam[-1] - 1.0) < 1e-10 and abs(e64.lam[-1] - 1.0) < 1e-12
+        # lambda_max = 1: to FP64 rounding on the FP64 path, to the FP32
+        # Gram's rounding on the mixed one
+        assert abs(e.lam[-1] - 1.0) < 1e-7 and abs(e64.lam[-1] - 1.0) < 1e-12
         assert _rel_top(e.lam, e64.lam, 32) < 1e-5
         er = c.evd(rank, k_out=100, norm="ref", prec="mixed", want_U=False)
         er64 = c64.evd(rank, k_out=100, norm="ref", prec="fp64", want_U=False)
@@ -249,7 +251,9 @@ def test_c4_ensemble_48_problems():
                 np.testing.assert_array_equal(o.trace_err, ref.trace_err)
                 assert np.array_equal(ctxs[i].get_factor(o.rank, "mixed"), s.get_factor(ref.rank, "mixed")), i
                 er = s.evd(ref.rank, k_out=32, norm="sinkhorn", prec="mixed", want_U=False)
-                np.testing.assert_allclose(evs[i].lam[-32:], er.lam[-32:], rtol=1e-12, atol=0)
+                # the two solves may stop at different steps (the first Rayleigh-Ritz
+                # check follows each context's last solve): ~1e-12 relative
+                np.testing.assert_allclose(evs[i].lam[-32:], er.lam[-32:], rtol=1e-10, atol=0)
         # FP64 ensemble: the three smallest problems against the oracle, and
         # every problem's top-32 against its mixed run on the same pivots
         o64, _ = ensemble_arpc(ctxs, eps, ranks, blocks, seeds=seeds, prec="fp64")
