@@ -192,6 +192,23 @@ kevd_status kevd_bandwidth_sweep(kevd_ctx* ctx, const int64_t* rows, int64_t nro
                                  const double* eps, int32_t neps,
                                  kevd_precision prec, double* S_out);
 
+/* kevd_bandwidth_sweep without any device allocation.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and kernels as kevd_bandwidth_sweep; every intermediate
+ * (row indices, gathered rows, eps coefficients, per-block partial sums)
+ * lives in a fixed scratch array of KEVD_SCRATCH_MAX_BYTES per device, which
+ * the library holds until the process ends.  The points must be resident
+ * (kevd_set_points).  A call whose intermediates exceed the array returns
+ * KEVD_EINVAL with the byte count (about 0.5 KB per row at dim 64 in FP64,
+ * plus 1 KB per partial-sum block: some 50,000 rows).  Every context on a
+ * device shares that device's array: calls that overlap in time on one
+ * device race and must be serialised by the caller. */
+#define KEVD_SCRATCH_MAX_BYTES (32u << 20)
+kevd_status kevd_bandwidth_sweep_noalloc(kevd_ctx* ctx, const int64_t* rows, int64_t nrows,
+                                         const double* eps, int32_t neps,
+                                         kevd_precision prec, double* S_out);
+
 /* Post-processing of a sweep, ks_bandwidth.cu:609-642: logS_i = log(S_i),
  * the centred-difference slope over log eps, the first maximum of the slope,
  * eopt = eps[idmax+1] and mdim = 2 * slope[idmax].  slope has neps-2

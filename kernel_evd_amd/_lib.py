@@ -75,6 +75,7 @@ SIGNATURES = {
     "kevd_evd": (C.c_int, [C.c_void_p, P(EvdParams), P(EvdResult)]),
     "kevd_get_u": (C.c_int, [C.c_void_p, P(C.c_float)]),
     "kevd_bandwidth_sweep": (C.c_int, [C.c_void_p, P(i64), i64, P(dbl), i32, C.c_int, P(dbl)]),
+    "kevd_bandwidth_sweep_noalloc": (C.c_int, [C.c_void_p, P(i64), i64, P(dbl), i32, C.c_int, P(dbl)]),
     "kevd_bandwidth_post": (C.c_int, [P(dbl), P(dbl), i32, P(dbl), P(dbl), P(dbl), P(dbl)]),
     "kevd_ks_generate": (C.c_int, [P(KsParams), i64, P(dbl)]),
     "kevd_delay_embed": (C.c_int, [P(dbl), i64, i32, i32, P(dbl)]),

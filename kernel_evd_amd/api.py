@@ -188,13 +188,15 @@ class Context:
         return U
 
     # --------------------------------------------------------- bandwidth
-    def bandwidth_sweep(self, rows, eps, prec="mixed"):
+    def bandwidth_sweep(self, rows, eps, prec="mixed", noalloc=False):
+        """kevd_bandwidth_sweep; noalloc=True calls the allocation-free
+        variant (kevd_bandwidth_sweep_noalloc)."""
         rows = np.ascontiguousarray(rows, dtype=np.int64)
         eps = np.ascontiguousarray(eps, dtype=np.float64)
         S = np.zeros(eps.size, dtype=np.float64)
-        self._check(self._lib.kevd_bandwidth_sweep(self._h, _p(rows, C.c_int64), rows.size,
-                                                   _p(eps, C.c_double), eps.size, PREC[prec],
-                                                   _p(S, C.c_double)))
+        f = self._lib.kevd_bandwidth_sweep_noalloc if noalloc else self._lib.kevd_bandwidth_sweep
+        self._check(f(self._h, _p(rows, C.c_int64), rows.size, _p(eps, C.c_double), eps.size, PREC[prec],
+                      _p(S, C.c_double)))
         return S
 
     def autotune_block(self, eps, rank, candidates=(32, 64, 128), seed=0, tol=1e-10, prec="mixed",

@@ -364,27 +364,83 @@ __global__ __launch_bounds__(256) void k_bw_reduce(const double* part, int nblk,
     if (threadIdx.x == 0) S[j] = red[0];
 }
 
+// Launch shape and buffer sizes of one sweep.
+struct SweepPlan {
+    int nset = 0, nep = 0;
+    size_t ncoef = 0, esz = 0;
+    int64_t rt = 0, ncs = 0, cpb = 0;
+    int nblk = 0;
+};
+
 template <typename T>
-kevd_status sweep(kevd_ctx* ctx, const int64_t* ri, int64_t nrows, const double* eps, int neps,
-                  double* S_out)
+SweepPlan plan_sweep(int64_t n, int64_t nrows, int neps)
+{
+    constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
+    const bool f32 = sizeof(T) == 4;
+    SweepPlan P;
+    P.esz = sizeof(T);
+    P.nep = (neps + ECH - 1) / ECH * ECH;
+    P.nset = (neps + EPS_SET - 1) / EPS_SET;
+    // FP64: the coefficients padded to whole chunks of ECH; FP32: the B
+    // operands of the outer-product sets (see k_bw_sweep_f32)
+    P.ncoef = f32 ? (size_t)P.nset * 2 * 64 : (size_t)P.nep;
+    P.rt = (nrows + BM - 1) / BM;
+    int64_t ctiles = (n + BN - 1) / BN;
+    int64_t want = 4096 / (P.rt > 0 ? P.rt : 1);
+    if (want < 1) want = 1;
+    if (want > ctiles) want = ctiles;
+    P.cpb = ((ctiles + want - 1) / want) * BN;
+    P.ncs = (n + P.cpb - 1) / P.cpb;
+    P.nblk = (int)(P.rt * P.ncs);
+    return P;
+}
+
+// Device buffers of one sweep: the rows' indices, the gathered rows and
+// their norms, the eps coefficients, the per-block partial sums and S.
+struct SweepBufs {
+    int64_t* ri = nullptr;
+    void* XR = nullptr;
+    double* nR = nullptr;
+    void* coef = nullptr;
+    double* part = nullptr;
+    double* S = nullptr;
+};
+
+// Carves SweepBufs out of one region at 256-byte aligned offsets; returns
+// the bytes used (base may be null to only measure).
+size_t carve_sweep(const SweepPlan& P, int64_t nrows, int32_t dp, char* base, SweepBufs* B)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base ? base + off : nullptr;
+        off += KEVD_ROUND_UP(bytes, 256);
+        return p;
+    };
+    SweepBufs b;
+    b.ri = (int64_t*)take((size_t)nrows * sizeof(int64_t));
+    b.XR = take((size_t)nrows * dp * P.esz);
+    b.nR = (double*)take((size_t)nrows * sizeof(double));
+    b.coef = take(P.ncoef * P.esz);
+    b.part = (double*)take((size_t)P.nblk * NEPS_MAX * sizeof(double));
+    b.S = (double*)take(NEPS_MAX * sizeof(double));
+    if (B) *B = b;
+    return off;
+}
+
+// One sweep on the context stream: rows and eps from the host, S to the
+// host.  Every buffer is written before it is read (the gather fills XR and
+// nR, the coefficient copy covers all ncoef entries, the sweep kernel writes
+// every partial), so none needs clearing.
+template <typename T>
+kevd_status sweep(kevd_ctx* ctx, const SweepPlan& P, const SweepBufs& B, const int64_t* rows, int64_t nrows,
+                  const double* eps, int neps, double* S_out)
 {
     int64_t n = ctx->n;
     int32_t dp = ctx->dp;
-    constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
-    bool f32 = sizeof(T) == 4;
-    T* XR;
-    double* nR;
-    T* coef;
-    double *part, *S;
-    KEVD_TRY(ctx_buf_t(ctx, "bw_XR", (size_t)nrows * dp, &XR));
-    KEVD_TRY(ctx_buf_t(ctx, "bw_nR", nrows, &nR));
-    KEVD_TRY(kblock_gather(ctx, ri, nrows, f32 ? KEVD_MIXED : KEVD_FP64, XR, nR));
-    int nep = (neps + ECH - 1) / ECH * ECH;
-    int nset = (neps + EPS_SET - 1) / EPS_SET;
-    // FP64: the coefficients padded to whole chunks of ECH; FP32: the B
-    // operands of the outer-product sets (see k_bw_sweep_f32)
-    size_t ncoef = f32 ? (size_t)nset * 2 * 64 : (size_t)nep;
-    std::vector<T> hc(ncoef, (T)0);
+    const bool f32 = sizeof(T) == 4;
+    KEVD_HIP(ctx, hipMemcpyAsync(B.ri, rows, nrows * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    KEVD_TRY(kblock_gather(ctx, B.ri, nrows, f32 ? KEVD_MIXED : KEVD_FP64, B.XR, B.nR));
+    std::vector<T> hc(P.ncoef, (T)0);
     for (int j = 0; j < neps; ++j) {
         if (f32) {
             float c = (float)(-1.4426950408889634074 / eps[j]);
@@ -395,44 +451,37 @@ kevd_status sweep(kevd_ctx* ctx, const int64_t* ri, int64_t nrows, const double*
             hc[j] = (T)(-1.0 / eps[j]);
         }
     }
-    KEVD_TRY(ctx_buf_t(ctx, "bw_coef", ncoef, &coef));
-    KEVD_HIP(ctx, hipMemcpyAsync(coef, hc.data(), ncoef * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
-    int64_t rt = (nrows + BM - 1) / BM;
-    int64_t ctiles = (n + BN - 1) / BN;
-    int64_t want = 4096 / (rt > 0 ? rt : 1);
-    if (want < 1) want = 1;
-    if (want > ctiles) want = ctiles;
-    int64_t cpb = ((ctiles + want - 1) / want) * BN;
-    int64_t ncs = (n + cpb - 1) / cpb;
-    int nblk = (int)(rt * ncs);
-    KEVD_TRY(ctx_buf_t(ctx, "bw_part", (size_t)nblk * NEPS_MAX, &part));
-    KEVD_TRY(ctx_buf_t(ctx, "bw_S", NEPS_MAX, &S));
+    KEVD_HIP(ctx, hipMemcpyAsync(B.coef, hc.data(), P.ncoef * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
     const T* X = f32 ? (const T*)ctx->X32 : (const T*)ctx->X64;
     const double* nrm = f32 ? ctx->nrm32 : ctx->nrm64;
-    dim3 grid((unsigned)rt, (unsigned)ncs);
+    const T* XR = (const T*)B.XR;
+    const T* coef = (const T*)B.coef;
+    const int64_t* ri = B.ri;
+    const double* nR = B.nR;
+    dim3 grid((unsigned)P.rt, (unsigned)P.ncs);
     if constexpr (sizeof(T) == 4) {
-        const float *XRf = (const float*)XR, *Xf = (const float*)X, *cf = (const float*)coef;
         if (dp == 32)
-            k_bw_sweep_f32r<32><<<grid, 256, 0, ctx->stream>>>(XRf, nR, ri, nrows, Xf, nrm, n, cf, nset, cpb, part);
+            k_bw_sweep_f32r<32><<<grid, 256, 0, ctx->stream>>>(XR, nR, ri, nrows, X, nrm, n, coef, P.nset, P.cpb,
+                                                               B.part);
         else if (dp == 64)
-            k_bw_sweep_f32r<64><<<grid, 256, 0, ctx->stream>>>(XRf, nR, ri, nrows, Xf, nrm, n, cf, nset, cpb, part);
+            k_bw_sweep_f32r<64><<<grid, 256, 0, ctx->stream>>>(XR, nR, ri, nrows, X, nrm, n, coef, P.nset, P.cpb,
+                                                               B.part);
         else
-            k_bw_sweep_f32<<<grid, 256, 0, ctx->stream>>>(XRf, nR, ri, nrows, Xf, nrm, n, dp, cf, nset, cpb, part);
+            k_bw_sweep_f32<<<grid, 256, 0, ctx->stream>>>(XR, nR, ri, nrows, X, nrm, n, dp, coef, P.nset, P.cpb,
+                                                          B.part);
+    } else {
+        k_bw_sweep<T><<<grid, 256, 0, ctx->stream>>>(XR, nR, ri, nrows, X, nrm, n, dp, coef, P.nep, P.cpb,
+                                                     B.part);
     }
-    else
-        k_bw_sweep<T><<<grid, 256, 0, ctx->stream>>>(XR, nR, ri, nrows, X, nrm, n, dp, coef, nep, cpb, part);
-    k_bw_reduce<<<neps, 256, 0, ctx->stream>>>(part, nblk, S);
+    k_bw_reduce<<<neps, 256, 0, ctx->stream>>>(B.part, P.nblk, B.S);
     KEVD_HIP(ctx, hipGetLastError());
-    KEVD_HIP(ctx, hipMemcpyAsync(S_out, S, neps * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KEVD_HIP(ctx, hipMemcpyAsync(S_out, B.S, neps * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return KEVD_OK;
 }
 
-}  // namespace
-
-extern "C" kevd_status kevd_bandwidth_sweep(kevd_ctx* ctx, const int64_t* rows, int64_t nrows,
-                                            const double* eps, int32_t neps, kevd_precision prec,
-                                            double* S_out)
+kevd_status sweep_validate(kevd_ctx* ctx, const int64_t* rows, int64_t nrows, const double* eps, int32_t neps,
+                           kevd_precision prec, const double* S_out)
 {
     if (!ctx || ctx->n < 1) return kevd_fail(ctx, KEVD_EINVAL, "bandwidth: no points");
     if (!rows || nrows < 1 || !eps || neps < 1 || neps > NEPS_MAX || !S_out)
@@ -444,12 +493,61 @@ extern "C" kevd_status kevd_bandwidth_sweep(kevd_ctx* ctx, const int64_t* rows, 
         if (rows[i] < 0 || rows[i] >= ctx->n)
             return kevd_fail(ctx, KEVD_EINVAL, "bandwidth: row index out of range");
     if (prec != KEVD_FP64 && prec != KEVD_MIXED) return kevd_fail(ctx, KEVD_EINVAL, "bandwidth: precision");
+    return KEVD_OK;
+}
+
+template <typename T>
+kevd_status sweep_ctx_bufs(kevd_ctx* ctx, const int64_t* rows, int64_t nrows, const double* eps, int neps,
+                           double* S_out)
+{
+    SweepPlan P = plan_sweep<T>(ctx->n, nrows, neps);
+    SweepBufs B;
+    KEVD_TRY(ctx_buf_t(ctx, "bw_ri", nrows, &B.ri));
+    KEVD_TRY(ctx_buf(ctx, "bw_XR", (size_t)nrows * ctx->dp * sizeof(T), &B.XR));
+    KEVD_TRY(ctx_buf_t(ctx, "bw_nR", nrows, &B.nR));
+    KEVD_TRY(ctx_buf(ctx, "bw_coef", P.ncoef * sizeof(T), &B.coef));
+    KEVD_TRY(ctx_buf_t(ctx, "bw_part", (size_t)P.nblk * NEPS_MAX, &B.part));
+    KEVD_TRY(ctx_buf_t(ctx, "bw_S", NEPS_MAX, &B.S));
+    return sweep<T>(ctx, P, B, rows, nrows, eps, neps, S_out);
+}
+
+template <typename T>
+kevd_status sweep_scratch(kevd_ctx* ctx, const int64_t* rows, int64_t nrows, const double* eps, int neps,
+                          double* S_out)
+{
+    SweepPlan P = plan_sweep<T>(ctx->n, nrows, neps);
+    size_t need = carve_sweep(P, nrows, ctx->dp, nullptr, nullptr);
+    if (need > KEVD_SCRATCH_BYTES)
+        return kevd_fail(ctx, KEVD_EINVAL,
+                         "bandwidth (allocation-free): " + std::to_string(nrows) + " rows need " +
+                             std::to_string(need) + " bytes of scratch, more than the fixed " +
+                             std::to_string(KEVD_SCRATCH_BYTES));
+    if (!ctx->scratch) return kevd_fail(ctx, KEVD_EHIP, "bandwidth (allocation-free): no scratch address");
+    SweepBufs B;
+    carve_sweep(P, nrows, ctx->dp, (char*)ctx->scratch, &B);
+    return sweep<T>(ctx, P, B, rows, nrows, eps, neps, S_out);
+}
+
+}  // namespace
+
+extern "C" kevd_status kevd_bandwidth_sweep(kevd_ctx* ctx, const int64_t* rows, int64_t nrows,
+                                            const double* eps, int32_t neps, kevd_precision prec,
+                                            double* S_out)
+{
+    KEVD_TRY(sweep_validate(ctx, rows, nrows, eps, neps, prec, S_out));
     KEVD_HIP(ctx, hipSetDevice(ctx->device));
-    int64_t* ri;
-    KEVD_TRY(ctx_buf_t(ctx, "bw_ri", nrows, &ri));
-    KEVD_HIP(ctx, hipMemcpyAsync(ri, rows, nrows * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
-    return prec == KEVD_MIXED ? sweep<float>(ctx, ri, nrows, eps, neps, S_out)
-                              : sweep<double>(ctx, ri, nrows, eps, neps, S_out);
+    return prec == KEVD_MIXED ? sweep_ctx_bufs<float>(ctx, rows, nrows, eps, neps, S_out)
+                              : sweep_ctx_bufs<double>(ctx, rows, nrows, eps, neps, S_out);
+}
+
+extern "C" kevd_status kevd_bandwidth_sweep_noalloc(kevd_ctx* ctx, const int64_t* rows, int64_t nrows,
+                                                    const double* eps, int32_t neps, kevd_precision prec,
+                                                    double* S_out)
+{
+    KEVD_TRY(sweep_validate(ctx, rows, nrows, eps, neps, prec, S_out));
+    KEVD_HIP(ctx, hipSetDevice(ctx->device));
+    return prec == KEVD_MIXED ? sweep_scratch<float>(ctx, rows, nrows, eps, neps, S_out)
+                              : sweep_scratch<double>(ctx, rows, nrows, eps, neps, S_out);
 }
 
 // ks_bandwidth.cu:609-642: log of the sums, centred-difference slope over

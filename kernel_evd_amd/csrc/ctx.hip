@@ -3,9 +3,33 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "gemm.h"
 #include "internal.h"
+
+// The allocation-free entry points keep their intermediates here: a global of
+// the code object, so every device the library is loaded on holds its own copy
+// at its own address.  Host code never passes this name to a kernel; it uses
+// the per-device address resolved below (kevd_ctx::scratch).
+static __device__ __align__(256) unsigned char scratch[KEVD_SCRATCH_BYTES];
+
+// Device address of `scratch` on the current device, resolved once per device.
+static kevd_status scratch_address(int device, void** out)
+{
+    static std::mutex mu;
+    static std::map<int, void*> per_device;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = per_device.find(device);
+    if (it == per_device.end()) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return KEVD_EHIP;
+        it = per_device.emplace(device, p).first;
+    }
+    *out = it->second;
+    return KEVD_OK;
+}
 
 // ------------------------------------------------------------ errors, bufs
 
@@ -161,6 +185,7 @@ extern "C" kevd_status kevd_create(int device, kevd_ctx** out)
         return KEVD_EHIP;
     }
     kevd_status s = ctx_check_arch(ctx);
+    if (s == KEVD_OK) s = scratch_address(device, &ctx->scratch);
     if (s != KEVD_OK) {
         delete ctx;
         return s;

@@ -8,6 +8,10 @@
 
 #include "common.h"
 
+// Size of the fixed device scratch array behind the allocation-free entry
+// points (one array per device, held until the process ends).
+constexpr size_t KEVD_SCRATCH_BYTES = KEVD_SCRATCH_MAX_BYTES;
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -59,6 +63,11 @@ struct kevd_ctx {
     // this context converged (0: none yet); the next solve runs its first
     // Rayleigh-Ritz check there.
     int eig_converged_at = 0;
+
+    // Device address of this device's fixed scratch array (KEVD_SCRATCH_BYTES),
+    // resolved when the context is created; shared by every context on the
+    // device.
+    void* scratch = nullptr;
 
     // kEVD output held on the device.
     float* U = nullptr;
