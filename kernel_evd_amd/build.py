@@ -67,7 +67,7 @@ def build_library(jobs=8):
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas",
+         ["-L" + os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas", "-lrocprofiler-sdk-roctx",
           "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
     return LIB
@@ -121,6 +121,30 @@ def build_drivers(jobs=8):
 
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         return list(ex.map(one, DRIVERS))
+
+
+def build_asan():
+    """The drivers' host code (KS generation, delay embedding, .npy and HDF5
+    I/O, point loading) with the address and undefined-behaviour sanitizers:
+    _build/asan/asan_check, run by the CPU tests.  g++ only (no device code is
+    instrumented); the library supplies the remaining symbols."""
+    out_dir = os.path.join(OBJ, "asan")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "asan_check")
+    h5 = find_hdf5()
+    h5flags = ([], [])
+    priv = os.path.join(BIN, "hdf5")
+    if h5 and os.path.isdir(priv):
+        import glob
+        libs = glob.glob(os.path.join(priv, "libhdf5.so.*"))
+        if libs:
+            h5flags = (["-DKEVD_HAVE_HDF5", "-isystem", h5[0]], [libs[0], "-Wl,-rpath," + priv])
+    srcs = [os.path.join(DRV, "asan_check.cpp"), os.path.join(DRV, "npy.cpp"), os.path.join(DRV, "arrays.cpp"),
+            os.path.join(CSRC, "ksgen.cpp")]
+    _run(["g++", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-std=c++17",
+          "-I" + INC, "-I" + CSRC, "-I" + DRV] + h5flags[0] + ["-o", exe] + srcs +
+         ["-L" + PKG, "-lkevd", "-Wl,-rpath," + PKG, "-Wl,-rpath," + os.path.join(ROCM, "lib")] + h5flags[1])
+    return exe
 
 
 def build_oracle():

@@ -515,6 +515,7 @@ kevd_status arpc_finish(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds
 
 extern "C" kevd_status kevd_arpc(kevd_ctx* ctx, const kevd_arpc_params* p, kevd_arpc_result* res)
 {
+    TraceRange trace_range("kevd_arpc");
     int max_rounds = 0;
     KEVD_TRY(arpck::arpc_validate(ctx, p, res, &max_rounds));
     KEVD_HIP(ctx, hipSetDevice(ctx->device));

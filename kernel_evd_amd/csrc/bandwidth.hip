@@ -534,6 +534,7 @@ extern "C" kevd_status kevd_bandwidth_sweep(kevd_ctx* ctx, const int64_t* rows, 
                                             const double* eps, int32_t neps, kevd_precision prec,
                                             double* S_out)
 {
+    TraceRange trace_range("kevd_bandwidth_sweep");
     KEVD_TRY(sweep_validate(ctx, rows, nrows, eps, neps, prec, S_out));
     KEVD_HIP(ctx, hipSetDevice(ctx->device));
     return prec == KEVD_MIXED ? sweep_ctx_bufs<float>(ctx, rows, nrows, eps, neps, S_out)
@@ -544,6 +545,7 @@ extern "C" kevd_status kevd_bandwidth_sweep_noalloc(kevd_ctx* ctx, const int64_t
                                                     const double* eps, int32_t neps, kevd_precision prec,
                                                     double* S_out)
 {
+    TraceRange trace_range("kevd_bandwidth_sweep_noalloc");
     KEVD_TRY(sweep_validate(ctx, rows, nrows, eps, neps, prec, S_out));
     KEVD_HIP(ctx, hipSetDevice(ctx->device));
     return prec == KEVD_MIXED ? sweep_scratch<float>(ctx, rows, nrows, eps, neps, S_out)

@@ -6,8 +6,13 @@
 #include <map>
 #include <mutex>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "gemm.h"
 #include "internal.h"
+
+TraceRange::TraceRange(const char* name) { roctxRangePushA(name); }
+TraceRange::~TraceRange() { roctxRangePop(); }
 
 // The allocation-free entry points keep their intermediates here: a global of
 // the code object, so every device the library is loaded on holds its own copy
@@ -332,12 +337,14 @@ static kevd_status set_points_impl(kevd_ctx* ctx, const double* X64h, const floa
 
 extern "C" kevd_status kevd_set_points(kevd_ctx* ctx, const double* X, int64_t n, int32_t dim)
 {
+    TraceRange trace_range("kevd_set_points");
     return set_points_impl(ctx, X, nullptr, n, dim);
 }
 
 extern "C" kevd_status kevd_set_points_f32(kevd_ctx* ctx, const float* X, int64_t n,
                                            int32_t dim)
 {
+    TraceRange trace_range("kevd_set_points_f32");
     return set_points_impl(ctx, nullptr, X, n, dim);
 }
 
@@ -379,6 +386,7 @@ __global__ __launch_bounds__(256) void k_pair_sqdist(const double* X64, int64_t 
 extern "C" kevd_status kevd_median_sqdist(kevd_ctx* ctx, uint64_t seed, int64_t nsample,
                                           double* eps_out)
 {
+    TraceRange trace_range("kevd_median_sqdist");
     if (!ctx || !eps_out || ctx->n < 2) return kevd_fail(ctx, KEVD_EINVAL, "median: no points");
     KEVD_HIP(ctx, hipSetDevice(ctx->device));
     int64_t n = ctx->n;
@@ -479,6 +487,7 @@ extern "C" kevd_status kevd_kernel_block(kevd_ctx* ctx, const int64_t* rows, int
                                          const int64_t* cols, int64_t nc, double eps,
                                          kevd_precision prec, double* K_out)
 {
+    TraceRange trace_range("kevd_kernel_block");
     if (!ctx || ctx->n < 1) return kevd_fail(ctx, KEVD_EINVAL, "kernel_block: no points");
     if (nr < 0 || nc < 0 || !(eps > 0.0) || (nr && !rows) || (nc && !cols) ||
         ((nr && nc) && !K_out))

@@ -322,6 +322,7 @@ std::map<TuneKey, int32_t> tune_cache;
 extern "C" kevd_status kevd_ensemble_arpc(kevd_ctx** ctxs, int32_t nprob, const kevd_arpc_params* params,
                                           kevd_arpc_result* results, int32_t* used_graph)
 {
+    TraceRange trace_range("kevd_ensemble_arpc");
     if (!ctxs || nprob < 1 || !params || !results || !ctxs[0]) return KEVD_EINVAL;
     kevd_ctx* m = ctxs[0];
     for (int i = 0; i < nprob; ++i) {
@@ -370,6 +371,7 @@ extern "C" kevd_status kevd_ensemble_arpc(kevd_ctx** ctxs, int32_t nprob, const 
 extern "C" kevd_status kevd_ensemble_evd(kevd_ctx** ctxs, int32_t nprob, const kevd_evd_params* p,
                                          kevd_evd_result* results, int32_t workers)
 {
+    TraceRange trace_range("kevd_ensemble_evd");
     if (!ctxs || nprob < 1 || !p || !results || !ctxs[0]) return KEVD_EINVAL;
     kevd_ctx* m = ctxs[0];
     for (int i = 0; i < nprob; ++i)
@@ -401,6 +403,7 @@ extern "C" kevd_status kevd_autotune_block(kevd_ctx* ctx, const kevd_arpc_params
                                            const int32_t* candidates, int32_t ncand, int32_t* best_block,
                                            double* cols_per_s)
 {
+    TraceRange trace_range("kevd_autotune_block");
     if (!ctx || !base || !candidates || ncand < 1 || !best_block)
         return kevd_fail(ctx, KEVD_EINVAL, "autotune: null argument or no candidates");
     std::vector<int32_t> cand_set(candidates, candidates + ncand);

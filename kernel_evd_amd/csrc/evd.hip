@@ -1811,7 +1811,10 @@ kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* re
     k_weights<<<gn, 256, 0, ctx->stream>>>(dt, n, 0, w1);  // 1/d~^2
     k_weights<<<gn, 256, 0, ctx->stream>>>(qt, n, 1, w2);  // 1/q~
     KEVD_HIP(ctx, hipGetLastError());
-    KEVD_TRY((gram<T, 2>(ctx, w1, w2, A, B)));
+    {
+        TraceRange tr("kEVD Grams A, B");
+        KEVD_TRY((gram<T, 2>(ctx, w1, w2, A, B)));
+    }
     // dense phase 1: A + sI = L L^T (R = L^T)
     double trsh[2];
     {
@@ -1842,7 +1845,10 @@ kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* re
     }
     double* Vk = nullptr;
     int eit = 0;
-    KEVD_TRY(eig_leading(ctx, B, r, k, p->all_eigenvalues != 0, lam, &Vk, &eit));
+    {
+        TraceRange tr("kEVD eigensolve");
+        KEVD_TRY(eig_leading(ctx, B, r, k, p->all_eigenvalues != 0, lam, &Vk, &eit));
+    }
     // Y = L^-T V_k  (the trailing k eigenvectors, ascending)
     {
         ProfScope ps(ctx, TC_DENSE);
@@ -2084,10 +2090,16 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     // s holds the scaling whose deviation was measured last
     k_weights<<<gn, 256, 0, ctx->stream>>>(s, n, 3, w1);  // s^2
     KEVD_HIP(ctx, hipGetLastError());
-    KEVD_TRY((gram<T, 1>(ctx, w1, nullptr, G, nullptr)));
+    {
+        TraceRange tr("kEVD Gram F^T S^2 F");
+        KEVD_TRY((gram<T, 1>(ctx, w1, nullptr, G, nullptr)));
+    }
     double* Vk = nullptr;
     int eit = 0;
-    KEVD_TRY(eig_leading(ctx, G, r, k, p->all_eigenvalues != 0, lam, &Vk, &eit));
+    {
+        TraceRange tr("kEVD eigensolve");
+        KEVD_TRY(eig_leading(ctx, G, r, k, p->all_eigenvalues != 0, lam, &Vk, &eit));
+    }
     k_inv_sqrt_tail<<<(k + 255) / 256, 256, 0, ctx->stream>>>(lam, r, k, isq);
     KEVD_TRY(ugemm<T>(ctx, Vk, k, isq, nullptr, s, ctx->U, nullptr));
     KEVD_HIP(ctx, hipMemcpyAsync(res->lam, lam, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -2114,6 +2126,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
 
 extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res)
 {
+    TraceRange trace_range("kevd_evd");
     if (!ctx || !p || !res) return kevd_fail(ctx, KEVD_EINVAL, "evd: null argument");
     if (!ctx->have_factor || ctx->rank < 1) return kevd_fail(ctx, KEVD_EINVAL, "evd: no factor");
     if (p->k_out < 1 || p->k_out > ctx->rank || p->k_out > 1024)

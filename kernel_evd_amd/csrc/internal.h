@@ -102,6 +102,16 @@ inline kevd_status ctx_buf_t(kevd_ctx* ctx, const char* name, size_t count, P** 
 }
 kevd_status ctx_pinned(kevd_ctx* ctx, size_t bytes, void** out);
 
+// A named roctx range over a host-side scope (an entry point or one of its
+// phases), for timeline tools (rocprofv3 --marker-trace).  The ranges mark
+// the host's enqueueing; device time is in the kernel trace.
+struct TraceRange {
+    explicit TraceRange(const char* name);
+    ~TraceRange();
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
+
 // Profiling: bracket a launch with an event pair on the context stream when
 // profiling is on; resolve the pairs into ctx->tm after the stream syncs.
 struct ProfScope {
