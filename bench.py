@@ -328,12 +328,14 @@ def c1_leg(X, peaks, n=262144, nrows=16384, neps=64, cpu_rows=256, with_cpu=True
     return res
 
 
-def lm_leg(ctx, eps, rank, block, trace_err_ff, prec="mixed"):
+def lm_leg(ctx, eps, rank, block, kout, norm, trace_err_ff, prec="mixed"):
     """The LM form of ARPC (regenerate=1, the reference's own algorithm: S
     and L^-1 kept, K(R,S) regenerated every round) on the same points,
     rank, block and seed: its time, trace error against the factor form's,
     the factor memory it does without, and how well its L^-1 inverts K(S,S)
-    (max |L^-1 K(S,S) L^-T - I|, every quantity on that path being FP64)."""
+    (max |L^-1 K(S,S) L^-T - I|, every quantity on that path being FP64).
+    Then the kEVD with no factor held (K(R,S) panels regenerated in every
+    pass, FP64), against the FP64 factor-form kEVD on the same pivots."""
     ctx.arpc(eps, rank, block=block, seed=0, prec=prec, regenerate=True)  # sizes the buffers
     t0 = time.perf_counter()
     a = ctx.arpc(eps, rank, block=block, seed=0, prec=prec, regenerate=True)
@@ -347,6 +349,18 @@ def lm_leg(ctx, eps, rank, block, trace_err_ff, prec="mixed"):
            "linv_identity_err_max": ident}
     log("LM ARPC: %.3f s, trace err %.3e (factor form %.3e), |L K L^T - I| %.1e"
         % (t, a.trace_err[-1], trace_err_ff, ident))
+    run_evd(ctx, a.rank, kout, "fp64", norm)  # sizes the panel buffers
+    t0 = time.perf_counter()
+    e_lm = run_evd(ctx, a.rank, kout, "fp64", norm)
+    t_evd = time.perf_counter() - t0
+    ctx.arpc(eps, a.rank, block=block, seed=0, prec="fp64", inject_pivots=a.pivots)
+    e_ff = run_evd(ctx, a.rank, kout, "fp64", norm)
+    k = min(kout, a.rank)
+    rel = np.abs(e_lm.lam[::-1][:k] - e_ff.lam[::-1][:k]) / np.abs(e_ff.lam[::-1][:k])
+    res["evd_without_factor"] = {"s": round(t_evd, 4), "precision": "fp64", "norm": norm,
+                                 "eig_rel_err_vs_factor_form_max": float(rel.max()),
+                                 "rowsum_dev": float(e_lm.rowsum_dev)}
+    log("kEVD without a factor: %.3f s, top-%d rel err vs factor form %.2e" % (t_evd, k, rel.max()))
     return res
 
 
@@ -625,7 +639,7 @@ def main():
         ctx64.close()
 
     if extras:
-        out["lm_arpc"] = lm_leg(ctx, eps, args.rank, args.block, float(a_mix.trace_err[-1]))
+        out["lm_arpc"] = lm_leg(ctx, eps, args.rank, args.block, args.kout, norm, float(a_mix.trace_err[-1]))
 
     if not args.no_ensemble:
         ctx.close()

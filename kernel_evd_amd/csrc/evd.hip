@@ -1031,6 +1031,50 @@ __global__ void k_inv_sqrt_tail(const double* lam, int64_t r, int k, double* out
     out[c] = l > 0.0 ? 1.0 / sqrt(l) : 0.0;
 }
 
+// ------------------------------------------------- regenerated K panels
+
+// K(p0 + i, S_j) for i < np, j < r, into out (row-major, leading dimension
+// ldo; columns r .. ldo-1 zero): the distance product on MFMA with FP64
+// norms and the downdate's Gaussian (gauss_val), exactly 1 on a point's own
+// pivot.  One BM x BN tile per block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_kpanel(const T* __restrict__ X, const double* __restrict__ nrm,
+                                                int32_t dp, int64_t p0, int64_t np, const T* __restrict__ XS,
+                                                const double* __restrict__ nrmS, const int64_t* __restrict__ piv,
+                                                int64_t r, GaussScale gs, T* __restrict__ out, int64_t ldo)
+{
+    typedef typename Cfg<T>::G G;
+    constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
+    __shared__ __attribute__((aligned(16))) T lds[G::LDS_ELEMS];
+    G g;
+    g.init();
+    const int64_t r0 = (int64_t)blockIdx.x * BM, c0 = (int64_t)blockIdx.y * BN;
+    g.mainloop(lds, X + (p0 + r0) * dp, dp, np - r0, XS + c0 * dp, dp, r - c0, dp);
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int b = 0; b < G::NTN; ++b)
+#pragma unroll
+            for (int v = 0; v < G::M::ACC; ++v) {
+                const int64_t row = r0 + g.row_of(a, v), col = c0 + g.col_of(b);
+                if (row >= np || col >= ldo) continue;
+                T kv = (T)0;
+                if (col < r) {
+                    double D = nrm[p0 + row] + nrmS[col] - 2.0 * (double)g.acc[a][b][v];
+                    D = D > 0.0 ? D : 0.0;
+                    kv = piv[col] == p0 + row ? (T)1 : gauss_val(D, gs, (T*)nullptr);
+                }
+                out[row * ldo + col] = kv;
+            }
+}
+
+// y += x over count entries.
+__global__ void k_add(double* y, const double* x, int64_t count)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) y[i] += x[i];
+}
+
 // ------------------------------------------------------------- host side
 
 struct EvdBufs {
@@ -1038,40 +1082,68 @@ struct EvdBufs {
     double *w1 = nullptr, *w2 = nullptr, *bmin = nullptr, *bdev = nullptr, *mm = nullptr;
 };
 
+// Rows [0, n) of an n x r row-major matrix with leading dimension ldf: the
+// resident factor, or one regenerated panel of K(:,S).
+template <typename T> struct Rows {
+    const T* F;
+    int64_t ldf, n, r;
+};
+
+// part[b][j] for the COLSUM_ROWS-row blocks of src (block b of src is
+// part row b): column partials of w^T src (w null: ones).
 template <typename T>
-kevd_status colsum(kevd_ctx* ctx, const double* w, double* c)
+void colsum_part(kevd_ctx* ctx, const Rows<T>& src, const double* w, double* part, int64_t rpad)
 {
-    int64_t n = ctx->n, r = ctx->rank;
-    int nb = (int)((n + COLSUM_ROWS - 1) / COLSUM_ROWS);
-    int64_t rpad = KEVD_ROUND_UP(r, 4);
+    int nb = (int)((src.n + COLSUM_ROWS - 1) / COLSUM_ROWS);
+    k_colsum<T><<<nb, 256, 0, ctx->stream>>>(src.F, src.ldf, src.n, src.r, w, part, rpad);
+}
+
+template <typename T>
+kevd_status colsum_rows(kevd_ctx* ctx, const Rows<T>& src, const double* w, double* c)
+{
+    int nb = (int)((src.n + COLSUM_ROWS - 1) / COLSUM_ROWS);
+    int64_t rpad = KEVD_ROUND_UP(src.r, 4);
     double* part = nullptr;
     KEVD_TRY(ctx_buf_t(ctx, "e_cpart", (size_t)nb * rpad, &part));
     ProfScope ps(ctx, TC_MATVEC);
-    k_colsum<T><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, w, part, rpad);
-    k_colsum_reduce<<<(unsigned)((r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, r, rpad, c);
+    colsum_part<T>(ctx, src, w, part, rpad);
+    k_colsum_reduce<<<(unsigned)((src.r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, src.r, rpad, c);
     KEVD_HIP(ctx, hipGetLastError());
     return KEVD_OK;
 }
 
+// Launch of k_rowdot over src with nb blocks (per-block min / max into
+// bmin, bdev).
+template <typename T>
+void rowdot_launch(kevd_ctx* ctx, const Rows<T>& src, const double* c, double* y, int mode, const double* s,
+                   double* s_new, double* bmin, double* bdev, int nb)
+{
+    if (src.r <= ROWDOT_LDS_MAX)
+        k_rowdot<T, true><<<nb, 256, src.r * sizeof(double), ctx->stream>>>(src.F, src.ldf, src.n, src.r, c, y,
+                                                                            bmin, mode, s, s_new, bdev);
+    else
+        k_rowdot<T, false><<<nb, 256, 0, ctx->stream>>>(src.F, src.ldf, src.n, src.r, c, y, bmin, mode, s, s_new,
+                                                       bdev);
+}
+
+inline int rowdot_blocks(const kevd_ctx* ctx, int64_t n)
+{
+    return (int)std::min<int64_t>((n + 15) / 16, 8 * (int64_t)ctx->num_cu);
+}
+
 // y = F c; returns min(y) (and max dev in Sinkhorn mode) through host_mm.
 template <typename T>
-kevd_status rowdot(kevd_ctx* ctx, const double* c, double* y, int mode, const double* s,
+kevd_status rowdot_rows(kevd_ctx* ctx, const Rows<T>& src, const double* c, double* y, int mode, const double* s,
                    double* s_new, double host_mm[2])
 {
-    int64_t n = ctx->n, r = ctx->rank;
-    int nb = (int)std::min<int64_t>((n + 15) / 16, 8 * (int64_t)ctx->num_cu);
+    int nb = rowdot_blocks(ctx, src.n);
     double *bmin, *bdev, *mm;
     KEVD_TRY(ctx_buf_t(ctx, "e_bmin", nb, &bmin));
     KEVD_TRY(ctx_buf_t(ctx, "e_bdev", nb, &bdev));
     KEVD_TRY(ctx_buf_t(ctx, "e_mm", 2, &mm));
     {
         ProfScope ps(ctx, TC_MATVEC);
-        if (r <= ROWDOT_LDS_MAX)
-            k_rowdot<T, true><<<nb, 256, r * sizeof(double), ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, c,
-                                                                           y, bmin, mode, s, s_new, bdev);
-        else
-            k_rowdot<T, false><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, c, y, bmin,
-                                                           mode, s, s_new, bdev);
+        rowdot_launch<T>(ctx, src, c, y, mode, s, s_new, bmin, bdev, nb);
         k_block_minmax<<<1, 256, 0, ctx->stream>>>(bmin, mode == 1 ? bdev : nullptr, nb, mm);
     }
     KEVD_HIP(ctx, hipGetLastError());
@@ -1081,10 +1153,10 @@ kevd_status rowdot(kevd_ctx* ctx, const double* c, double* y, int mode, const do
 }
 
 template <typename T, int NG>
-kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, double* G2)
+kevd_status gram_rows(kevd_ctx* ctx, const Rows<T>& src, const double* w1, const double* w2, double* G1, double* G2)
 {
     typedef GramCfg<T, NG> C;
-    int64_t n = ctx->n, r = ctx->rank;
+    int64_t n = src.n, r = src.r;
     int ti_n = (int)((r + C::BI - 1) / C::BI), tj_n = (int)((r + C::BJ - 1) / C::BJ);
     std::vector<int> TI, TJ;
     for (int I = 0; I < ti_n; ++I)
@@ -1099,7 +1171,7 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
     const int64_t slots = C::WG_PER_CU * (int64_t)ctx->num_cu;
     // A split's rows are addressed by 32-bit buffer offsets: at most 2^31
     // bytes of F per split.
-    const int64_t rp_max = ((int64_t)INT32_MAX / (ctx->ldf * (int64_t)sizeof(T))) / C::KC * C::KC - C::KC;
+    const int64_t rp_max = ((int64_t)INT32_MAX / (src.ldf * (int64_t)sizeof(T))) / C::KC * C::KC - C::KC;
     if (rp_max < C::KC) return kevd_fail(ctx, KEVD_EINVAL, "Gram: factor rows too long for 32-bit offsets");
     int ns0 = (int)std::max<int64_t>({(int64_t)1, (2048 + ntiles - 1) / ntiles, (n + rp_max - 1) / rp_max});
     int nsplit = 1;
@@ -1135,8 +1207,8 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
     KEVD_HIP(ctx, hipMemcpyAsync(dTJ, TJ.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     {
         ProfScope ps(ctx, TC_GRAM);
-        k_gram<T, NG><<<nitems, C::NT, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, wt1, wt2, dTI, dTJ,
-                                                      ntiles, nitems, rps, part);
+        k_gram<T, NG><<<nitems, C::NT, 0, ctx->stream>>>(src.F, src.ldf, n, r, wt1, wt2, dTI, dTJ, ntiles, nitems,
+                                                      rps, part);
     }
     k_gram_reduce<C::BI, C::BJ, NG><<<dim3(ntiles, GRAM_REDUCE_SLICES), 256, 0, ctx->stream>>>(
         part, nsplit, ntiles, dTI, dTJ, r, G1, G2);
@@ -1145,26 +1217,135 @@ kevd_status gram(kevd_ctx* ctx, const double* w1, const double* w2, double* G1, 
     return KEVD_OK;
 }
 
+// Y (r x k column-major FP64, scaled per column by colscale) and an
+// optional extra column, packed as the B operand of the U product.
 template <typename T>
-kevd_status ugemm(kevd_ctx* ctx, const double* Y, int k, const double* colscale, const double* extra,
-                  const double* rowscale, float* U, double* P1)
+kevd_status pack_y(kevd_ctx* ctx, int64_t r, const double* Y, int k, const double* colscale, const double* extra,
+                   T** Yp_out, int64_t* ldy_out)
 {
-    int64_t n = ctx->n, r = ctx->rank;
     int kc = k + (extra ? 1 : 0);
     int64_t ldy = KEVD_ROUND_UP(r, 64);
     T* Yp;
     KEVD_TRY(ctx_buf_t(ctx, "e_Ypack", (size_t)kc * ldy, &Yp));
-    k_pack_y<T><<<dim3((unsigned)((r + 255) / 256), kc), 256, 0, ctx->stream>>>(Y, r, k, colscale, extra,
-                                                                                Yp, ldy);
-    dim3 grid((unsigned)((n + Cfg<T>::BM - 1) / Cfg<T>::BM), (unsigned)((kc + Cfg<T>::BN - 1) / Cfg<T>::BN));
+    k_pack_y<T><<<dim3((unsigned)((r + 255) / 256), kc), 256, 0, ctx->stream>>>(Y, r, k, colscale, extra, Yp,
+                                                                                ldy);
+    KEVD_HIP(ctx, hipGetLastError());
+    *Yp_out = Yp;
+    *ldy_out = ldy;
+    return KEVD_OK;
+}
+
+// U = diag(rowscale) src Yp (and P1 from the extra column) for src's rows;
+// U and P1 point at src's first row.
+template <typename T>
+kevd_status ugemm_rows(kevd_ctx* ctx, const Rows<T>& src, const T* Yp, int64_t ldy, int kc, int k,
+                       const double* rowscale, float* U, double* P1)
+{
+    dim3 grid((unsigned)((src.n + Cfg<T>::BM - 1) / Cfg<T>::BM), (unsigned)((kc + Cfg<T>::BN - 1) / Cfg<T>::BN));
     {
         ProfScope ps(ctx, TC_UGEMM);
-        k_ugemm<T><<<grid, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, Yp, ldy, kc, k, rowscale,
-                                                  U, P1);
+        k_ugemm<T><<<grid, 256, 0, ctx->stream>>>(src.F, src.ldf, src.n, src.r, Yp, ldy, kc, k, rowscale, U, P1);
     }
     KEVD_HIP(ctx, hipGetLastError());
     return KEVD_OK;
 }
+
+// One fused Sinkhorn pass over src (see k_sinkhorn_rows / k_sinkhorn_pass):
+// q = the next iterate's factor per row, the per-block partials of
+// src^T q into part (row b of part for block b of src) and the per-block
+// statistics into bstat.  Returns false when the rank is beyond the fused
+// kernels' register tiling.
+template <typename T>
+bool sk_fused_ok(int64_t r)
+{
+    constexpr int V = 16 / (int)sizeof(T), VPT_MAX = sizeof(T) == 4 ? 4 : 8;
+    int vpt = (int)(((r + V - 1) / V + 255) / 256);
+    return vpt <= VPT_MAX;
+}
+
+template <typename T>
+void sk_pass_launch(kevd_ctx* ctx, const Rows<T>& src, const double* u, const double* s, int damped, double* q,
+                    double* part, int64_t rpad, double* bstat)
+{
+    constexpr int V = 16 / (int)sizeof(T), VPT_MAX = sizeof(T) == 4 ? 4 : 8;
+    const int64_t r = src.r;
+    const int nb = (int)((src.n + SK_ROWS - 1) / SK_ROWS);
+    const int vpt = (int)(((r + V - 1) / V + 255) / 256);
+#define KEVD_SK_ROWS(VQ)                                                                                   \
+    k_sinkhorn_rows<T, VQ><<<nb, 256, 0, ctx->stream>>>(src.F, src.ldf, src.n, r, u, s, damped, q, part, rpad, \
+                                                       bstat)
+#define KEVD_SK_PASS(VP)                                                                                   \
+    k_sinkhorn_pass<T, VP><<<nb, 256, 0, ctx->stream>>>(src.F, src.ldf, src.n, r, u, s, damped, q, part, rpad, \
+                                                       bstat)
+    const int vpq = (int)(((r + V - 1) / V + 63) / 64);
+    if constexpr (sizeof(T) == 4) {
+        if (vpq <= 8) {
+            if (vpq <= 2)
+                KEVD_SK_ROWS(2);
+            else if (vpq <= 4)
+                KEVD_SK_ROWS(4);
+            else
+                KEVD_SK_ROWS(8);
+            return;
+        }
+    }
+    if (vpt <= 1)
+        KEVD_SK_PASS(1);
+    else if (vpt <= 2)
+        KEVD_SK_PASS(2);
+    else if (vpt <= 4)
+        KEVD_SK_PASS(4);
+    else
+        KEVD_SK_PASS(VPT_MAX);
+#undef KEVD_SK_PASS
+#undef KEVD_SK_ROWS
+}
+
+// The passes of a kEVD over the resident factor F (n x rank).
+template <typename T> struct FactorOps {
+    kevd_ctx* ctx;
+    Rows<T> src;
+    explicit FactorOps(kevd_ctx* c) : ctx(c), src{(const T*)c->F, c->ldf, c->n, c->rank} {}
+    kevd_status begin() { return KEVD_OK; }
+    // c = F^T w (w null: ones)
+    kevd_status colsum(const double* w, double* c) { return colsum_rows<T>(ctx, src, w, c); }
+    // y = F c (Sinkhorn mode 1 as k_rowdot); min(y), max dev to host
+    kevd_status rowdot(const double* c, double* y, int mode, const double* s, double* s_new, double mm[2])
+    {
+        return rowdot_rows<T>(ctx, src, c, y, mode, s, s_new, mm);
+    }
+    kevd_status gram1(const double* w, double* G) { return gram_rows<T, 1>(ctx, src, w, nullptr, G, nullptr); }
+    kevd_status gram2(const double* w1, const double* w2, double* A, double* B)
+    {
+        return gram_rows<T, 2>(ctx, src, w1, w2, A, B);
+    }
+    // U = diag(rowscale) F Y diag(colscale) (and P1 = diag(rowscale) F extra)
+    kevd_status ugemm(const double* Y, int k, const double* colscale, const double* extra, const double* rowscale,
+                      float* U, double* P1)
+    {
+        T* Yp;
+        int64_t ldy;
+        KEVD_TRY(pack_y<T>(ctx, src.r, Y, k, colscale, extra, &Yp, &ldy));
+        return ugemm_rows<T>(ctx, src, Yp, ldy, k + (extra ? 1 : 0), k, rowscale, U, P1);
+    }
+    // one fused Sinkhorn pass: q, v = F^T q and the five statistics (st3)
+    kevd_status skpass(const double* u, const double* s, int damped, double* q, double* v, double* st3)
+    {
+        const int nb = (int)((src.n + SK_ROWS - 1) / SK_ROWS);
+        const int64_t rpad = KEVD_ROUND_UP(src.r, 4);
+        double *part, *bstat;
+        KEVD_TRY(ctx_buf_t(ctx, "e_skpart", (size_t)nb * rpad, &part));
+        KEVD_TRY(ctx_buf_t(ctx, "e_skstat", (size_t)SK_NSTAT * nb, &bstat));
+        {
+            ProfScope ps(ctx, TC_MATVEC);
+            sk_pass_launch<T>(ctx, src, u, s, damped, q, part, rpad, bstat);
+            k_colsum_reduce<<<(unsigned)((src.r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, src.r, rpad, v);
+            k_stat_reduce<<<1, 256, 0, ctx->stream>>>(bstat, nb, st3);
+        }
+        KEVD_HIP(ctx, hipGetLastError());
+        return KEVD_OK;
+    }
+};
 
 #define BLAS(call)                                                                  \
     do {                                                                            \
@@ -1173,6 +1354,208 @@ kevd_status ugemm(kevd_ctx* ctx, const double* Y, int k, const double* colscale,
             return kevd_fail(ctx, KEVD_EHIP, std::string(#call) + " failed: " +     \
                                                  rocblas_status_to_string(b_));     \
     } while (0)
+
+// The passes of a kEVD with no factor held: the reference's LM form
+// (ks_kevd.cu regenerates K(R,S) row panels in every pass, 755-800,
+// 1091-1168, 1627-1690, 2229-2275).  With the pivots S and L^-1
+// (K(S,S) = L L^T) kept by the LM form of ARPC, F = K(:,S) L^-T is never
+// formed: every pass regenerates K(R,S) for PANEL_ROWS rows at a time
+// (k_kpanel) and runs the factor-form kernels on the panel, and the
+// r-vectors and r x r results move between the two bases with L^-1:
+//   F^T w = L^-1 (K^T w),  F c = K (L^-T c),  F^T W F = L^-1 (K^T W K) L^-T,
+//   F Y = K (L^-T Y).
+// L^-1 is row-major lower, so read column-major it is Lt = L^-T (upper).
+// Device memory beyond the points: one panel (PANEL_ROWS x rank) and the
+// per-block partials, instead of the n x rank factor.
+constexpr int64_t PANEL_ROWS = 65536;  // a multiple of every kernel's row block
+
+template <typename T> struct PanelOps {
+    kevd_ctx* ctx;
+    int64_t n, r, ldo = 0, P = 0;
+    const double* Lt = nullptr;  // L^-T, column-major upper, leading dimension ldl
+    int64_t ldl = 0;
+    T *panel = nullptr, *XS = nullptr;
+    double *nrmS = nullptr, *tv = nullptr, *tY = nullptr, *tG = nullptr;
+    int64_t* piv = nullptr;
+    GaussScale gs{};
+    explicit PanelOps(kevd_ctx* c) : ctx(c), n(c->n), r(c->rank) {}
+
+    kevd_status begin()
+    {
+        if (!ctx->lm_linv || (int64_t)ctx->pivots.size() != r)
+            return kevd_fail(ctx, KEVD_EINVAL, "evd: no pivots and L^-1 to regenerate the factor from");
+        ldo = KEVD_ROUND_UP(r, 64);
+        P = KEVD_ROUND_UP(std::min<int64_t>(n, PANEL_ROWS), 512);
+        Lt = ctx->lm_linv;
+        ldl = ctx->lm_ldl;
+        KEVD_TRY(ctx_buf_t(ctx, "p_panel", (size_t)P * ldo, &panel));
+        KEVD_TRY(ctx_buf_t(ctx, "p_XS", (size_t)r * ctx->dp, &XS));
+        KEVD_TRY(ctx_buf_t(ctx, "p_nrmS", r, &nrmS));
+        KEVD_TRY(ctx_buf_t(ctx, "p_piv", r, &piv));
+        KEVD_TRY(ctx_buf_t(ctx, "p_tv", r, &tv));
+        KEVD_HIP(ctx, hipMemcpyAsync(piv, ctx->pivots.data(), r * sizeof(int64_t), hipMemcpyHostToDevice,
+                                     ctx->stream));
+        KEVD_TRY(kblock_gather(ctx, piv, r, sizeof(T) == 4 ? KEVD_MIXED : KEVD_FP64, XS, nrmS));
+        gs = GaussScale{-1.0 / ctx->factor_eps, -1.4426950408889634074 / ctx->factor_eps};
+        return KEVD_OK;
+    }
+    int64_t panels() const { return (n + P - 1) / P; }
+    Rows<T> gen(int64_t p0)
+    {
+        const int64_t np = std::min(P, n - p0);
+        const T* X = sizeof(T) == 4 ? (const T*)ctx->X32 : (const T*)ctx->X64;
+        const double* nrm = sizeof(T) == 4 ? ctx->nrm32 : ctx->nrm64;
+        dim3 grid((unsigned)((np + Cfg<T>::BM - 1) / Cfg<T>::BM), (unsigned)((ldo + Cfg<T>::BN - 1) / Cfg<T>::BN));
+        k_kpanel<T><<<grid, 256, 0, ctx->stream>>>(X, nrm, ctx->dp, p0, np, XS, nrmS, piv, r, gs, panel, ldo);
+        return Rows<T>{panel, ldo, np, r};
+    }
+    // x := L^-1 x (to the factor's basis) or x := L^-T x (to the kernel's)
+    kevd_status to_f(double* x)
+    {
+        rocblas_handle h = blas_handle(ctx);
+        BLAS(rocblas_dtrmv(h, rocblas_fill_upper, rocblas_operation_transpose, rocblas_diagonal_non_unit,
+                           (rocblas_int)r, Lt, (rocblas_int)ldl, x, 1));
+        return KEVD_OK;
+    }
+    kevd_status to_k(double* x)
+    {
+        rocblas_handle h = blas_handle(ctx);
+        BLAS(rocblas_dtrmv(h, rocblas_fill_upper, rocblas_operation_none, rocblas_diagonal_non_unit,
+                           (rocblas_int)r, Lt, (rocblas_int)ldl, x, 1));
+        return KEVD_OK;
+    }
+    kevd_status copy_to_k(const double* x, double* out)
+    {
+        KEVD_HIP(ctx, hipMemcpyAsync(out, x, r * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        return to_k(out);
+    }
+
+    kevd_status colsum(const double* w, double* c)
+    {
+        const int nbt = (int)((n + COLSUM_ROWS - 1) / COLSUM_ROWS);
+        const int64_t rpad = KEVD_ROUND_UP(r, 4);
+        double* part;
+        KEVD_TRY(ctx_buf_t(ctx, "p_cpart", (size_t)nbt * rpad, &part));
+        {
+            ProfScope ps(ctx, TC_MATVEC);
+            for (int64_t p0 = 0; p0 < n; p0 += P)
+                colsum_part<T>(ctx, gen(p0), w ? w + p0 : nullptr, part + (p0 / COLSUM_ROWS) * rpad, rpad);
+            k_colsum_reduce<<<(unsigned)((r + 63) / 64), 1024, 0, ctx->stream>>>(part, nbt, r, rpad, c);
+        }
+        KEVD_HIP(ctx, hipGetLastError());
+        return to_f(c);
+    }
+    kevd_status rowdot(const double* c, double* y, int mode, const double* s, double* s_new, double mm[2])
+    {
+        KEVD_TRY(copy_to_k(c, tv));
+        const int nbmax = rowdot_blocks(ctx, P);
+        double *bmin, *bdev, *mmd;
+        KEVD_TRY(ctx_buf_t(ctx, "p_bmin", (size_t)panels() * nbmax, &bmin));
+        KEVD_TRY(ctx_buf_t(ctx, "p_bdev", (size_t)panels() * nbmax, &bdev));
+        KEVD_TRY(ctx_buf_t(ctx, "e_mm", 2, &mmd));
+        int off = 0;
+        {
+            ProfScope ps(ctx, TC_MATVEC);
+            for (int64_t p0 = 0; p0 < n; p0 += P) {
+                Rows<T> src = gen(p0);
+                const int nb = rowdot_blocks(ctx, src.n);
+                rowdot_launch<T>(ctx, src, tv, y + p0, mode, s ? s + p0 : nullptr, s_new ? s_new + p0 : nullptr,
+                                 bmin + off, bdev + off, nb);
+                off += nb;
+            }
+            k_block_minmax<<<1, 256, 0, ctx->stream>>>(bmin, mode == 1 ? bdev : nullptr, off, mmd);
+        }
+        KEVD_HIP(ctx, hipGetLastError());
+        KEVD_HIP(ctx, hipMemcpyAsync(mm, mmd, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return KEVD_OK;
+    }
+    // G = L^-1 (K^T diag(w) K) L^-T, the Gram accumulated over the panels
+    template <int NG>
+    kevd_status gram(const double* w1, const double* w2, double* G1, double* G2)
+    {
+        const size_t rr = (size_t)r * r;
+        double *acc1, *acc2 = nullptr, *t1, *t2 = nullptr;
+        KEVD_TRY(ctx_buf_t(ctx, "p_gacc1", rr, &acc1));
+        KEVD_TRY(ctx_buf_t(ctx, "p_gt1", rr, &t1));
+        if (NG > 1) {
+            KEVD_TRY(ctx_buf_t(ctx, "p_gacc2", rr, &acc2));
+            KEVD_TRY(ctx_buf_t(ctx, "p_gt2", rr, &t2));
+        }
+        const unsigned gr = (unsigned)((rr + 255) / 256);
+        for (int64_t p0 = 0; p0 < n; p0 += P) {
+            Rows<T> src = gen(p0);
+            double* o1 = p0 == 0 ? acc1 : t1;
+            double* o2 = p0 == 0 ? acc2 : t2;
+            KEVD_TRY((gram_rows<T, NG>(ctx, src, w1 + p0, NG > 1 ? w2 + p0 : nullptr, o1, o2)));
+            if (p0 > 0) {
+                k_add<<<gr, 256, 0, ctx->stream>>>(acc1, t1, (int64_t)rr);
+                if (NG > 1) k_add<<<gr, 256, 0, ctx->stream>>>(acc2, t2, (int64_t)rr);
+            }
+        }
+        KEVD_HIP(ctx, hipGetLastError());
+        KEVD_TRY(sandwich(acc1, t1, G1));
+        if (NG > 1) KEVD_TRY(sandwich(acc2, t2, G2));
+        return KEVD_OK;
+    }
+    // G = L^-1 Ak L^-T = Lt^T Ak Lt (tmp: r x r scratch)
+    kevd_status sandwich(const double* Ak, double* tmp, double* G)
+    {
+        rocblas_handle h = blas_handle(ctx);
+        const double one = 1.0;
+        const rocblas_int R = (rocblas_int)r, LD = (rocblas_int)ldl;
+        ProfScope ps(ctx, TC_DENSE);
+        BLAS(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                           rocblas_diagonal_non_unit, R, R, &one, Lt, LD, Ak, R, tmp, R));
+        BLAS(rocblas_dtrmm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                           rocblas_diagonal_non_unit, R, R, &one, Lt, LD, tmp, R, G, R));
+        return KEVD_OK;
+    }
+    kevd_status gram1(const double* w, double* G) { return gram<1>(w, nullptr, G, nullptr); }
+    kevd_status gram2(const double* w1, const double* w2, double* A, double* B) { return gram<2>(w1, w2, A, B); }
+    kevd_status ugemm(const double* Y, int k, const double* colscale, const double* extra, const double* rowscale,
+                      float* U, double* P1)
+    {
+        rocblas_handle h = blas_handle(ctx);
+        const double one = 1.0;
+        double *Yk, *ek = nullptr;
+        KEVD_TRY(ctx_buf_t(ctx, "p_Yk", (size_t)r * k, &Yk));
+        BLAS(rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                           rocblas_diagonal_non_unit, (rocblas_int)r, (rocblas_int)k, &one, Lt, (rocblas_int)ldl, Y,
+                           (rocblas_int)r, Yk, (rocblas_int)r));  // Yk = L^-T Y
+        if (extra) {
+            KEVD_TRY(ctx_buf_t(ctx, "p_ek", r, &ek));
+            KEVD_TRY(copy_to_k(extra, ek));
+        }
+        T* Yp;
+        int64_t ldy;
+        KEVD_TRY(pack_y<T>(ctx, r, Yk, k, colscale, ek, &Yp, &ldy));
+        const int kc = k + (extra ? 1 : 0);
+        for (int64_t p0 = 0; p0 < n; p0 += P)
+            KEVD_TRY(ugemm_rows<T>(ctx, gen(p0), Yp, ldy, kc, k, rowscale + p0, U + p0 * k,
+                                   P1 ? P1 + p0 : nullptr));
+        return KEVD_OK;
+    }
+    kevd_status skpass(const double* u, const double* s, int damped, double* q, double* v, double* st3)
+    {
+        KEVD_TRY(copy_to_k(u, tv));
+        const int nbt = (int)((n + SK_ROWS - 1) / SK_ROWS);
+        const int64_t rpad = KEVD_ROUND_UP(r, 4);
+        double *part, *bstat;
+        KEVD_TRY(ctx_buf_t(ctx, "p_skpart", (size_t)nbt * rpad, &part));
+        KEVD_TRY(ctx_buf_t(ctx, "p_skstat", (size_t)SK_NSTAT * nbt, &bstat));
+        {
+            ProfScope ps(ctx, TC_MATVEC);
+            for (int64_t p0 = 0; p0 < n; p0 += P)
+                sk_pass_launch<T>(ctx, gen(p0), tv, s + p0, damped, q + p0, part + (p0 / SK_ROWS) * rpad, rpad,
+                                  bstat + SK_NSTAT * (p0 / SK_ROWS));
+            k_colsum_reduce<<<(unsigned)((r + 63) / 64), 1024, 0, ctx->stream>>>(part, nbt, r, rpad, v);
+            k_stat_reduce<<<1, 256, 0, ctx->stream>>>(bstat, nbt, st3);
+        }
+        KEVD_HIP(ctx, hipGetLastError());
+        return to_f(v);
+    }
+};
 
 // Symmetric eigensolve of the column-major r x r FP64 matrix W (lower
 // triangle used); V overwrites W, lam ascending.
@@ -1600,7 +1983,7 @@ kevd_status orth_basis(kevd_ctx* ctx, double* Y, int64_t r, int l, bool househol
         BLAS(rocsolver_dorgqr(h, R, Lb, Lb, Y, R, tau));
         return KEVD_OK;
     }
-    double one = 1.0, zero = 0.0;
+    double one = 1.0;
     const double c = 11.0 * ((double)r * l + (double)l * (l + 1)) * 0x1.0p-53;
     for (int pass = 0; pass < passes; ++pass) {
         KEVD_TRY(gram_tn(ctx, Y, Y, r, l, G));
@@ -1773,8 +2156,8 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
     return kevd_fail(ctx, KEVD_EEIG, "subspace iteration: orthonormalisation failed");
 }
 
-template <typename T>
-kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res)
+template <typename T, typename Ops>
+kevd_status evd_ref(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd_evd_result* res)
 {
     int64_t n = ctx->n, r = ctx->rank;
     int k = p->k_out;
@@ -1798,14 +2181,14 @@ kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* re
     KEVD_TRY(ctx_buf_t(ctx, "e_pinfo", 1, &info));
     unsigned gn = (unsigned)((n + 255) / 256);
     double mm[2];
-    KEVD_TRY(colsum<T>(ctx, nullptr, t));  // t = F^T 1
-    KEVD_TRY(rowdot<T>(ctx, t, dt, 0, nullptr, nullptr, mm));  // d~ = F t
+    KEVD_TRY(ops.colsum(nullptr, t));  // t = F^T 1
+    KEVD_TRY(ops.rowdot(t, dt, 0, nullptr, nullptr, mm));  // d~ = F t
     res->min_degree = mm[0];
     if (!(mm[0] > 0.0))
         return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive degree d~ (min " + std::to_string(mm[0]) + ")");
     k_weights<<<gn, 256, 0, ctx->stream>>>(dt, n, 2, w1);  // 1/d~
-    KEVD_TRY(colsum<T>(ctx, w1, cq));                      // cq = F^T d~^-1
-    KEVD_TRY(rowdot<T>(ctx, cq, qt, 0, nullptr, nullptr, mm));  // q~ = F cq
+    KEVD_TRY(ops.colsum(w1, cq));                      // cq = F^T d~^-1
+    KEVD_TRY(ops.rowdot(cq, qt, 0, nullptr, nullptr, mm));  // q~ = F cq
     if (!(mm[0] > 0.0))
         return kevd_fail(ctx, KEVD_ENONPOS_Q, "non-positive q~ (min " + std::to_string(mm[0]) + ")");
     k_weights<<<gn, 256, 0, ctx->stream>>>(dt, n, 0, w1);  // 1/d~^2
@@ -1813,7 +2196,7 @@ kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* re
     KEVD_HIP(ctx, hipGetLastError());
     {
         TraceRange tr("kEVD Grams A, B");
-        KEVD_TRY((gram<T, 2>(ctx, w1, w2, A, B)));
+        KEVD_TRY(ops.gram2(w1, w2, A, B));
     }
     // dense phase 1: A + sI = L L^T (R = L^T)
     double trsh[2];
@@ -1858,7 +2241,7 @@ kevd_status evd_ref(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* re
     }
     // U = D~^-1 F Y, and P1 = D~^-1 F (B cq)
     k_weights<<<gn, 256, 0, ctx->stream>>>(dt, n, 2, w1);
-    KEVD_TRY(ugemm<T>(ctx, Vk, k, nullptr, v2, w1, ctx->U, P1));
+    KEVD_TRY(ops.ugemm(Vk, k, nullptr, v2, w1, ctx->U, P1));
     int nbd = 1024;
     double* bdev;
     KEVD_TRY(ctx_buf_t(ctx, "e_rsdev", nbd, &bdev));
@@ -1930,8 +2313,8 @@ struct SkCheb {
     }
 };
 
-template <typename T>
-kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res)
+template <typename T, typename Ops>
+kevd_status evd_sinkhorn(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd_evd_result* res)
 {
     int64_t n = ctx->n, r = ctx->rank;
     int k = p->k_out;
@@ -1950,7 +2333,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     // start from s = 1 / sqrt(mean(K^ 1)) so that s K^ s is O(1); with
     // u = F^T 1, mean(K^ 1) = 1^T F F^T 1 / n = |u|^2 / n.
     double mm[2];
-    KEVD_TRY(colsum<T>(ctx, nullptr, u));
+    KEVD_TRY(ops.colsum(nullptr, u));
     std::vector<double> hu((size_t)r);
     KEVD_HIP(ctx, hipMemcpyAsync(hu.data(), u, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1962,9 +2345,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     double dev = INFINITY, smin_fused = NAN;
     int maxit = p->sinkhorn_max_iter > 0 ? p->sinkhorn_max_iter : 1000;
     double tol = p->sinkhorn_tol > 0.0 ? p->sinkhorn_tol : 1e-12;
-    constexpr int V = 16 / (int)sizeof(T), VPT_MAX = sizeof(T) == 4 ? 4 : 8;
-    int vpt = (int)(((r + V - 1) / V + 255) / 256);
-    if (vpt <= VPT_MAX) {
+    if (sk_fused_ok<T>(r)) {
         // One fused pass per iteration.  Near the fixed point s* the error
         // e in s = s* (1 + e) maps under s <- 1 / (K^ s) to -P e, with
         // P = S* K^ S* bistochastic and positive semidefinite (K^ = F F^T).
@@ -1976,11 +2357,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
         // on s and, by linearity, on u = F^T s).  Three rises of the
         // deviation in a row switch to the damped update s <- sqrt(s / K^ s)
         // (rate <= 1/2).
-        int nb = (int)((n + SK_ROWS - 1) / SK_ROWS);
-        int64_t rpad = KEVD_ROUND_UP(r, 4);
-        double *part, *bstat, *st3;
-        KEVD_TRY(ctx_buf_t(ctx, "e_skpart", (size_t)nb * rpad, &part));
-        KEVD_TRY(ctx_buf_t(ctx, "e_skstat", (size_t)SK_NSTAT * nb, &bstat));
+        double* st3;
         KEVD_TRY(ctx_buf_t(ctx, "e_skst3", SK_NSTAT, &st3));
         // iterate s_j in Sb[j % 3] (built by k_lincomb3 before its pass),
         // q_j in q; u_j = F^T s_j in ucur, u_{j-1} in uprev, v_j = F^T q_j
@@ -2004,41 +2381,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
         double prev = INFINITY, st[SK_NSTAT];
         int j = 0;
         for (it = 1; it <= maxit; ++it, ++j) {
-            {
-                ProfScope ps(ctx, TC_MATVEC);
-#define KEVD_SK_ROWS(VQ)                                                                                     \
-    k_sinkhorn_rows<T, VQ><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, ucur, Sb[j % 3],  \
-                                                       damped, q, part, rpad, bstat)
-#define KEVD_SK_PASS(VP)                                                                                     \
-    k_sinkhorn_pass<T, VP><<<nb, 256, 0, ctx->stream>>>((const T*)ctx->F, ctx->ldf, n, r, ucur, Sb[j % 3],  \
-                                                       damped, q, part, rpad, bstat)
-                const int vpq = (int)(((r + V - 1) / V + 63) / 64);
-                bool rows_done = false;
-                if constexpr (sizeof(T) == 4) {
-                    if (vpq <= 8) {
-                        if (vpq <= 2)
-                            KEVD_SK_ROWS(2);
-                        else if (vpq <= 4)
-                            KEVD_SK_ROWS(4);
-                        else
-                            KEVD_SK_ROWS(8);
-                        rows_done = true;
-                    }
-                }
-                if (rows_done) {
-                } else if (vpt <= 1)
-                    KEVD_SK_PASS(1);
-                else if (vpt <= 2)
-                    KEVD_SK_PASS(2);
-                else if (vpt <= 4)
-                    KEVD_SK_PASS(4);
-                else
-                    KEVD_SK_PASS(VPT_MAX);
-#undef KEVD_SK_PASS
-#undef KEVD_SK_ROWS
-                k_colsum_reduce<<<(unsigned)((r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, r, rpad, uv);
-                k_stat_reduce<<<1, 256, 0, ctx->stream>>>(bstat, nb, st3);
-            }
+            KEVD_TRY(ops.skpass(ucur, Sb[j % 3], damped, q, uv, st3));
             KEVD_HIP(ctx, hipGetLastError());
             KEVD_HIP(ctx, hipMemcpyAsync(st, st3, SK_NSTAT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
             KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -2073,8 +2416,8 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
         // rank beyond the register tiling: two passes per damped iteration
         k_fill_d<<<gn, 256, 0, ctx->stream>>>(s, n, 1.0 / std::sqrt(mean));
         for (it = 1; it <= maxit; ++it) {
-            KEVD_TRY(colsum<T>(ctx, s, u));
-            KEVD_TRY(rowdot<T>(ctx, u, Ks, 1, s, s2, mm));
+            KEVD_TRY(ops.colsum(s, u));
+            KEVD_TRY(ops.rowdot(u, Ks, 1, s, s2, mm));
             if (!(mm[0] > 0.0)) {
                 if (it == 1) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
                 return kevd_fail(ctx, KEVD_ENONPOS_Q, "Sinkhorn: non-positive K^ s");
@@ -2092,7 +2435,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     KEVD_HIP(ctx, hipGetLastError());
     {
         TraceRange tr("kEVD Gram F^T S^2 F");
-        KEVD_TRY((gram<T, 1>(ctx, w1, nullptr, G, nullptr)));
+        KEVD_TRY(ops.gram1(w1, G));
     }
     double* Vk = nullptr;
     int eit = 0;
@@ -2101,7 +2444,7 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
         KEVD_TRY(eig_leading(ctx, G, r, k, p->all_eigenvalues != 0, lam, &Vk, &eit));
     }
     k_inv_sqrt_tail<<<(k + 255) / 256, 256, 0, ctx->stream>>>(lam, r, k, isq);
-    KEVD_TRY(ugemm<T>(ctx, Vk, k, isq, nullptr, s, ctx->U, nullptr));
+    KEVD_TRY(ops.ugemm(Vk, k, isq, nullptr, s, ctx->U, nullptr));
     KEVD_HIP(ctx, hipMemcpyAsync(res->lam, lam, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (std::isnan(smin_fused)) {
         // two-pass path: min(s) by a block-partial reduction
@@ -2122,20 +2465,50 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_resul
     return KEVD_OK;
 }
 
+template <typename T, typename Ops>
+kevd_status run_norm(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd_evd_result* res)
+{
+    KEVD_TRY(ops.begin());
+    return p->norm == KEVD_NORM_REF ? evd_ref<T>(ctx, ops, p, res) : evd_sinkhorn<T>(ctx, ops, p, res);
+}
+
+template <typename T>
+kevd_status run_evd(kevd_ctx* ctx, bool panels, const kevd_evd_params* p, kevd_evd_result* res)
+{
+    if constexpr (sizeof(T) == 8) {
+        if (panels) {
+            PanelOps<T> ops(ctx);
+            return run_norm<T>(ctx, ops, p, res);
+        }
+    }
+    FactorOps<T> ops(ctx);
+    return run_norm<T>(ctx, ops, p, res);
+}
+
 }  // namespace
 
 extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res)
 {
     TraceRange trace_range("kevd_evd");
     if (!ctx || !p || !res) return kevd_fail(ctx, KEVD_EINVAL, "evd: null argument");
-    if (!ctx->have_factor || ctx->rank < 1) return kevd_fail(ctx, KEVD_EINVAL, "evd: no factor");
+    // no factor but the LM form's pivots and L^-1: regenerate K(R,S) panels
+    const bool panels = !ctx->have_factor && ctx->lm && ctx->lm_linv;
+    if (!(ctx->have_factor || panels) || ctx->rank < 1) return kevd_fail(ctx, KEVD_EINVAL, "evd: no factor");
     if (p->k_out < 1 || p->k_out > ctx->rank || p->k_out > 1024)
         return kevd_fail(ctx, KEVD_EINVAL, "evd: need 1 <= k_out <= min(rank, 1024)");
     if (p->norm != KEVD_NORM_REF && p->norm != KEVD_NORM_SINKHORN)
         return kevd_fail(ctx, KEVD_EINVAL, "evd: unknown normalisation");
     if (!res->lam) return kevd_fail(ctx, KEVD_EINVAL, "evd: lam output missing");
-    if (p->prec != ctx->fprec)
+    if (p->prec != KEVD_FP64 && p->prec != KEVD_MIXED) return kevd_fail(ctx, KEVD_EINVAL, "evd: precision");
+    if (!panels && p->prec != ctx->fprec)
         return kevd_fail(ctx, KEVD_EINVAL, "evd: precision must match the factor's precision");
+    // Without a factor every pass works in the kernel's basis, F = K(:,S) L^-T:
+    // the rounding of K and of its Gram is carried through L^-1, whose norm
+    // reaches tol^-1/2, so FP32 kernel values would cost up to tol^-1 in the
+    // r x r matrices (measured: top eigenvalues off by O(1)).  FP64 only.
+    if (panels && p->prec != KEVD_FP64)
+        return kevd_fail(ctx, KEVD_EINVAL,
+                         "evd: the kEVD without a factor (after the LM form of ARPC) runs in FP64 only");
     if (ctx->rank > 46340) return kevd_fail(ctx, KEVD_EINVAL, "evd: rank too large for the r x r phase");
     KEVD_HIP(ctx, hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();
@@ -2160,11 +2533,8 @@ extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_ev
     }
     ctx->k_out = p->k_out;
     res->sinkhorn_iters = 0;
-    kevd_status s;
-    if (p->norm == KEVD_NORM_REF)
-        s = ctx->fprec == KEVD_MIXED ? evd_ref<float>(ctx, p, res) : evd_ref<double>(ctx, p, res);
-    else
-        s = ctx->fprec == KEVD_MIXED ? evd_sinkhorn<float>(ctx, p, res) : evd_sinkhorn<double>(ctx, p, res);
+    const bool f32 = !panels && ctx->fprec == KEVD_MIXED;
+    kevd_status s = f32 ? run_evd<float>(ctx, panels, p, res) : run_evd<double>(ctx, panels, p, res);
     if (s != KEVD_OK) {
         (void)hipStreamSynchronize(ctx->stream);
         prof_reset(ctx);

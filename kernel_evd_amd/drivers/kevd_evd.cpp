@@ -6,7 +6,7 @@
 //
 //   kevd-evd --points X.npy|u.h5 [--delay Nq] [--grid-points M]
 //            (--factor F.npy | --pivots S.npy|spiv.h5 [--linv Linv.npy|linv.h5]
-//             (--eps E | --eps-median))
+//             (--eps E | --eps-median) [--regenerate])
 //            [--kout 100] [--norm ref|sinkhorn] [--sinkhorn-tol 1e-12]
 //            [--precision mixed|fp64] [--all-eigenvalues] [--lam-out lam.npy]
 //            [--U-out U.npy] [--out evd.h5 [--leading-only]] [--device 0]
@@ -17,8 +17,10 @@
 // implies --all-eigenvalues unless --leading-only.  --linv reads the
 // reference's /linv and reports how far the rebuilt L^-1 is from it.
 // With --pivots, F = K(:,S) L^-T is rebuilt by the ARPC kernels with the
-// pivot list injected (K(S,S) = L L^T), which is what the reference's kEVD
-// regenerates panel by panel from /spiv and /linv.
+// pivot list injected (K(S,S) = L L^T).  With --pivots --regenerate no
+// factor is stored at all: the LM form of ARPC keeps S and L^-1 and the
+// kEVD regenerates K(R,S) panel by panel in every pass, as the reference's
+// kEVD does from /spiv and /linv (FP64 only).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -40,7 +42,8 @@ int main(int argc, char** argv)
     Args a(argc, argv);
     if (a.has("help")) {
         std::printf("kevd-evd --points X.npy|u.h5 [--delay Nq] [--grid-points M]\n"
-                    "         (--factor F.npy | --pivots S.npy|spiv.h5 [--linv linv.h5] (--eps E | --eps-median))\n"
+                    "         (--factor F.npy | --pivots S.npy|spiv.h5 [--linv linv.h5] (--eps E | --eps-median)\n"
+                    "          [--regenerate])\n"
                     "         [--kout 100] [--norm ref|sinkhorn] [--sinkhorn-tol 1e-12]\n"
                     "         [--precision mixed|fp64] [--all-eigenvalues] [--lam-out lam.npy] [--U-out U.npy]\n"
                     "         [--out evd.h5 [--leading-only]] [--device 0]\n");
@@ -95,6 +98,9 @@ int main(int argc, char** argv)
             p.prec = prec;
             p.inject_pivots = S.data();
             p.n_inject = (int64_t)S.size();
+            // --regenerate: keep only S and L^-1 (the LM form) and let the
+            // kEVD regenerate K(R,S) panels instead of storing F (FP64)
+            p.regenerate = a.has("regenerate") ? 1 : 0;
             int32_t mr = (int32_t)(4 * ((p.rank + 127) / 128) + 64);
             std::vector<int64_t> piv(S.size());
             std::vector<int32_t> acc((size_t)mr);

@@ -78,6 +78,11 @@ def test_pipeline_end_to_end(tmp_path):
     run("kevd-evd", "--points", X, "--pivots", S, "--eps", repr(eps), "--kout", 16,
         "--precision", "fp64", "--lam-out", lam2_f)
     assert np.max(np.abs(np.load(lam2_f)[-16:] - lam[-16:])) < 1e-8
+    # and with no factor stored at all (K(R,S) panels regenerated per pass)
+    lam3_f = tmp_path / "lam3.npy"
+    run("kevd-evd", "--points", X, "--pivots", S, "--eps", repr(eps), "--kout", 16,
+        "--precision", "fp64", "--regenerate", "--lam-out", lam3_f)
+    assert np.max(np.abs(np.load(lam3_f)[-16:] - lam[-16:])) < 1e-8
     bw = tmp_path / "bw0.out"
     run("kevd-bandwidth", "--points", X, "--rows", 500, "--grid", "ref36", "--out", bw)
     lines = open(bw).read().strip().split("\n")
