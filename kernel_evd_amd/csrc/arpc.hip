@@ -47,12 +47,19 @@ __global__ __launch_bounds__(256) void k_weights_scan(const double* __restrict__
     weights_scan_body(d, n, pmax, nb, shift_base, prefix, btot, st, blockIdx.x);
 }
 
+// A run that has used its round budget stops here (stop reason 8), so a
+// replayed block of rounds may run past the budget harmlessly.
 __global__ __launch_bounds__(256) void k_draw(const double* __restrict__ psum, const double* __restrict__ pmax,
                                               const uint64_t* __restrict__ btot, int nb,
                                               const uint64_t* __restrict__ prefix, int64_t n, double trK,
                                               uint64_t seed, int b, double tol, int64_t* Sprime, ArpcState* st,
-                                              double* err, const int64_t* inject, int64_t n_inject)
+                                              double* err, const int64_t* inject, int64_t n_inject, int max_rounds)
 {
+    if (threadIdx.x == 0 && !st->done && st->round >= max_rounds) {
+        st->done = 1;
+        st->stop_reason = 8;
+    }
+    __syncthreads();
     draw_body(psum, pmax, btot, nb, prefix, n, trK, seed, b, tol, Sprime, st, err, inject, n_inject);
 }
 
@@ -215,13 +222,13 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
     for (int i = 0; i < 4; ++i) KEVD_HIP(ctx, hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
     kevd_status status = KEVD_OK;
     unsigned gdown = (unsigned)((n + Cfg<T>::BM - 1) / Cfg<T>::BM);
-    for (int rd = 0; rd < max_rounds; ++rd) {
+    auto round = [&]() {
         {
             ProfScope ps(ctx, TC_SAMPLE);
             k_reduce_d<<<nb, 256, 0, s>>>(d, n, pmax, psum, st, 0);
             k_weights_scan<<<nb, 256, 0, s>>>(d, n, pmax, nb, shift_base, prefix, btot, st);
             k_draw<<<1, 256, 0, s>>>(psum, pmax, btot, nb, prefix, n, trK, p->seed, p->block, p->tol, Sprime, st,
-                                     err_dev, inject_dev, n_inject);
+                                     err_dev, inject_dev, n_inject, max_rounds);
         }
         {
             ProfScope ps(ctx, TC_REJECT);
@@ -238,20 +245,44 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
                                                 FS, LinvT, st, gs);
         }
         k_commit<<<1, BMAX, 0, s>>>(st, Si, pivots_dev, acc_dev);
-        hipError_t le = hipGetLastError();
+    };
+    // Without per-kernel timing the rounds are captured as a hipGraph of
+    // rpg consecutive rounds and replayed: one launch (and one status copy)
+    // per rpg rounds.  Every kernel returns at once after the device state
+    // says done, and the draw enforces the round budget, so replays past the
+    // end cost a few microseconds each.  Polling runs two launches behind.
+    const int est = (int)((p->rank + p->block - 1) / p->block);
+    const int rpg = ctx->profiling ? 1 : std::max(1, std::min(8, est / 8));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = !ctx->profiling && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (use_graph) {
+        for (int i = 0; i < rpg; ++i) round();
+        use_graph = hipStreamEndCapture(s, &graph) == hipSuccess && graph != nullptr;
+        if (use_graph) use_graph = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        (void)hipGetLastError();
+    }
+    const int step = use_graph ? rpg : 1;
+    for (int rd = 0, li = 0; rd < max_rounds; rd += step, ++li) {
+        hipError_t le = hipSuccess;
+        if (use_graph)
+            le = hipGraphLaunch(exec, s);
+        else
+            round();
+        if (le == hipSuccess) le = hipGetLastError();
         if (le != hipSuccess) {
             status = kevd_hip_fail(ctx, le, "ARPC round launch", __FILE__, __LINE__);
             break;
         }
-        int slot = rd & 3;
+        int slot = li & 3;
         hipError_t ce = hipMemcpyAsync(&host_st[slot], st, sizeof(ArpcState), hipMemcpyDeviceToHost, s);
         if (ce == hipSuccess) ce = hipEventRecord(ev[slot], s);
         if (ce != hipSuccess) {
             status = kevd_hip_fail(ctx, ce, "ARPC status copy", __FILE__, __LINE__);
             break;
         }
-        if (rd >= 2) {
-            int old = (rd - 2) & 3;
+        if (li >= 2) {
+            int old = (li - 2) & 3;
             hipError_t we = hipEventSynchronize(ev[old]);
             if (we != hipSuccess) {
                 status = kevd_hip_fail(ctx, we, "ARPC status wait", __FILE__, __LINE__);
@@ -269,6 +300,8 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
     hipError_t se = hipStreamSynchronize(s);
     if (status == KEVD_OK && se != hipSuccess) status = kevd_hip_fail(ctx, se, "ARPC sync", __FILE__, __LINE__);
     for (int i = 0; i < 4; ++i) (void)hipEventDestroy(ev[i]);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     return status;
 }
 
@@ -335,7 +368,7 @@ kevd_status run_rounds_lm(kevd_ctx* ctx, const kevd_arpc_params* p, int max_roun
             k_reduce_d<<<nb, 256, 0, s>>>(B.d, n, B.pmax, B.psum, B.st, 0);
             k_weights_scan<<<nb, 256, 0, s>>>(B.d, n, B.pmax, nb, shift_base, B.prefix, B.btot, B.st);
             k_draw<<<1, 256, 0, s>>>(B.psum, B.pmax, B.btot, nb, B.prefix, n, trK, p->seed, p->block, p->tol,
-                                     B.Sprime, B.st, B.err, B.inj, n_inject);
+                                     B.Sprime, B.st, B.err, B.inj, n_inject, max_rounds);
         }
         {
             ProfScope ps(ctx, TC_REJECT);
