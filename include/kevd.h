@@ -182,6 +182,28 @@ kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* r
 /* U on the device after kevd_evd, as FP32 n x k_out row-major. */
 kevd_status kevd_get_u(kevd_ctx* ctx, float* U_host);
 
+/* The intermediate record of the kEVD: the reference's dump_h5 switch and
+ * its kevd_dump file's datasets (ks_kevd.cu:484, 531-542, 809-815, 935-941,
+ * 1183-1218, 1274-1280, 1562-1576, 1705-1744, 1945-1969, 2164-2170).  With
+ * on = 1, every later kevd_evd with KEVD_NORM_REF keeps on the host, in the
+ * reference's basis K(:,S) with K(S,S) = L L^T (the factor must come from
+ * kevd_arpc, else kevd_evd returns KEVD_EINVAL):
+ *   c1, p1, cq, pq (r), dtil, qtil (n), lam (r),
+ *   Atil (lower triangle, strict upper zero), A, R (upper, strict lower
+ *   zero), G, Btil, W (r x r), V (r x r after a full eigensolve, else the
+ *   r x k_out leading block), Z (r x k_out),
+ * FP64, matrices column-major (the reference's flat order).  U is
+ * kevd_get_u.  Each entry is copied to the host as it is formed, which
+ * synchronises the stream: the timings of a recorded run include it.
+ * on = 0 stops recording and frees the record.  A Sinkhorn run records
+ * nothing. */
+kevd_status kevd_set_dump(kevd_ctx* ctx, int32_t on);
+
+/* One recorded array by its dataset name.  *count: in, the capacity of out;
+ * out, the number of values.  out = NULL only reports the number.  An
+ * unknown or unrecorded name, or too small a capacity, is KEVD_EINVAL. */
+kevd_status kevd_get_intermediate(kevd_ctx* ctx, const char* name, double* out, int64_t* count);
+
 /* ------------------------------------------------------------- bandwidth */
 
 /* S(eps_j) = sum_{i in rows} sum_{all j'} exp(-||x_i - x_j'||^2 / eps_j),

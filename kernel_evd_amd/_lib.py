@@ -74,6 +74,8 @@ SIGNATURES = {
     "kevd_get_linv": (C.c_int, [C.c_void_p, P(dbl)]),
     "kevd_evd": (C.c_int, [C.c_void_p, P(EvdParams), P(EvdResult)]),
     "kevd_get_u": (C.c_int, [C.c_void_p, P(C.c_float)]),
+    "kevd_set_dump": (C.c_int, [C.c_void_p, i32]),
+    "kevd_get_intermediate": (C.c_int, [C.c_void_p, C.c_char_p, P(dbl), P(i64)]),
     "kevd_bandwidth_sweep": (C.c_int, [C.c_void_p, P(i64), i64, P(dbl), i32, C.c_int, P(dbl)]),
     "kevd_bandwidth_sweep_noalloc": (C.c_int, [C.c_void_p, P(i64), i64, P(dbl), i32, C.c_int, P(dbl)]),
     "kevd_bandwidth_post": (C.c_int, [P(dbl), P(dbl), i32, P(dbl), P(dbl), P(dbl), P(dbl)]),

@@ -187,6 +187,32 @@ class Context:
         self._check(self._lib.kevd_get_u(self._h, _p(U, C.c_float)))
         return U
 
+    # The reference's dump datasets (ks_kevd.cu:531-542 and its later
+    # h5add_double sites), in their order there.
+    DUMP_NAMES = ("c1", "p1", "dtil", "cq", "pq", "Atil", "A", "R", "G", "qtil", "Btil", "W", "V", "lam", "Z")
+
+    def set_dump(self, on=True):
+        """Record the kEVD's intermediates on the next evd calls
+        (kevd_set_dump)."""
+        self._check(self._lib.kevd_set_dump(self._h, 1 if on else 0))
+
+    def intermediate(self, name):
+        """One recorded intermediate, FP64; matrices come back r x r (V, Z:
+        r x columns) from the column-major record."""
+        cnt = C.c_int64(0)
+        self._check(self._lib.kevd_get_intermediate(self._h, name.encode(), None, C.byref(cnt)))
+        out = np.empty(cnt.value, dtype=np.float64)
+        self._check(self._lib.kevd_get_intermediate(self._h, name.encode(), _p(out, C.c_double), C.byref(cnt)))
+        if name in ("Atil", "A", "R", "G", "Btil", "W", "V", "Z"):
+            r = C.c_int64(0)  # the rank, from a length-r entry
+            self._check(self._lib.kevd_get_intermediate(self._h, b"c1", None, C.byref(r)))
+            return out.reshape(r.value, out.size // r.value, order="F")
+        return out
+
+    def intermediates(self):
+        """Every recorded intermediate by name."""
+        return {name: self.intermediate(name) for name in self.DUMP_NAMES}
+
     # --------------------------------------------------------- bandwidth
     def bandwidth_sweep(self, rows, eps, prec="mixed", noalloc=False):
         """kevd_bandwidth_sweep; noalloc=True calls the allocation-free

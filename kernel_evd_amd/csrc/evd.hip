@@ -20,6 +20,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -2156,6 +2157,118 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
     return kevd_fail(ctx, KEVD_EEIG, "subspace iteration: orthonormalisation failed");
 }
 
+// ------------------------------------------------ intermediate record
+// The reference's dump datasets (ks_kevd.cu:531-542, 809-815, 935-941,
+// 1183-1218, 1274-1280, 1562-1576, 1705-1744, 1945-1969, 2164-2170), in its
+// basis: the kernel columns K(:,S), with K(S,S) = L L^T, where this kEVD
+// works with F = K(:,S) L^-T.  From t = F^T 1, u = F^T d~^-1 and the F-basis
+// Grams A = F^T D~^-2 F = L^-1 Atil L^-T, B = F^T Q~^-1 F = L^-1 Btil L^-T:
+//   c1 = L t, p1 = L^-T t, cq = L u, pq = L^-T u,
+//   Atil = L A L^T (lower triangle, strict upper zero: the reference's syrk
+//   fills only the lower one), R = chol(A + sI) upper (strict lower zero),
+//   G = R L^-1, Btil = L B L^T, W = R B R^T, Z = L^-T R^-1 V_k,
+// and dtil = d~, qtil = q~, V, lam as computed.  Matrices are column-major.
+// Every entry is copied to the host when it is formed (synchronously).
+
+// out(i, j) = i <= j ? Lc(j, i) : 0: the upper factor R = Lc^T from the
+// lower Cholesky factor in the lower triangle of Lc.
+__global__ __launch_bounds__(256) void k_rec_upper(const double* Lc, int64_t r, double* out)
+{
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= r * r) return;
+    int64_t i = e % r, j = e / r;
+    out[e] = i <= j ? Lc[j + i * r] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_rec_tril(double* M, int64_t r)
+{
+    int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= r * r) return;
+    if (e % r < e / r) M[e] = 0.0;
+}
+
+struct Record {
+    kevd_ctx* ctx = nullptr;
+    rocblas_handle h = nullptr;
+    int64_t r = 0;
+    double *L = nullptr, *M = nullptr, *M2 = nullptr, *v = nullptr;
+    bool on = false;
+
+    kevd_status begin(kevd_ctx* c, rocblas_handle hd)
+    {
+        ctx = c, h = hd, r = c->rank;
+        on = c->dump;
+        if (!on) return KEVD_OK;
+        KEVD_TRY(ctx_buf_t(ctx, "rec_L", (size_t)r * r, &L));
+        KEVD_TRY(ctx_buf_t(ctx, "rec_M", (size_t)r * r, &M));
+        KEVD_TRY(ctx_buf_t(ctx, "rec_M2", (size_t)r * r, &M2));
+        KEVD_TRY(ctx_buf_t(ctx, "rec_v", r, &v));
+        return pivot_cholesky(ctx, L);
+    }
+    kevd_status put(const char* name, const double* dev, size_t count)
+    {
+        if (!on) return KEVD_OK;
+        std::vector<double>& out = ctx->record[name];
+        out.resize(count);
+        KEVD_HIP(ctx, hipMemcpyAsync(out.data(), dev, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return KEVD_OK;
+    }
+    // name = L x (lower = true) or L^-T x
+    kevd_status vec(const char* name, const double* x, bool times_l)
+    {
+        if (!on) return KEVD_OK;
+        KEVD_HIP(ctx, hipMemcpyAsync(v, x, r * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        rocblas_status st = times_l ? rocblas_dtrmv(h, rocblas_fill_lower, rocblas_operation_none,
+                                                    rocblas_diagonal_non_unit, (rocblas_int)r, L, (rocblas_int)r, v, 1)
+                                    : rocblas_dtrsv(h, rocblas_fill_lower, rocblas_operation_transpose,
+                                                    rocblas_diagonal_non_unit, (rocblas_int)r, L, (rocblas_int)r, v, 1);
+        if (st != rocblas_status_success) return kevd_fail(ctx, KEVD_EHIP, "record: triangular vector op failed");
+        return put(name, v, r);
+    }
+    // name = L X L^T, lower triangle only when tril
+    kevd_status sandwich(const char* name, const double* X, bool tril)
+    {
+        if (!on) return KEVD_OK;
+        const double one = 1.0;
+        const rocblas_int R = (rocblas_int)r;
+        if (rocblas_dtrmm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
+                          R, R, &one, L, R, X, R, M, R) != rocblas_status_success ||
+            rocblas_dtrmm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                          rocblas_diagonal_non_unit, R, R, &one, L, R, M, R, M2, R) != rocblas_status_success)
+            return kevd_fail(ctx, KEVD_EHIP, "record: triangular product failed");
+        if (tril) k_rec_tril<<<(unsigned)((r * r + 255) / 256), 256, 0, ctx->stream>>>(M2, r);
+        KEVD_HIP(ctx, hipGetLastError());
+        return put(name, M2, (size_t)r * r);
+    }
+    // R from the lower Cholesky factor Lc of A + sI, and G = R L^-1
+    kevd_status chol(const double* Lc)
+    {
+        if (!on) return KEVD_OK;
+        const double one = 1.0;
+        const rocblas_int R = (rocblas_int)r;
+        k_rec_upper<<<(unsigned)((r * r + 255) / 256), 256, 0, ctx->stream>>>(Lc, r, M);
+        KEVD_HIP(ctx, hipGetLastError());
+        KEVD_TRY(put("R", M, (size_t)r * r));
+        if (rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
+                          R, R, &one, L, R, M, R) != rocblas_status_success)
+            return kevd_fail(ctx, KEVD_EHIP, "record: triangular solve failed");
+        return put("G", M, (size_t)r * r);
+    }
+    // Z = L^-T Y for the r x k block Y = R^-1 V_k
+    kevd_status z(const double* Y, int k)
+    {
+        if (!on) return KEVD_OK;
+        const double one = 1.0;
+        KEVD_HIP(ctx, hipMemcpyAsync(M, Y, (size_t)r * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        if (rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                          rocblas_diagonal_non_unit, (rocblas_int)r, (rocblas_int)k, &one, L, (rocblas_int)r, M,
+                          (rocblas_int)r) != rocblas_status_success)
+            return kevd_fail(ctx, KEVD_EHIP, "record: triangular solve failed");
+        return put("Z", M, (size_t)r * k);
+    }
+};
+
 template <typename T, typename Ops>
 kevd_status evd_ref(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd_evd_result* res)
 {
@@ -2181,14 +2294,22 @@ kevd_status evd_ref(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd_evd_
     KEVD_TRY(ctx_buf_t(ctx, "e_pinfo", 1, &info));
     unsigned gn = (unsigned)((n + 255) / 256);
     double mm[2];
+    Record rec;
+    KEVD_TRY(rec.begin(ctx, h));
     KEVD_TRY(ops.colsum(nullptr, t));  // t = F^T 1
     KEVD_TRY(ops.rowdot(t, dt, 0, nullptr, nullptr, mm));  // d~ = F t
     res->min_degree = mm[0];
+    KEVD_TRY(rec.vec("c1", t, true));
+    KEVD_TRY(rec.vec("p1", t, false));
+    KEVD_TRY(rec.put("dtil", dt, n));
     if (!(mm[0] > 0.0))
         return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive degree d~ (min " + std::to_string(mm[0]) + ")");
     k_weights<<<gn, 256, 0, ctx->stream>>>(dt, n, 2, w1);  // 1/d~
     KEVD_TRY(ops.colsum(w1, cq));                      // cq = F^T d~^-1
     KEVD_TRY(ops.rowdot(cq, qt, 0, nullptr, nullptr, mm));  // q~ = F cq
+    KEVD_TRY(rec.vec("cq", cq, true));
+    KEVD_TRY(rec.vec("pq", cq, false));
+    KEVD_TRY(rec.put("qtil", qt, n));
     if (!(mm[0] > 0.0))
         return kevd_fail(ctx, KEVD_ENONPOS_Q, "non-positive q~ (min " + std::to_string(mm[0]) + ")");
     k_weights<<<gn, 256, 0, ctx->stream>>>(dt, n, 0, w1);  // 1/d~^2
@@ -2198,6 +2319,9 @@ kevd_status evd_ref(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd_evd_
         TraceRange tr("kEVD Grams A, B");
         KEVD_TRY(ops.gram2(w1, w2, A, B));
     }
+    KEVD_TRY(rec.sandwich("Atil", A, true));
+    KEVD_TRY(rec.put("A", A, (size_t)r * r));
+    KEVD_TRY(rec.sandwich("Btil", B, false));
     // dense phase 1: A + sI = L L^T (R = L^T)
     double trsh[2];
     {
@@ -2212,6 +2336,7 @@ kevd_status evd_ref(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd_evd_
     res->shift = trsh[1];
     if (pinfo != 0)
         return kevd_fail(ctx, KEVD_EPOTRF, "Cholesky of A + sI failed at minor " + std::to_string(pinfo));
+    KEVD_TRY(rec.chol(A));
     // dense phase 2: W = R B R^T = L^T B L
     double one = 1.0, zero = 0.0;
     {
@@ -2226,12 +2351,17 @@ kevd_status evd_ref(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd_evd_
                            rocblas_diagonal_non_unit, (rocblas_int)r, (rocblas_int)r, &one, A,
                            (rocblas_int)r, W, (rocblas_int)r, B, (rocblas_int)r));  // B = L^T W
     }
+    KEVD_TRY(rec.put("W", B, (size_t)r * r));
     double* Vk = nullptr;
     int eit = 0;
     {
         TraceRange tr("kEVD eigensolve");
         KEVD_TRY(eig_leading(ctx, B, r, k, p->all_eigenvalues != 0, lam, &Vk, &eit));
     }
+    // all r eigenvectors when the solve was a full one (they sit in B)
+    const bool full_v = Vk == B + (r - k) * r;
+    KEVD_TRY(rec.put("V", full_v ? B : Vk, (size_t)r * (full_v ? r : k)));
+    KEVD_TRY(rec.put("lam", lam, r));
     // Y = L^-T V_k  (the trailing k eigenvectors, ascending)
     {
         ProfScope ps(ctx, TC_DENSE);
@@ -2239,6 +2369,7 @@ kevd_status evd_ref(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd_evd_
                            rocblas_diagonal_non_unit, (rocblas_int)r, (rocblas_int)k, &one, A,
                            (rocblas_int)r, Vk, (rocblas_int)r));
     }
+    KEVD_TRY(rec.z(Vk, k));
     // U = D~^-1 F Y, and P1 = D~^-1 F (B cq)
     k_weights<<<gn, 256, 0, ctx->stream>>>(dt, n, 2, w1);
     KEVD_TRY(ops.ugemm(Vk, k, nullptr, v2, w1, ctx->U, P1));
@@ -2510,6 +2641,10 @@ extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_ev
         return kevd_fail(ctx, KEVD_EINVAL,
                          "evd: the kEVD without a factor (after the LM form of ARPC) runs in FP64 only");
     if (ctx->rank > 46340) return kevd_fail(ctx, KEVD_EINVAL, "evd: rank too large for the r x r phase");
+    ctx->record.clear();
+    if (ctx->dump && p->norm == KEVD_NORM_REF && ctx->pivots.size() != (size_t)ctx->rank)
+        return kevd_fail(ctx, KEVD_EINVAL,
+                         "evd: the intermediate record needs the pivot Cholesky factor, so a factor from kevd_arpc");
     KEVD_HIP(ctx, hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();
     ctx->tm.evd_ms = 0;
@@ -2557,5 +2692,30 @@ extern "C" kevd_status kevd_get_u(kevd_ctx* ctx, float* U_host)
     KEVD_HIP(ctx, hipMemcpyAsync(U_host, ctx->U, (size_t)ctx->n * ctx->k_out * sizeof(float),
                                  hipMemcpyDeviceToHost, ctx->stream));
     KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KEVD_OK;
+}
+
+extern "C" kevd_status kevd_set_dump(kevd_ctx* ctx, int32_t on)
+{
+    if (!ctx) return KEVD_EINVAL;
+    ctx->dump = on != 0;
+    if (!ctx->dump) ctx->record.clear();
+    return KEVD_OK;
+}
+
+extern "C" kevd_status kevd_get_intermediate(kevd_ctx* ctx, const char* name, double* out, int64_t* count)
+{
+    if (!ctx || !name || !count) return kevd_fail(ctx, KEVD_EINVAL, "get_intermediate: null argument");
+    auto it = ctx->record.find(name);
+    if (it == ctx->record.end())
+        return kevd_fail(ctx, KEVD_EINVAL, std::string("get_intermediate: no recorded '") + name + "'");
+    const int64_t m = (int64_t)it->second.size();
+    if (out) {
+        if (*count < m)
+            return kevd_fail(ctx, KEVD_EINVAL, std::string("get_intermediate: '") + name + "' holds " +
+                                                   std::to_string(m) + " values");
+        std::memcpy(out, it->second.data(), (size_t)m * sizeof(double));
+    }
+    *count = m;
     return KEVD_OK;
 }

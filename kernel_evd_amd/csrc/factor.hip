@@ -122,7 +122,45 @@ __global__ __launch_bounds__(256) void k_linv_colmajor(const double* Linv, int64
     int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= r * r) return;
     int64_t c = i / r, a = i % r;
-    out[a + c * r] = Linv[a * ldl + c];
+    out[a + c * r] = c <= a ? Linv[a * ldl + c] : 0.0;
+}
+
+// K(S,S) = L L^T for the pivots S of the held factor: L lower triangular,
+// column-major r x r FP64 with the strict upper triangle zero.  The F form
+// holds L as its pivot rows F(S,:); the LM form holds L^-1, inverted here.
+kevd_status pivot_cholesky(kevd_ctx* ctx, double* L)
+{
+    int64_t r = ctx->rank;
+    bool lm = ctx->lm && ctx->lm_linv;
+    if (r <= 0 || ctx->pivots.size() != (size_t)r || (!lm && !ctx->have_factor))
+        return kevd_fail(ctx, KEVD_EINVAL, "the pivot Cholesky factor needs a factor built by kevd_arpc");
+    unsigned g = (unsigned)((r * r + 255) / 256);
+    if (lm) {
+        int* info = nullptr;
+        KEVD_TRY(ctx_buf_t(ctx, "l_info", 1, &info));
+        k_linv_colmajor<<<g, 256, 0, ctx->stream>>>(ctx->lm_linv, ctx->lm_ldl, r, L);
+        KEVD_HIP(ctx, hipGetLastError());
+        rocblas_handle h = blas_handle(ctx);
+        if (!h) return kevd_fail(ctx, KEVD_EHIP, "rocblas handle creation failed");
+        if (rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, (rocblas_int)r, L,
+                             (rocblas_int)r, info) != rocblas_status_success)
+            return kevd_fail(ctx, KEVD_EHIP, "rocsolver_dtrtri failed");
+        int hinfo = 0;
+        KEVD_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (hinfo != 0) return kevd_fail(ctx, KEVD_EINVAL, "the kept L^-1 is singular");
+        return KEVD_OK;
+    }
+    int64_t* piv = nullptr;
+    KEVD_TRY(ctx_buf_t(ctx, "l_piv", r, &piv));
+    KEVD_HIP(ctx, hipMemcpyAsync(piv, ctx->pivots.data(), r * sizeof(int64_t), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    if (ctx->fprec == KEVD_MIXED)
+        k_gather_L<float><<<g, 256, 0, ctx->stream>>>((const float*)ctx->F, ctx->ldf, piv, r, L);
+    else
+        k_gather_L<double><<<g, 256, 0, ctx->stream>>>((const double*)ctx->F, ctx->ldf, piv, r, L);
+    KEVD_HIP(ctx, hipGetLastError());
+    return KEVD_OK;
 }
 
 extern "C" kevd_status kevd_get_linv(kevd_ctx* ctx, double* Linv_host)

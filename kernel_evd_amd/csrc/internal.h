@@ -69,6 +69,11 @@ struct kevd_ctx {
     // device.
     void* scratch = nullptr;
 
+    // The kEVD's intermediate record (kevd_set_dump): the reference's dump
+    // datasets by name, FP64 on the host, column-major for matrices.
+    bool dump = false;
+    std::map<std::string, std::vector<double>> record;
+
     // kEVD output held on the device.
     float* U = nullptr;
     size_t U_bytes = 0;
@@ -128,5 +133,6 @@ void prof_resolve(kevd_ctx* ctx);
 kevd_status ctx_check_arch(kevd_ctx* ctx);
 
 // Cross-unit entry points.
+kevd_status pivot_cholesky(kevd_ctx* ctx, double* L);
 kevd_status kblock_gather(kevd_ctx* ctx, const int64_t* idx_dev, int64_t count,
                           kevd_precision prec, void* Xout, double* nrm_out);

@@ -9,7 +9,8 @@
 //             (--eps E | --eps-median) [--regenerate])
 //            [--kout 100] [--norm ref|sinkhorn] [--sinkhorn-tol 1e-12]
 //            [--precision mixed|fp64] [--all-eigenvalues] [--lam-out lam.npy]
-//            [--U-out U.npy] [--out evd.h5 [--leading-only]] [--device 0]
+//            [--U-out U.npy] [--out evd.h5 [--leading-only]]
+//            [--dump kevd_dump.h5|DIR] [--device 0]
 // Without --all-eigenvalues only the leading eigenvalues are computed and
 // the rest of lam is NaN.  --out writes the reference's EVD file
 // (ks_kevd.cu:488-501, 1930-1938, 2300-2306): /lam (all r eigenvalues,
@@ -21,6 +22,18 @@
 // factor is stored at all: the LM form of ARPC keeps S and L^-1 and the
 // kEVD regenerates K(R,S) panel by panel in every pass, as the reference's
 // kEVD does from /spiv and /linv (FP64 only).
+// --dump writes the reference's kevd_dump file (its dump_h5 switch,
+// ks_kevd.cu:484, 531-542 and the later h5add_double sites): c1, p1, dtil,
+// cq, pq, Atil, A, R, G, qtil, Btil, W, V, lam, Z (FP64) and U (FP32), each
+// one flat 1-D array, matrices column-major (read back with order='F'), in
+// the reference's kernel basis K(:,S).  A name ending in .h5 is one HDF5
+// file of those datasets; anything else is a directory of NAME.npy files.
+// It needs --pivots (a factor file carries no kernel basis) and the ref
+// normalisation, and implies --all-eigenvalues unless --leading-only (V is
+// then the r x k_out block).
+#include <sys/stat.h>
+
+#include <cerrno>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -46,7 +59,7 @@ int main(int argc, char** argv)
                     "          [--regenerate])\n"
                     "         [--kout 100] [--norm ref|sinkhorn] [--sinkhorn-tol 1e-12]\n"
                     "         [--precision mixed|fp64] [--all-eigenvalues] [--lam-out lam.npy] [--U-out U.npy]\n"
-                    "         [--out evd.h5 [--leading-only]] [--device 0]\n");
+                    "         [--out evd.h5 [--leading-only]] [--dump kevd_dump.h5|DIR] [--device 0]\n");
         return 0;
     }
     auto t0 = std::chrono::steady_clock::now();
@@ -140,12 +153,22 @@ int main(int argc, char** argv)
         ep.sinkhorn_max_iter = (int32_t)a.integer("sinkhorn-max-iter", 1000);
         ep.prec = prec;
         const bool h5out = a.has("out");
-        ep.all_eigenvalues = a.has("all-eigenvalues") || (h5out && !a.has("leading-only")) ? 1 : 0;
+        const bool dump = a.has("dump");
+        ep.all_eigenvalues = a.has("all-eigenvalues") || ((h5out || dump) && !a.has("leading-only")) ? 1 : 0;
+        if (dump) {
+            if (ep.norm != KEVD_NORM_REF) {
+                std::fprintf(stderr, "--dump records the ref normalisation's intermediates\n");
+                kevd_destroy(c);
+                return 1;
+            }
+            s = kevd_set_dump(c, 1);
+            if (s != KEVD_OK) return fail(c, s, "kevd_set_dump");
+        }
         std::vector<double> lam((size_t)r);
         std::vector<float> U;
         kevd_evd_result er{};
         er.lam = lam.data();
-        if (a.has("U-out") || h5out) {
+        if (a.has("U-out") || h5out || dump) {
             U.resize((size_t)npts * ep.k_out);
             er.U = U.data();
         }
@@ -166,6 +189,31 @@ int main(int argc, char** argv)
             }
             save_array(a.str("out"), "<f8", {r}, lam.data(), "/lam", false);
             save_array(a.str("out"), "<f4", {npts, (int64_t)ep.k_out}, U.data(), "/U", true);
+        }
+        if (dump) {
+            const std::string path = a.str("dump");
+            const bool h5 = is_h5(path);
+            if (!h5 && mkdir(path.c_str(), 0755) != 0 && errno != EEXIST) {
+                std::fprintf(stderr, "cannot create the directory %s\n", path.c_str());
+                kevd_destroy(c);
+                return 1;
+            }
+            bool first = true;
+            std::vector<double> v;
+            for (const char* name : {"c1", "p1", "dtil", "cq", "pq", "Atil", "A", "R", "G", "qtil", "Btil", "W",
+                                     "V", "lam", "Z"}) {
+                int64_t cnt = 0;
+                s = kevd_get_intermediate(c, name, nullptr, &cnt);
+                if (s != KEVD_OK) return fail(c, s, "kevd_get_intermediate");
+                v.resize((size_t)cnt);
+                s = kevd_get_intermediate(c, name, v.data(), &cnt);
+                if (s != KEVD_OK) return fail(c, s, "kevd_get_intermediate");
+                const std::string ds = std::string("/") + name;
+                save_array(h5 ? path : path + ds + ".npy", "<f8", {cnt}, v.data(), ds, h5 && !first);
+                first = false;
+            }
+            save_array(h5 ? path : path + "/U.npy", "<f4", {npts * (int64_t)ep.k_out}, U.data(), "/U", h5);
+            std::printf("intermediates written to %s\n", path.c_str());
         }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "%s\n", e.what());

@@ -237,6 +237,98 @@ def lowrank_evd(F, k_out):
     return lam, U, P1 - 1.0, s
 
 
+def _chol_lower(M):
+    """Cholesky factor, lower, by columns (any float dtype, numpy has no
+    long double LAPACK)."""
+    n = M.shape[0]
+    L = np.zeros_like(M)
+    for j in range(n):
+        d = M[j, j] - np.dot(L[j, :j], L[j, :j])
+        if not d > 0:
+            raise np.linalg.LinAlgError("not positive definite at %d" % j)
+        L[j, j] = np.sqrt(d)
+        L[j + 1:, j] = (M[j + 1:, j] - L[j + 1:, :j] @ L[j, :j]) / L[j, j]
+    return L
+
+
+def _inv_lower(L):
+    """Inverse of a lower triangular matrix, row by row."""
+    n = L.shape[0]
+    X = np.zeros_like(L)
+    for i in range(n):
+        X[i, i] = 1 / L[i, i]
+        X[i, :i] = -(L[i, :i] @ X[:i, :i]) / L[i, i]
+    return X
+
+
+def kevd_record(X, eps, S, k_out, extended=False):
+    """The reference kEVD's dump datasets, computed as the reference does,
+    over the kernel columns K = K(:,S) with L^-1 from a dense Cholesky of
+    K(S,S) = L L^T (ks_kevd.cu; the dumped names of 531-542 and the later
+    h5add_double sites):
+      c1 = K^T 1 (755), p1 = L^-T (L^-1 c1) (883), dtil = K p1,
+      cq = K^T dtil^-1 (1122), pq = L^-T (L^-1 cq) (1220), qtil = K pq,
+      Atil = K^T D^-2 K (lower triangle kept, 1405), A = L^-1 Atil L^-T
+      (1413-1440), s = DBL_EPSILON tr(A) (1483), R = chol(A + sI) upper
+      (1500), G = R L^-1 (1538), Btil = K^T Q^-1 K (1662), W = G Btil G^T
+      (1764), W = V diag(lam) V^T, Z = L^-T R^-1 V_k (1973), and
+      U = D^-1 K Z (2173).
+    Nothing here goes through the factor F = K L^-T.  Matrices are
+    returned as 2-D FP64 arrays; Atil with its strict upper triangle zero,
+    R with its strict lower triangle zero.
+
+    This route multiplies rounding errors by L^-1 twice: in FP64, R and G
+    carry relative errors up to ~eps kappa(K(S,S))^1.3 (2e-7 in G at
+    kappa = 2.8e6).  extended=True runs everything up to W in long double
+    (x87, eps 1.1e-19; kernel values from direct differences), which brings
+    those errors below the FP64 F-form's; the eigensolve of W stays FP64."""
+    S = np.asarray(S, dtype=np.int64)
+    r, n = S.size, X.shape[0]
+    if extended:
+        T = np.longdouble
+        XL = np.asarray(X, dtype=T)
+        d2 = ((XL[:, None, :] - XL[None, S, :]) ** 2).sum(-1)
+        K = np.exp(-d2 / T(eps))
+        Lc = _chol_lower(K[S, :])
+        Li = _inv_lower(Lc)
+        chol = _chol_lower
+    else:
+        T = np.float64
+        X = np.asarray(X, dtype=np.float64)
+        K = kernel_block(X, np.arange(n), S, eps)
+        Lc = np.linalg.cholesky(K[S, :])
+        Li = np.tril(np.linalg.inv(Lc))
+        chol = np.linalg.cholesky
+    out = {}
+    c1 = K.T @ np.ones(n, dtype=T)
+    p1 = Li.T @ (Li @ c1)
+    dt = K @ p1
+    cq = K.T @ (1 / dt)
+    pq = Li.T @ (Li @ cq)
+    qt = K @ pq
+    Kd = K / dt[:, None]
+    Atil = Kd.T @ Kd
+    A = Li @ Atil @ Li.T
+    s = T(np.finfo(np.float64).eps) * np.trace(A)
+    R = chol(A + s * np.eye(r, dtype=T)).T
+    G = R @ Li
+    Btil = K.T @ (K / qt[:, None])
+    W = G @ Btil @ G.T
+    W = (W + W.T) / 2
+    for name, v in (("c1", c1), ("p1", p1), ("dtil", dt), ("cq", cq), ("pq", pq), ("qtil", qt),
+                    ("Atil", np.tril(Atil)), ("A", A), ("R", R), ("G", G), ("Btil", Btil), ("W", W)):
+        out[name] = np.asarray(v, dtype=np.float64)
+    lam, V = np.linalg.eigh(out["W"])
+    out["V"], out["lam"] = V, lam
+    Vk = V[:, r - k_out:].astype(T)
+    Z = Li.T @ np.linalg.solve(R.astype(np.float64), V[:, r - k_out:]) if not extended else \
+        Li.T @ _inv_lower(R.T).T @ Vk
+    out["Z"] = np.asarray(Z, dtype=np.float64)
+    out["U"] = np.asarray((K @ Z) / dt[:, None], dtype=np.float64)
+    out["shift"] = float(s)
+    return out
+
+
 def _cheb_schedule():
     """Step coefficients of the accelerated Sinkhorn iteration (the same
     schedule the library runs): after the pass that measured s_j with RMS

@@ -90,6 +90,37 @@ def test_pipeline_end_to_end(tmp_path):
 
 
 @pytest.mark.gpu
+def test_evd_dump_directory(tmp_path):
+    """kevd-evd --dump DIR: the reference's kevd_dump datasets as flat .npy
+    files, against the oracle's reference-basis restatement."""
+    X = tmp_path / "X.npy"
+    run("kevd-gen", "--n", 2000, "--discard", 1000, "--seed", 4, "--out", X)
+    pts = np.load(X)
+    eps = orc.median_sqdist(pts)
+    S = tmp_path / "S.npy"
+    run("kevd-arpc", "--points", X, "--eps", repr(eps), "--rank", 120, "--block", 16, "--precision", "fp64",
+        "--pivots-out", S, "--factor-out", tmp_path / "F.npy")
+    d = tmp_path / "dump"
+    run("kevd-evd", "--points", X, "--pivots", S, "--eps", repr(eps), "--kout", 8, "--precision", "fp64",
+        "--dump", d)
+    ref = orc.kevd_record(pts, eps, np.load(S), 8, extended=True)
+    r = 120
+    for name in ("c1", "dtil", "cq", "qtil", "lam"):
+        v = np.load(d / (name + ".npy"))
+        assert v.ndim == 1
+        np.testing.assert_allclose(v, ref[name], rtol=1e-9, atol=1e-10)
+    for name in ("Atil", "A", "R", "G", "Btil", "W"):
+        v = np.load(d / (name + ".npy")).reshape(r, r, order="F")
+        assert np.linalg.norm(v - ref[name]) < 1e-8 * np.linalg.norm(ref[name]), name
+    assert np.load(d / "V.npy").size == r * r and np.load(d / "Z.npy").size == r * 8
+    assert np.load(d / "U.npy").dtype == np.float32 and np.load(d / "U.npy").size == 2000 * 8
+    # a factor file carries no pivots: no kernel basis to record in
+    bad = run("kevd-evd", "--points", X, "--factor", tmp_path / "F.npy", "--kout", 8, "--precision", "fp64",
+              "--dump", tmp_path / "dump2", check=False)
+    assert bad.returncode == 1, bad.stdout
+
+
+@pytest.mark.gpu
 def test_bw0_out_matches_reference_columns(tmp_path):
     """--all-pairs --grid ref36: column 2 of bw0.out is the reference's
     log(2 S_lower / NM + 1) (ks_bandwidth.cu:615-618) = log(S_full / NM),

@@ -129,3 +129,25 @@ def test_arpc_and_evd_on_reference_files(h5_ok, tmp_path):
     for j in range(10):
         c = np.dot(U[:, j], Ur[:, j]) / (np.linalg.norm(U[:, j]) * np.linalg.norm(Ur[:, j]))
         assert abs(abs(c) - 1.0) < 1e-5, (j, c)
+
+
+@pytest.mark.gpu
+def test_evd_dump_file(h5_ok, tmp_path):
+    """--dump: the reference's kevd_dump datasets, flat 1-D, FP64 but U
+    FP32, and the same values as the .npy directory form."""
+    X = tmp_path / "X.npy"
+    run("kevd-gen", "--n", 1500, "--discard", 1000, "--seed", 2, "--out", X)
+    run("kevd-arpc", "--points", X, "--eps-median", "--rank", 80, "--block", 16, "--precision", "fp64",
+        "--pivots-out", tmp_path / "spiv.h5")
+    args = ["--points", X, "--pivots", tmp_path / "spiv.h5", "--eps-median", "--kout", 6, "--precision", "fp64"]
+    print("step kevd-evd --dump h5", flush=True)
+    run("kevd-evd", *args, "--dump", tmp_path / "dump.h5")
+    print("step kevd-evd --dump npy", flush=True)
+    run("kevd-evd", *args, "--dump", tmp_path / "dump")
+    sizes = {"c1": 80, "p1": 80, "dtil": 1500, "cq": 80, "pq": 80, "Atil": 6400, "A": 6400, "R": 6400,
+             "G": 6400, "qtil": 1500, "Btil": 6400, "W": 6400, "V": 6400, "lam": 80, "Z": 480, "U": 9000}
+    for name, size in sizes.items():
+        v, ty = h5read(tmp_path / "dump.h5", "/" + name)
+        assert ty == ("H5T_IEEE_F32LE" if name == "U" else "H5T_IEEE_F64LE"), name
+        assert v.size == size, name
+        np.testing.assert_array_equal(v, np.load(tmp_path / "dump" / (name + ".npy")).astype(np.float64))

@@ -12,6 +12,10 @@ retyped copy of its own formulas.
   bandwidth sums      brute force; eps -> inf and eps -> 0 limits
   kEVD (NumPy)        full rank equals the dense ks_reference restatement;
                       lambda_max = 1, P 1 = 1 (bistochastic by construction)
+  kEVD record         the reference-basis passes (kevd_record): d~, q~ are
+                      the Nystrom degrees K(:,S) K(S,S)^-1 K(S,:) 1 by a
+                      direct solve; p1, pq solve K(S,S); R^T R = A + sI;
+                      lam and U equal the F-form restatement's
   Sinkhorn            s .* (K s) = 1 at convergence
   KS ETDRK4           linear-regime closed form exp(L t); mean conservation;
                       4th-order self-convergence in dt
@@ -164,6 +168,61 @@ def test_lowrank_evd_full_rank_equals_dense():
     for j in range(1, 6):
         u, v = U[:, -1 - j], V[:, j]
         assert min(np.abs(u - v).max(), np.abs(u + v).max()) < 1e-6 * np.abs(v).max() * 150
+
+
+def test_kevd_record_reference_basis():
+    X = gauss_points(300, 3, seed=21)
+    eps = orc.median_sqdist(X)
+    ref = orc.arpc(X, eps, 60, block=16, seed=1)
+    S, k = ref.pivots, 8
+    rec = orc.kevd_record(X, eps, S, k)
+    K = orc.kernel_block(X, np.arange(300), S, eps)
+    KSS = K[S, :]
+    # Nystrom degrees by a direct solve, no Cholesky factor involved
+    Kn = K @ np.linalg.solve(KSS, K.T)
+    np.testing.assert_allclose(rec["dtil"], Kn @ np.ones(300), rtol=1e-9)
+    np.testing.assert_allclose(rec["qtil"], Kn @ (1.0 / rec["dtil"]), rtol=1e-9)
+    np.testing.assert_allclose(KSS @ rec["p1"], rec["c1"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(KSS @ rec["pq"], rec["cq"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(rec["c1"], K.sum(axis=0), rtol=1e-12)
+    # Atil: lower triangle of K^T D^-2 K; R upper with R^T R = A + sI
+    assert np.all(np.triu(rec["Atil"], 1) == 0.0) and np.all(np.tril(rec["R"], -1) == 0.0)
+    Kd = K / rec["dtil"][:, None]
+    np.testing.assert_allclose(rec["Atil"], np.tril(Kd.T @ Kd), rtol=1e-9, atol=1e-12)
+    r = S.size
+    np.testing.assert_allclose(rec["R"].T @ rec["R"], rec["A"] + rec["shift"] * np.eye(r), rtol=1e-9, atol=1e-12)
+    # the same spectrum and eigenvectors as the F-form restatement on the C
+    # oracle's factor F = K(:,S) L^-T
+    lam, U, dev, sF = orc.lowrank_evd(ref.F, k)
+    np.testing.assert_allclose(rec["lam"], lam, atol=1e-11)
+    assert abs(rec["lam"][-1] - 1.0) < 1e-10 and abs(rec["shift"] - sF) < 1e-12 * abs(sF) + 1e-30
+    for j in range(k):
+        u, v = rec["U"][:, j], U[:, j]
+        assert min(np.abs(u - v).max(), np.abs(u + v).max()) < 1e-7 * np.abs(v).max()
+
+
+def test_kevd_record_extended_precision():
+    # rank 240 of 3000 points: kappa(K(S,S)) = 2.8e6.  The FP64 F-form
+    # route (F from the C oracle, R = chol(F^T D~^-2 F + sI), G L = R by a
+    # solve with L = F(S,:)) and the long double kernel-basis route agree
+    # in G to 5e-10; the FP64 kernel-basis route is 400x further off.
+    X = gauss_points(3000, 4, seed=31)
+    eps = orc.median_sqdist(X)
+    ref = orc.arpc(X, eps, 240, block=32, seed=5)
+    S, F = ref.pivots, ref.F
+    ext = orc.kevd_record(X, eps, S, 8, extended=True)
+    dbl = orc.kevd_record(X, eps, S, 8)
+    n, r = F.shape
+    dt = F @ (F.T @ np.ones(n))
+    Af = (F / dt[:, None]).T @ (F / dt[:, None])
+    Rf = np.linalg.cholesky(Af + np.finfo(float).eps * np.trace(Af) * np.eye(r)).T
+    Gf = np.linalg.solve(F[S, :].T, Rf.T).T
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(Gf, ext["G"]) < 5e-9
+    assert rel(Rf, ext["R"]) < 1e-10
+    assert rel(dbl["G"], ext["G"]) > 1e-8
+    for name in ("c1", "dtil", "cq", "qtil", "Atil", "Btil", "W", "lam"):
+        assert rel(dbl[name], ext[name]) < 1e-10, name
 
 
 def test_sinkhorn_fixed_point():
