@@ -20,10 +20,12 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 #include <vector>
 
+#include "bf16x6.h"
 #include "gemm.h"
 #include "internal.h"
 
@@ -880,6 +882,202 @@ void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T*
     if (since > 0 || first) g.promote(out, first);
 }
 
+// ---------------------------------------------- weighted Gram, bf16x6
+// G = F^T diag(w) F for FP32 F on the bf16 matrix cores (bf16x6 products,
+// FP32 accumulation, FP64 promotion every PROMOTE rows as in k_gram).
+// 256 x 256 output tiles of the lower triangle on 8 waves (2 x 4) of
+// 128 x 64 each; 16 rows of F per chunk, one workgroup per CU.  Thread t
+// stages one column of one panel (threads 0..255 the weighted A panel,
+// 256..511 the B panel): it reads the column's 16 rows (one dword per row;
+// a wave reads 256 contiguous bytes of each row), splits each value into
+// bf16 hi, mid, lo and writes each part as two 16-byte halves of 8 rows.
+// LDS per stage: [panel][part][half][column][8 rows], so an MFMA lane's
+// fragment (8 rows of one column) is one 16-byte read and 32 lanes read or
+// write 512 contiguous bytes.
+struct GramX6 {
+    static constexpr int BI = 256, BJ = 256, KC = 16, NT = 512, WGM = 2, WGN = 4;
+    static constexpr int TM = 4, TN = 2, WR = 128, WC = 64, PB = 16, PROMOTE = KEVD_GRAM_PROMOTE;
+    static constexpr int HALF = 256 * 16;  // bytes of one half slab (256 columns x 8 rows)
+    static constexpr int PART = 2 * HALF;
+    static constexpr int PANEL = 3 * PART;
+    static constexpr int STAGE = 2 * PANEL;  // 48 KiB
+    static_assert(WGM * WR == BI && WGN * WC == BJ && WGM * WGN * 64 == NT, "Gram x6 wave layout");
+};
+
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_gram_x6(const float* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const float* __restrict__ w,
+               const int* __restrict__ tileI, const int* __restrict__ tileJ, int ntiles, int nitems,
+               int64_t rows_per_split, double* part)
+{
+    typedef GramX6 C;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[2 * C::STAGE];
+    // the split-major item order and XCD assignment of k_gram
+    int id = blockIdx.x, xcd = id & 7, per = nitems >> 3, rem = nitems & 7;
+    int item = xcd * per + (xcd < rem ? xcd : rem) + (id >> 3);
+    int tile = item % ntiles, split = item / ntiles;
+    const int64_t ci = (int64_t)tileI[tile] * C::BI, cj = (int64_t)tileJ[tile] * C::BJ;
+    const int64_t k_beg = (int64_t)split * rows_per_split;
+    const int64_t k_end = k_beg + rows_per_split < n ? k_beg + rows_per_split : n;
+    double* out = part + ((int64_t)split * ntiles + tile) * (int64_t)C::BI * C::BJ;
+    // F and w over this split's rows: rows at or past k_end read as zero
+    const int nb = k_end > k_beg ? (int)((k_end - k_beg) * ldf * 4) : 0;
+    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(F + k_beg * ldf), (short)0, nb,
+                                                                  0x00020000);
+    __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(w + k_beg), (short)0, k_end > k_beg ? (int)((k_end - k_beg) * 4) : 0, 0x00020000);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int panel = t >> 8;  // wave-uniform
+    const int scol = t & 255;
+    const int64_t gcol = (panel ? cj : ci) + scol;
+    const bool colok = gcol < r;
+    const uint32_t coff = (uint32_t)(gcol * 4);
+    const uint32_t rowb = (uint32_t)(ldf * 4);
+    const int wm = wave / C::WGN, wn = wave % C::WGN, li = lane & 31, hh = lane >> 5;
+    // a wave block with every row index below every column index holds only
+    // entries the reduction never reads
+    const bool live = ci + (int64_t)(wm + 1) * C::WR - 1 >= cj + (int64_t)wn * C::WC;
+
+    floatx16 acc[C::TM][C::TN];
+    auto zero = [&]() {
+#pragma unroll
+        for (int a = 0; a < C::TM; ++a)
+#pragma unroll
+            for (int b = 0; b < C::TN; ++b)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.0f;
+    };
+    zero();
+
+    // The whole offset goes in the VGPR operand: the range check that turns
+    // rows at or past k_end into zeros applies to it.
+    float v[C::KC], wv[C::KC];
+    auto gload = [&](int64_t kro) {
+        const uint32_t base = (uint32_t)kro * rowb + coff;
+#pragma unroll
+        for (int k = 0; k < C::KC; ++k)
+            v[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, base + k * rowb, 0, 0));
+        if (panel == 0) {
+            // the whole vector is bit-cast: clang reads element 0 for a
+            // __builtin_bit_cast of a single ext-vector element (q.y)
+            typedef __attribute__((ext_vector_type(4))) float f4;
+#pragma unroll
+            for (int j = 0; j < C::KC / 4; ++j) {
+                f4 q = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(rw, (uint32_t)kro * 4 + 16 * j, 0, 0));
+                wv[4 * j] = q.x;
+                wv[4 * j + 1] = q.y;
+                wv[4 * j + 2] = q.z;
+                wv[4 * j + 3] = q.w;
+            }
+        }
+    };
+    auto sstore = [&](unsigned char* st) {
+        unsigned char* dst = st + panel * C::PANEL + scol * 16;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            bf16x8 p0, p1, p2;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                float x = colok ? v[8 * h + e] : 0.0f;
+                if (panel == 0) x *= wv[8 * h + e];
+                __bf16 a0, a1, a2;
+                bf16_split3(x, a0, a1, a2);
+                p0[e] = a0, p1[e] = a1, p2[e] = a2;
+            }
+            *reinterpret_cast<bf16x8*>(dst + h * C::HALF) = p0;
+            *reinterpret_cast<bf16x8*>(dst + C::PART + h * C::HALF) = p1;
+            *reinterpret_cast<bf16x8*>(dst + 2 * C::PART + h * C::HALF) = p2;
+        }
+    };
+    auto compute = [&](const unsigned char* st) {
+        if (!live) return;
+        bf16x8 af[C::TM][3], bf[C::TN][3];
+#pragma unroll
+        for (int b = 0; b < C::TN; ++b)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                bf[b][q] = *reinterpret_cast<const bf16x8*>(st + C::PANEL + q * C::PART + hh * C::HALF +
+                                                            (wn * C::WC + b * 32 + li) * 16);
+#pragma unroll
+        for (int a = 0; a < C::TM; ++a)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                af[a][q] = *reinterpret_cast<const bf16x8*>(st + q * C::PART + hh * C::HALF +
+                                                            (wm * C::WR + a * 32 + li) * 16);
+        // products outermost: eight independent accumulators between two
+        // MFMAs into the same one
+#pragma unroll
+        for (int p = 0; p < X6_NPROD; ++p)
+#pragma unroll
+            for (int a = 0; a < C::TM; ++a)
+#pragma unroll
+                for (int b = 0; b < C::TN; ++b) mfma_bf16_32(acc[a][b], af[a][x6_a(p)], bf[b][x6_b(p)]);
+    };
+    // value v of tile (a, b): row wm*WR + 32 a + 8 (v / 4) + 4 hh + v % 4,
+    // column wn*WC + 32 b + li; the partial tile is row-major BI x BJ FP64
+    auto promote = [&](bool first) {
+        __amdgpu_buffer_rsrc_t ro =
+            __builtin_amdgcn_make_buffer_rsrc(out, (short)0, C::BI * C::BJ * (int)sizeof(double), 0x00020000);
+        const int voff = ((wm * C::WR + 4 * hh) * C::BJ + wn * C::WC + li) * (int)sizeof(double);
+        typedef __attribute__((ext_vector_type(2))) unsigned u2;
+        constexpr int NVAL = C::TM * C::TN * 16;
+#pragma unroll
+        for (int b0 = 0; b0 < NVAL; b0 += C::PB) {
+            double x[C::PB];
+#pragma unroll
+            for (int q = 0; q < C::PB; ++q) {
+                int e = b0 + q, vv = e % 16, b = (e / 16) % C::TN, a = e / (16 * C::TN);
+                x[q] = (double)acc[a][b][vv];
+            }
+            if (!first) {
+                double y[C::PB];
+#pragma unroll
+                for (int q = 0; q < C::PB; ++q) {
+                    int e = b0 + q, vv = e % 16, b = (e / 16) % C::TN, a = e / (16 * C::TN);
+                    int soff = ((a * 32 + 8 * (vv >> 2) + (vv & 3)) * C::BJ + b * 32) * (int)sizeof(double);
+                    y[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(ro, voff, soff, 0));
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < C::PB; ++q) x[q] += y[q];
+            }
+#pragma unroll
+            for (int q = 0; q < C::PB; ++q) {
+                int e = b0 + q, vv = e % 16, b = (e / 16) % C::TN, a = e / (16 * C::TN);
+                int soff = ((a * 32 + 8 * (vv >> 2) + (vv & 3)) * C::BJ + b * 32) * (int)sizeof(double);
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x[q]), ro, voff, soff, 0);
+            }
+        }
+        zero();
+    };
+
+    const int64_t nk = k_end > k_beg ? (k_end - k_beg + C::KC - 1) / C::KC : 0;
+    bool first = true;
+    int since = 0;
+    if (nk > 0) {
+        gload(0);
+        sstore(lds);
+        if (nk > 1) gload(C::KC);
+    }
+    __syncthreads();
+    for (int64_t c = 0; c < nk; ++c) {
+        unsigned char* cur = lds + (c & 1) * C::STAGE;
+        unsigned char* nxt = lds + ((c + 1) & 1) * C::STAGE;
+        compute(cur);
+        // the registers hold chunk c + 1: store it, then load c + 2 behind
+        // the next chunk's products
+        if (c + 1 < nk) sstore(nxt);
+        if (c + 2 < nk) gload((c + 2) * C::KC);
+        since += C::KC;
+        if (since >= C::PROMOTE) {
+            promote(first);
+            first = false;
+            since = 0;
+        }
+        __syncthreads();
+    }
+    if (since > 0 || first) promote(first);
+}
+
 template <typename T>
 __global__ void k_cast(const double* x, int64_t n, T* out)
 {
@@ -1153,9 +1351,86 @@ kevd_status rowdot_rows(kevd_ctx* ctx, const Rows<T>& src, const double* c, doub
     return KEVD_OK;
 }
 
+// G = F^T diag(w) F for FP32 F by k_gram_x6 and the split reduction.
+kevd_status gram_x6(kevd_ctx* ctx, const Rows<float>& src, const double* w, double* G, const char* wbuf)
+{
+    typedef GramX6 C;
+    int64_t n = src.n, r = src.r;
+    int ti_n = (int)((r + C::BI - 1) / C::BI), tj_n = (int)((r + C::BJ - 1) / C::BJ);
+    std::vector<int> TI, TJ;
+    for (int I = 0; I < ti_n; ++I)
+        for (int J = 0; J < tj_n; ++J)
+            if ((int64_t)I * C::BI + C::BI - 1 >= (int64_t)J * C::BJ) {
+                TI.push_back(I);
+                TJ.push_back(J);
+            }
+    int ntiles = (int)TI.size();
+    // One workgroup per CU; at least ~2048 workgroups, the split count
+    // chosen so that the grid fills whole waves of the device's CUs.  A
+    // split's rows are addressed by 32-bit buffer offsets.
+    const int64_t slots = ctx->num_cu;
+    const int64_t rp_max = ((int64_t)INT32_MAX / (src.ldf * 4)) / C::KC * C::KC - C::KC;
+    if (rp_max < C::KC) return kevd_fail(ctx, KEVD_EINVAL, "Gram: factor rows too long for 32-bit offsets");
+    int ns0 = (int)std::max<int64_t>({(int64_t)1, (2048 + ntiles - 1) / ntiles, (n + rp_max - 1) / rp_max});
+    int nsplit = 1;
+    int64_t rps = n;
+    double best = -1.0;
+    for (int ns = ns0; ns <= 2 * ns0; ++ns) {
+        int64_t rp = KEVD_ROUND_UP((n + ns - 1) / ns, C::KC);
+        if (rp < 2048) rp = 2048;
+        if (rp > rp_max) rp = rp_max;
+        int64_t nse = (n + rp - 1) / rp, wk = nse * ntiles;
+        double eff = (double)wk / (double)(slots * ((wk + slots - 1) / slots));
+        if (eff > best + 1e-9) {
+            best = eff;
+            nsplit = (int)nse;
+            rps = rp;
+        }
+    }
+    int nitems = ntiles * nsplit;
+    int *dTI, *dTJ;
+    double* part;
+    float* wt = nullptr;
+    KEVD_TRY(ctx_buf_t(ctx, wbuf, n, &wt));
+    k_cast<float><<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(w, n, wt);
+    KEVD_TRY(ctx_buf_t(ctx, "e_xTI", ntiles, &dTI));
+    KEVD_TRY(ctx_buf_t(ctx, "e_xTJ", ntiles, &dTJ));
+    KEVD_TRY(ctx_buf_t(ctx, "e_gpart", (size_t)nsplit * ntiles * C::BI * C::BJ, &part));
+    KEVD_HIP(ctx, hipMemcpyAsync(dTI, TI.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    KEVD_HIP(ctx, hipMemcpyAsync(dTJ, TJ.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    {
+        ProfScope ps(ctx, TC_GRAM);
+        k_gram_x6<<<nitems, C::NT, 0, ctx->stream>>>((const float*)src.F, src.ldf, n, r, wt, dTI, dTJ, ntiles,
+                                                      nitems, rps, part);
+    }
+    k_gram_reduce<C::BI, C::BJ, 1><<<dim3(ntiles, GRAM_REDUCE_SLICES), 256, 0, ctx->stream>>>(
+        part, nsplit, ntiles, dTI, dTJ, r, G, nullptr);
+    KEVD_HIP(ctx, hipGetLastError());
+    ctx->tm.gram_flop += (double)n * (double)r * (double)(r + 1);
+    return KEVD_OK;
+}
+
+// The bf16x6 Gram for FP32 factors (KEVD_FP32_GRAM=1 in the environment
+// selects the FP32-MFMA k_gram instead).
+static bool use_gram_x6()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("KEVD_FP32_GRAM");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
 template <typename T, int NG>
 kevd_status gram_rows(kevd_ctx* ctx, const Rows<T>& src, const double* w1, const double* w2, double* G1, double* G2)
 {
+    if constexpr (std::is_same<T, float>::value) {
+        if (use_gram_x6()) {
+            KEVD_TRY(gram_x6(ctx, src, w1, G1, "e_gw1"));
+            if (NG > 1) KEVD_TRY(gram_x6(ctx, src, w2, G2, "e_gw2"));
+            return KEVD_OK;
+        }
+    }
     typedef GramCfg<T, NG> C;
     int64_t n = src.n, r = src.r;
     int ti_n = (int)((r + C::BI - 1) / C::BI), tj_n = (int)((r + C::BJ - 1) / C::BJ);
