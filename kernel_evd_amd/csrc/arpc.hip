@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_gather_pivots(const T* __restrict__ Xsr
     gather_pivots_body<T>(Xsrc, nrm, dp, F, ldf, Si, st, XS, nrmS, FS, blockIdx.x);
 }
 
-template <typename T>
+template <typename T, bool X6>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_downdate(
     const T* __restrict__ X, int32_t dp, const double* __restrict__ nrm, T* __restrict__ F, int64_t ldf,
     double* __restrict__ d, int64_t n, const T* __restrict__ XS, const double* __restrict__ nrmS,
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     __shared__ __attribute__((aligned(16))) T lds[DL::ELEMS];
     __shared__ double rs[DL::WN][DL::BM];
     __shared__ DowndateEp<T> ep;
-    downdate_body<T>(X, dp, nrm, F, ldf, d, n, XS, nrmS, Si, FS, LinvT, st, gs, blockIdx.x, lds, rs, &ep);
+    downdate_body<T, X6>(X, dp, nrm, F, ldf, d, n, XS, nrmS, Si, FS, LinvT, st, gs, blockIdx.x, lds, rs, &ep);
 }
 
 __global__ __launch_bounds__(BMAX) void k_commit(ArpcState* st, const int64_t* __restrict__ Si, int64_t* pivots,
@@ -241,8 +241,12 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
         }
         {
             ProfScope ps(ctx, TC_DOWNDATE);
-            k_downdate<T><<<gdown, 256, 0, s>>>(Xsrc, ctx->dp, nrm, F, ctx->ldf, d, n, XS, nrmS, Si,
-                                                FS, LinvT, st, gs);
+            if (std::is_same<T, float>::value && use_bf16x6())
+                k_downdate<T, true><<<gdown, 256, 0, s>>>(Xsrc, ctx->dp, nrm, F, ctx->ldf, d, n, XS, nrmS, Si,
+                                                          FS, LinvT, st, gs);
+            else
+                k_downdate<T, false><<<gdown, 256, 0, s>>>(Xsrc, ctx->dp, nrm, F, ctx->ldf, d, n, XS, nrmS, Si,
+                                                           FS, LinvT, st, gs);
         }
         k_commit<<<1, BMAX, 0, s>>>(st, Si, pivots_dev, acc_dev);
     };

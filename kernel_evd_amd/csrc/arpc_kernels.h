@@ -18,6 +18,9 @@
 //   commit                           S = S u S_i, Nr += k   (1017-1019)
 #pragma once
 
+#include <type_traits>
+
+#include "bf16x6.h"
 #include "gemm.h"
 #include "internal.h"
 
@@ -681,7 +684,9 @@ __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64
 // This is row block blk (rows blk * BM ...) of one problem; it returns
 // early (before any barrier) when the problem is done or accepted nothing
 // this round.
-template <typename T>
+// X6: the overlap of an FP32 factor on the bf16 matrix cores (bf16x6
+// products, bf16x6.h) instead of FP32 MFMA.
+template <typename T, bool X6 = false>
 __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const double* __restrict__ nrm,
                                      T* __restrict__ F, int64_t ldf, double* __restrict__ d, int64_t n,
                                      const T* __restrict__ XS, const double* __restrict__ nrmS,
@@ -699,7 +704,15 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
     G g;
     dd_prologue<T>(g, X, dp, nrm, d, n, XS, nrmS, Si, k, gs, r0, lds, ep);
     // overlap with the current factor (ends with a barrier)
-    if (Nr > 0) g.mainloop_live(lds, F + r0 * ldf, ldf, n - r0, FS, ldf, BN, Nr);
+    if (Nr > 0) {
+        if constexpr (X6 && std::is_same<T, float>::value) {
+            static_assert(X6Stage<G>::BYTES <= DowndateLds<T>::ELEMS * (int)sizeof(T), "x6 stage fits");
+            x6_nt_mainloop_live<G>(g, reinterpret_cast<unsigned char*>(lds), F + r0 * ldf, ldf, n - r0, FS, ldf,
+                                   BN, Nr);
+        } else {
+            g.mainloop_live(lds, F + r0 * ldf, ldf, n - r0, FS, ldf, BN, Nr);
+        }
+    }
     dd_finish<T>(g, F, ldf, d, n, LinvT, k, Nr, r0, lds, rs, ep);
 }
 

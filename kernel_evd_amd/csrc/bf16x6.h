@@ -44,3 +44,134 @@ __device__ inline void mfma_bf16_32(floatx16& c, const bf16x8& a, const bf16x8& 
 {
     c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
+
+// v_mfma_f32_16x16x32_bf16: lane l supplies A(l % 16, 8 (l / 16) + e) and
+// B(8 (l / 16) + e, l % 16); the 16 x 16 FP32 result has the layout of
+// v_mfma_f32_16x16x4_f32 (value v of lane l is row 4 (l / 16) + v, column
+// l % 16).
+__device__ inline void mfma_bf16_16(floatx4& c, const bf16x8& a, const bf16x8& b)
+{
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// acc += A[0:BM, 0:K] B[0:BN, 0:K]^T for FP32 operands whose rows are
+// K-contiguous (as NTGemm::mainloop), with bf16x6 products on
+// v_mfma_f32_16x16x32_bf16, into the accumulators of an FP32 NTGemm G with
+// 16 x 16 tiles, one wave column (WN = 1) and KC = 32 (same accumulator
+// layout, so the caller's epilogues are unchanged).  NACT live column tiles.
+//
+// One LDS stage [part][row][32 + 8] bf16 for the BM + BN rows (80-byte rows:
+// the 16 rows an MFMA fragment read touches fall on distinct banks); each
+// thread splits the float4s it loaded into the three parts.  The next
+// chunk's loads are in flight in registers while the current one
+// multiplies; two barriers per chunk, the second workgroup on the CU
+// covering them.  Rows past arows / brows read as zero through buffer
+// resources, columns past K are zeroed.  Ends with a barrier.
+template <typename G>
+struct X6Stage {
+    static constexpr int BM = G::NBM, BN = G::NBN, KC = 32, MT = 16;
+    static constexpr int ROWB = (KC + 8) * 2;  // bytes per row of one part
+    static constexpr int PART = (BM + BN) * ROWB;
+    static constexpr int BYTES = 3 * PART;
+    static constexpr int NV = (BM + BN) * (KC / 4) / 256;  // float4 loads per thread
+    static_assert(G::NKC == KC && G::M::MT == MT && G::NBM == BM, "16x16 tiles, KC = 32");
+    static_assert(((BM + BN) * (KC / 4)) % 256 == 0 && (BM * (KC / 4)) % 256 == 0, "load shape");
+};
+
+template <typename G, int NACT>
+__device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A, int64_t lda, int64_t arows,
+                                        const float* B, int64_t ldb, int64_t brows, int64_t K)
+{
+    typedef X6Stage<G> S;
+    constexpr int BM = S::BM, BN = S::BN, KC = S::KC, NV = S::NV, TM = G::NTM;
+    constexpr int NVA = BM * (KC / 4) / 256;  // the first NVA loads are A rows
+    typedef __attribute__((ext_vector_type(4))) float f4;
+    const int64_t nk = (K + KC - 1) / KC, nfull = K / KC;
+    if (nk <= 0) return;
+    const __amdgpu_buffer_rsrc_t rsA = G::rows_rsrc(A, lda, arows, BM), rsB = G::rows_rsrc(B, ldb, brows, BN);
+    f4 rg[NV];
+    auto load = [&](int64_t c) {
+        const int64_t k0 = c * KC;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const int lin = threadIdx.x + 256 * j;
+            const bool isA = j < NVA;
+            const int row = (isA ? lin : lin - BM * (KC / 4)) / (KC / 4), cv = lin % (KC / 4);
+            const int64_t kk = k0 + 4 * cv;
+            const uint32_t off = (uint32_t)(((int64_t)row * (isA ? lda : ldb) + kk) * 4);
+            rg[j] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(isA ? rsA : rsB, off, 0, 0));
+            if (c >= nfull) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (kk + q >= K) rg[j][q] = 0.0f;
+            }
+        }
+    };
+    auto store = [&]() {
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            const int lin = threadIdx.x + 256 * j;
+            const int srow = lin / (KC / 4), cv = lin % (KC / 4);  // A rows then B rows
+            bf16x4 p0, p1, p2;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                __bf16 a0, a1, a2;
+                bf16_split3(rg[j][q], a0, a1, a2);
+                p0[q] = a0, p1[q] = a1, p2[q] = a2;
+            }
+            unsigned char* dst = lds + srow * S::ROWB + cv * 8;
+            *reinterpret_cast<bf16x4*>(dst) = p0;
+            *reinterpret_cast<bf16x4*>(dst + S::PART) = p1;
+            *reinterpret_cast<bf16x4*>(dst + 2 * S::PART) = p2;
+        }
+    };
+    auto compute = [&]() {
+        bf16x8 af[TM][3];
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                af[x][q] = *reinterpret_cast<const bf16x8*>(lds + q * S::PART +
+                                                            (g.wm * TM * 16 + x * 16 + g.li) * S::ROWB + g.h * 16);
+        // two column tiles at a time: four independent accumulators between
+        // two MFMAs into the same one
+#pragma unroll
+        for (int y = 0; y < NACT; y += 2) {
+            bf16x8 bf[2][3];
+#pragma unroll
+            for (int yo = 0; yo < 2; ++yo)
+                if (y + yo < NACT)
+#pragma unroll
+                    for (int q = 0; q < 3; ++q)
+                        bf[yo][q] = *reinterpret_cast<const bf16x8*>(
+                            lds + q * S::PART + (BM + (y + yo) * 16 + g.li) * S::ROWB + g.h * 16);
+#pragma unroll
+            for (int p = 0; p < X6_NPROD; ++p)
+#pragma unroll
+                for (int x = 0; x < TM; ++x)
+#pragma unroll
+                    for (int yo = 0; yo < 2; ++yo)
+                        if (y + yo < NACT) mfma_bf16_16(g.acc[x][y + yo], af[x][x6_a(p)], bf[yo][x6_b(p)]);
+        }
+    };
+    load(0);
+    for (int64_t c = 0; c < nk; ++c) {
+        if (c > 0) __syncthreads();  // every wave is done with chunk c - 1
+        store();
+        __syncthreads();
+        if (c + 1 < nk) load(c + 1);
+        compute();
+    }
+    __syncthreads();
+}
+
+template <typename G, int N = G::NTN>
+__device__ inline void x6_nt_mainloop_live(G& g, unsigned char* lds, const float* A, int64_t lda, int64_t arows,
+                                           const float* B, int64_t ldb, int64_t brows, int64_t K)
+{
+    if (g.nact == N)
+        x6_nt_mainloop_n<G, N>(g, lds, A, lda, arows, B, ldb, brows, K);
+    else if constexpr (N > 1)
+        x6_nt_mainloop_live<G, N - 1>(g, lds, A, lda, arows, B, ldb, brows, K);
+}

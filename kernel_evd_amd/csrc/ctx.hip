@@ -2,6 +2,7 @@
 // bandwidth estimator and the standalone kernel-block entry point.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -37,6 +38,15 @@ static kevd_status scratch_address(int device, void** out)
 }
 
 // ------------------------------------------------------------ errors, bufs
+
+bool use_bf16x6()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("KEVD_FP32_MFMA");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
 
 kevd_status kevd_fail(kevd_ctx* ctx, kevd_status s, const std::string& msg)
 {

@@ -119,7 +119,7 @@ __global__ void k_ens_zero(int* ctr)
 // item until the list is exhausted.  Items of finished problems (or of a
 // round that accepted nothing) return at once.  The loop's two barriers keep
 // the item slot consistent whether or not the body itself synchronised.
-template <typename T>
+template <typename T, bool X6>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ens_downdate(
     const EnsProb<T>* P, const int2* items, int nitems, int* ctr)
 {
@@ -136,8 +136,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         if (it >= nitems) break;
         int2 w = items[it];
         const EnsProb<T>& q = P[w.x];
-        downdate_body<T>(q.X, q.dp, q.nrm, q.F, q.ldf, q.d, q.n, q.XS, q.nrmS, q.Si, q.FS, q.LinvT, q.st, q.gs,
-                         w.y, lds, rs, &ep);
+        downdate_body<T, X6>(q.X, q.dp, q.nrm, q.F, q.ldf, q.d, q.n, q.XS, q.nrmS, q.Si, q.FS, q.LinvT, q.st,
+                             q.gs, w.y, lds, rs, &ep);
     }
 }
 
@@ -251,7 +251,10 @@ kevd_status ens_rounds(kevd_ctx* m, kevd_ctx** ctxs, int nprob, const kevd_arpc_
         k_ens_rejchol<T><<<nprob, 256, 0, s>>>(dP);
         k_ens_gather<T><<<dim3(BMAX, nprob), 256, 0, s>>>(dP);
         k_ens_zero<<<1, 64, 0, s>>>(ctr);
-        k_ens_downdate<T><<<gdown, 256, 0, s>>>(dP, ditems, nit, ctr);
+        if (std::is_same<T, float>::value && use_bf16x6())
+            k_ens_downdate<T, true><<<gdown, 256, 0, s>>>(dP, ditems, nit, ctr);
+        else
+            k_ens_downdate<T, false><<<gdown, 256, 0, s>>>(dP, ditems, nit, ctr);
         k_ens_commit<T><<<nprob, BMAX, 0, s>>>(dP);
         k_ens_all_done<T><<<1, 256, 0, s>>>(dP, nprob, flag);
     };

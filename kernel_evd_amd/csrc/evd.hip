@@ -1410,22 +1410,11 @@ kevd_status gram_x6(kevd_ctx* ctx, const Rows<float>& src, const double* w, doub
     return KEVD_OK;
 }
 
-// The bf16x6 Gram for FP32 factors (KEVD_FP32_GRAM=1 in the environment
-// selects the FP32-MFMA k_gram instead).
-static bool use_gram_x6()
-{
-    static const bool on = [] {
-        const char* e = std::getenv("KEVD_FP32_GRAM");
-        return !(e && e[0] == '1');
-    }();
-    return on;
-}
-
 template <typename T, int NG>
 kevd_status gram_rows(kevd_ctx* ctx, const Rows<T>& src, const double* w1, const double* w2, double* G1, double* G2)
 {
     if constexpr (std::is_same<T, float>::value) {
-        if (use_gram_x6()) {
+        if (use_bf16x6()) {
             KEVD_TRY(gram_x6(ctx, src, w1, G1, "e_gw1"));
             if (NG > 1) KEVD_TRY(gram_x6(ctx, src, w2, G2, "e_gw2"));
             return KEVD_OK;

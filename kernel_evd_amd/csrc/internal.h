@@ -132,6 +132,11 @@ void prof_resolve(kevd_ctx* ctx);
 // The gfx950 MFMA paths need a gfx950 device; reject anything else loudly.
 kevd_status ctx_check_arch(kevd_ctx* ctx);
 
+// The FP32 GEMMs of the mixed path (the downdate's overlap, the kEVD Grams)
+// run with bf16x6 products on the bf16 matrix cores; KEVD_FP32_MFMA=1 in the
+// environment selects plain FP32 MFMA instead (read once per process).
+bool use_bf16x6();
+
 // Cross-unit entry points.
 kevd_status pivot_cholesky(kevd_ctx* ctx, double* L);
 kevd_status kblock_gather(kevd_ctx* ctx, const int64_t* idx_dev, int64_t count,
