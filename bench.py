@@ -41,12 +41,19 @@ HBM_PEAK_GBS = 8000.0
 
 # Measured HBM bytes per launch at this workload (rocprofv3 FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, the factors a cold 2 GiB copy gives), from
-# profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md.
-TRAFFIC_BYTES = {"k_downdate<float>": 4.276e9, "k_gram<float,1>": 110.89e9,
-                 "k_sinkhorn_rows<float,8>": 8.507e9}
-TRAFFIC_SOURCE = "profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md"
+# profiles/traffic_k_downdate_x6_k_gram_x6_k_sinkhorn_ks_n1M_r2000.md (bf16x6
+# kernels) and profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md
+# (the FP32-MFMA kernels, KEVD_FP32_MFMA=1).
+TRAFFIC_BYTES = {"k_downdate<float, true>": 4.290e9, "k_gram_x6": 47.68e9,
+                 "k_downdate<float, false>": 4.276e9, "k_gram<float,1>": 110.89e9,
+                 "k_sinkhorn_rows<float,8>": 8.508e9}
 FP32_MFMA_PEAK_TF = 157.3
 FP64_MFMA_PEAK_TF = 78.6
+BF16_MFMA_PEAK_TF = 2500.0
+# The mixed path's downdate overlap and Grams issue six bf16 products per
+# FP32 product (bf16x6): their FP32-equivalent ceiling is the bf16 one / 6.
+X6_PEAK_TF = BF16_MFMA_PEAK_TF / 6.0
+X6 = os.environ.get("KEVD_FP32_MFMA") != "1"
 
 
 def log(*a):
@@ -483,6 +490,11 @@ def main():
     log("mixed: %.1f ms/step (arpc %.1f, evd %.1f); rank %d in %d rounds; downdate %.1f ms over %d launches; gram %.1f ms"
         % (ms_step, 1e3 * np.mean(t_arpc), 1e3 * np.mean(t_evd), a_mix.rank, a_mix.rounds, dd_ms, dd_launch, gr_ms))
 
+    dd_kernel = "k_downdate<float, %s>" % ("true" if X6 else "false")
+    gr_kernel = ("k_gram_x6" if X6 else "k_gram<float,%d>" % (1 if norm == "sinkhorn" else 2))
+    mfma_peak = X6_PEAK_TF if X6 else FP32_MFMA_PEAK_TF
+    mfma_meas = peaks["mfma_bf16_tflops"] / 6.0 if X6 else peaks["mfma_f32_tflops"]
+    gram_meas = peaks["mfma_bf16_32_tflops"] / 6.0 if X6 else peaks["mfma_f32_32_tflops"]
     evd_desc = ("symmetric Sinkhorn to max row-sum deviation %g" % SINKHORN_TOL if norm == "sinkhorn"
                 else "two-step degree normalisation (ks_kevd)")
     out = {
@@ -496,7 +508,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-mfma+fp64",
+        "dtype": ("fp32 as bf16x6 on bf16 MFMA+fp64" if X6 else "fp32-mfma+fp64"),
         "data": "synthetic: Kuramoto-Sivashinsky ETDRK4 trajectory generated in-run (L=22, 64 pts, seed 0)",
         "config": {"workload": "ARPC rank %d block %d + bistochastic kEVD top %d, %s"
                                % (args.rank, args.block, args.kout, evd_desc),
@@ -516,36 +528,49 @@ def main():
         "trace_err_mixed": float(a_mix.trace_err[-1]),
         "sinkhorn_iters": int(e_mix.sinkhorn_iters) if norm == "sinkhorn" else None,
         "rowsum_dev": float(e_mix.rowsum_dev),
-        "roofline": {"bound": "mfma", "kernel": "k_downdate<float>",
+        "roofline": {"bound": "mfma", "kernel": "k_downdate<float, %s>" % ("true" if X6 else "false"),
                      "achieved": round(dd_flop / (dd_ms * 1e-3) / 1e12, 3) if dd_ms > 0 else None,
-                     "peak": FP32_MFMA_PEAK_TF, "unit": "TFLOP/s",
-                     "frac": round(dd_flop / (dd_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4) if dd_ms > 0 else None,
-                     "traffic": TRAFFIC_BYTES["k_downdate<float>"] if args.n == 1 << 20 and args.rank == 2000 else None,
+                     "peak": round(mfma_peak, 2), "unit": "TFLOP/s",
+                     "frac": round(dd_flop / (dd_ms * 1e-3) / 1e12 / mfma_peak, 4) if dd_ms > 0 else None,
+                     "peak_note": ("FP32-equivalent algorithmic flop (2 n k (Nr + dim) + n k^2 per round); "
+                                   "the overlap runs as bf16x6, so the ceiling is the published dense bf16 "
+                                   "2.5 PFLOP/s / 6" if X6 else "FP32 MFMA, published 157.3 TFLOP/s"),
+                     "frac_of_fp32_mfma_peak": round(dd_flop / (dd_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4)
+                     if dd_ms > 0 else None,
+                     "traffic": TRAFFIC_BYTES.get(dd_kernel) if args.n == 1 << 20 and args.rank == 2000 else None,
                      "traffic_unit": "bytes per launch (average over the step's launches)",
-                     "traffic_source": TRAFFIC_SOURCE,
+                     "traffic_source": ("profiles/traffic_k_downdate_x6_k_gram_x6_k_sinkhorn_ks_n1M_r2000.md" if X6
+                                        else "profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md"),
                      "flop_per_step": dd_flop, "launches_per_step": dd_launch,
                      "algorithmic_bytes_per_launch": downdate_bytes(a_mix.accepted, args.n, 64) / max(dd_launch, 1),
                      "avg_launch_ms": round(dd_ms / max(dd_launch, 1), 4),
-                     "peak_measured": round(peaks["mfma_f32_tflops"], 2),
-                     "frac_of_measured": round(dd_flop / (dd_ms * 1e-3) / 1e12 / peaks["mfma_f32_tflops"], 4)
+                     "peak_measured": round(mfma_meas, 2),
+                     "frac_of_measured": round(dd_flop / (dd_ms * 1e-3) / 1e12 / mfma_meas, 4)
                      if dd_ms > 0 else None},
-        "gram_roofline": {"kernel": "k_gram<float,%d>" % (1 if norm == "sinkhorn" else 2),
+        "gram_roofline": {"kernel": gr_kernel,
                           "achieved": round(gr_flop / (gr_ms * 1e-3) / 1e12, 3) if gr_ms > 0 else None,
-                          "peak": FP32_MFMA_PEAK_TF, "unit": "TFLOP/s",
-                          "frac": round(gr_flop / (gr_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4) if gr_ms > 0 else None,
-                          "frac_of_measured": round(gr_flop / (gr_ms * 1e-3) / 1e12 / peaks["mfma_f32_tflops"], 4)
-                          if gr_ms > 0 else None},
+                          "peak": round(mfma_peak, 2), "unit": "TFLOP/s",
+                          "frac": round(gr_flop / (gr_ms * 1e-3) / 1e12 / mfma_peak, 4) if gr_ms > 0 else None,
+                          "frac_of_fp32_mfma_peak": round(gr_flop / (gr_ms * 1e-3) / 1e12 / FP32_MFMA_PEAK_TF, 4)
+                          if gr_ms > 0 else None,
+                          "frac_of_measured": round(gr_flop / (gr_ms * 1e-3) / 1e12 / gram_meas, 4)
+                          if gr_ms > 0 else None,
+                          "traffic": TRAFFIC_BYTES.get(gr_kernel) if args.n == 1 << 20 and args.rank == 2000
+                          and norm == "sinkhorn" else None,
+                          "algorithmic_bytes": 4.0 * args.n * a_mix.rank},
         # the step's three GEMMs on the factor together: the downdate that
         # builds F, the Gram F^T S^2 F (or both two-step Grams) and U = S F V
-        "factor_gemms": {"kernels": ["k_downdate<float>", "k_gram<float,%d>" % (1 if norm == "sinkhorn" else 2),
-                                     "k_ugemm<float>"],
+        "factor_gemms": {"kernels": [dd_kernel, gr_kernel, "k_ugemm<float>"],
                          "flop": dd_flop + gr_flop + 2.0 * args.n * a_mix.rank * (args.kout + 1),
                          "ms": round(dd_ms + gr_ms + agg["u_ms"] / steps, 3),
                          "achieved": round((dd_flop + gr_flop + 2.0 * args.n * a_mix.rank * (args.kout + 1))
                                            / ((dd_ms + gr_ms + agg["u_ms"] / steps) * 1e-3) / 1e12, 3)
                          if dd_ms + gr_ms > 0 else None,
-                         "unit": "TFLOP/s", "peak": FP32_MFMA_PEAK_TF},
+                         "unit": "TFLOP/s", "peak": FP32_MFMA_PEAK_TF,
+                         "peak_note": "FP32 MFMA peak, for comparison across builds (the U product stays FP32)"},
         "measured_peaks": {"mfma_f32_16x16x4_tflops": round(peaks["mfma_f32_tflops"], 2),
+                           "mfma_bf16_16x16x32_tflops": round(peaks["mfma_bf16_tflops"], 2),
+                           "mfma_bf16_32x32x16_tflops": round(peaks["mfma_bf16_32_tflops"], 2),
                            "mfma_f32_32x32x2_tflops": round(peaks["mfma_f32_32_tflops"], 2),
                            "mfma_f64_tflops": round(peaks["mfma_f64_tflops"], 2),
                            "exp_f32_per_s": peaks["exp_f32_per_s"],

@@ -341,6 +341,8 @@ typedef struct {
     double exp_f32_per_s;       /* v_exp_f32 results per second            */
     double hbm_copy_gbs;        /* device copy, read + written GB/s        */
     double hbm_read_gbs;        /* streaming read of one buffer, GB/s      */
+    double mfma_bf16_tflops;    /* v_mfma_f32_16x16x32_bf16, TFLOP/s        */
+    double mfma_bf16_32_tflops; /* v_mfma_f32_32x32x16_bf16, TFLOP/s        */
 } kevd_peaks;
 kevd_status kevd_measure_peaks(kevd_ctx* ctx, int64_t copy_bytes, kevd_peaks* out);
 

@@ -55,7 +55,8 @@ class Timings(C.Structure):
 
 class Peaks(C.Structure):
     _fields_ = [("mfma_f32_tflops", dbl), ("mfma_f32_32_tflops", dbl), ("mfma_f64_tflops", dbl),
-                ("exp_f32_per_s", dbl), ("hbm_copy_gbs", dbl), ("hbm_read_gbs", dbl)]
+                ("exp_f32_per_s", dbl), ("hbm_copy_gbs", dbl), ("hbm_read_gbs", dbl),
+                ("mfma_bf16_tflops", dbl), ("mfma_bf16_32_tflops", dbl)]
 
 
 # name -> (restype, argtypes); every function declared in include/kevd.h

@@ -60,9 +60,11 @@ __device__ inline void mfma_bf16_16(floatx4& c, const bf16x8& a, const bf16x8& b
 // 16 x 16 tiles, one wave column (WN = 1) and KC = 32 (same accumulator
 // layout, so the caller's epilogues are unchanged).  NACT live column tiles.
 //
-// One LDS stage [part][row][32 + 8] bf16 for the BM + BN rows (80-byte rows:
-// the 16 rows an MFMA fragment read touches fall on distinct banks); each
-// thread splits the float4s it loaded into the three parts.  The next
+// One LDS stage [part][row][32] bf16 for the BM + BN rows.  A row's four
+// 16-byte slots (8 K values each) are XOR-swizzled by (row >> 2) & 3, which
+// makes both accesses conflict-free: a fragment read (16 rows, one slot
+// each) and a split store (4 rows x 8 eight-byte halves per 32 lanes).
+// Each thread splits the float4s it loaded into the three parts.  The next
 // chunk's loads are in flight in registers while the current one
 // multiplies; two barriers per chunk, the second workgroup on the CU
 // covering them.  Rows past arows / brows read as zero through buffer
@@ -70,7 +72,7 @@ __device__ inline void mfma_bf16_16(floatx4& c, const bf16x8& a, const bf16x8& b
 template <typename G>
 struct X6Stage {
     static constexpr int BM = G::NBM, BN = G::NBN, KC = 32, MT = 16;
-    static constexpr int ROWB = (KC + 8) * 2;  // bytes per row of one part
+    static constexpr int ROWB = KC * 2;  // bytes per row of one part
     static constexpr int PART = (BM + BN) * ROWB;
     static constexpr int BYTES = 3 * PART;
     static constexpr int NV = (BM + BN) * (KC / 4) / 256;  // float4 loads per thread
@@ -120,12 +122,14 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
                 bf16_split3(rg[j][q], a0, a1, a2);
                 p0[q] = a0, p1[q] = a1, p2[q] = a2;
             }
-            unsigned char* dst = lds + srow * S::ROWB + cv * 8;
+            unsigned char* dst = lds + srow * S::ROWB + (((cv >> 1) ^ ((srow >> 2) & 3)) << 4) + (cv & 1) * 8;
             *reinterpret_cast<bf16x4*>(dst) = p0;
             *reinterpret_cast<bf16x4*>(dst + S::PART) = p1;
             *reinterpret_cast<bf16x4*>(dst + 2 * S::PART) = p2;
         }
     };
+    // a fragment row is 16 t + li: its slot swizzle depends on li only
+    const int slot = (g.h ^ ((g.li >> 2) & 3)) << 4;
     auto compute = [&]() {
         bf16x8 af[TM][3];
 #pragma unroll
@@ -133,7 +137,7 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
 #pragma unroll
             for (int q = 0; q < 3; ++q)
                 af[x][q] = *reinterpret_cast<const bf16x8*>(lds + q * S::PART +
-                                                            (g.wm * TM * 16 + x * 16 + g.li) * S::ROWB + g.h * 16);
+                                                            (g.wm * TM * 16 + x * 16 + g.li) * S::ROWB + slot);
         // two column tiles at a time: four independent accumulators between
         // two MFMAs into the same one
 #pragma unroll
@@ -145,7 +149,7 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
 #pragma unroll
                     for (int q = 0; q < 3; ++q)
                         bf[yo][q] = *reinterpret_cast<const bf16x8*>(
-                            lds + q * S::PART + (BM + (y + yo) * 16 + g.li) * S::ROWB + g.h * 16);
+                            lds + q * S::PART + (BM + (y + yo) * 16 + g.li) * S::ROWB + slot);
 #pragma unroll
             for (int p = 0; p < X6_NPROD; ++p)
 #pragma unroll
@@ -153,8 +157,12 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
 #pragma unroll
                     for (int yo = 0; yo < 2; ++yo)
                         if (y + yo < NACT) mfma_bf16_16(g.acc[x][y + yo], af[x][x6_a(p)], bf[yo][x6_b(p)]);
+            // one column pair's fragments live at a time
+            __builtin_amdgcn_sched_barrier(0);
         }
     };
+    // A second register set (loads two chunks ahead) does not fit beside
+    // the accumulators and fragments at two waves per SIMD (it spills).
     load(0);
     for (int64_t c = 0; c < nk; ++c) {
         if (c > 0) __syncthreads();  // every wave is done with chunk c - 1

@@ -8,7 +8,8 @@
 // this has no counterpart there.
 //
 //   MFMA      waves issue chains of independent MFMAs on register operands
-//             (no memory traffic); 2 M N K flop per instruction.
+//             (no memory traffic); 2 M N K flop per instruction: FP32,
+//             FP64 and the bf16 forms the bf16x6 GEMMs use.
 //   exp       16 independent v_exp_f32 recurrences per lane, x <- 2^-x (the
 //             negation is a free source modifier, the value stays in (0, 1]).
 //   HBM copy  a 16-byte-per-lane grid-stride copy of a buffer far larger
@@ -16,6 +17,7 @@
 //   HBM read  the same grid-stride pattern reading only (an XOR of the
 //             words per lane is written once), the ceiling of the
 //             read-dominated streaming passes.
+#include "bf16x6.h"
 #include "internal.h"
 
 namespace {
@@ -50,6 +52,53 @@ __global__ __launch_bounds__(PK_THREADS) void k_peak_f32_32(int iters, float* ou
     for (int i = 0; i < iters; ++i)
 #pragma unroll
         for (int j = 0; j < CH; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[j], 0, 0, 0);
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < CH; ++j)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) s += acc[j][v];
+    out[blockIdx.x * PK_THREADS + threadIdx.x] = s;
+}
+
+// bf16 operands as the bf16x6 GEMMs issue them (v_mfma_f32_16x16x32_bf16 in
+// the downdate, v_mfma_f32_32x32x16_bf16 in the Gram)
+__global__ __launch_bounds__(PK_THREADS) void k_peak_bf16_16(int iters, float* out)
+{
+    floatx4 acc[PK_CHAINS];
+#pragma unroll
+    for (int j = 0; j < PK_CHAINS; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 a, b;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        a[e] = (__bf16)(1e-3f * (float)((threadIdx.x + e) & 63));
+        b[e] = (__bf16)1e-3f;
+    }
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+        for (int j = 0; j < PK_CHAINS; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < PK_CHAINS; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+    out[blockIdx.x * PK_THREADS + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(PK_THREADS) void k_peak_bf16_32(int iters, float* out)
+{
+    constexpr int CH = PK_CHAINS / 4;
+    floatx16 acc[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[j][v] = 0.f;
+    bf16x8 a, b;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        a[e] = (__bf16)(1e-3f * (float)((threadIdx.x + e) & 63));
+        b[e] = (__bf16)1e-3f;
+    }
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+        for (int j = 0; j < CH; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
     float s = 0.f;
 #pragma unroll
     for (int j = 0; j < CH; ++j)
@@ -163,6 +212,11 @@ extern "C" kevd_status kevd_measure_peaks(kevd_ctx* ctx, int64_t copy_bytes, kev
     out->mfma_f32_tflops = waves * it32 * PK_CHAINS * (2.0 * 16 * 16 * 4) / t * 1e-12;
     KEVD_TRY(best_time(ctx, 3, [&] { k_peak_f32_32<<<nblk, PK_THREADS, 0, s>>>(it32, sink); }, &t));
     out->mfma_f32_32_tflops = waves * it32 * (PK_CHAINS / 4) * (2.0 * 32 * 32 * 2) / t * 1e-12;
+    const int itb = 12000;
+    KEVD_TRY(best_time(ctx, 3, [&] { k_peak_bf16_16<<<nblk, PK_THREADS, 0, s>>>(itb, sink); }, &t));
+    out->mfma_bf16_tflops = waves * itb * PK_CHAINS * (2.0 * 16 * 16 * 32) / t * 1e-12;
+    KEVD_TRY(best_time(ctx, 3, [&] { k_peak_bf16_32<<<nblk, PK_THREADS, 0, s>>>(itb, sink); }, &t));
+    out->mfma_bf16_32_tflops = waves * itb * (PK_CHAINS / 4) * (2.0 * 32 * 32 * 16) / t * 1e-12;
     KEVD_TRY(best_time(ctx, 3, [&] { k_peak_f64<<<nblk, PK_THREADS, 0, s>>>(it64, sink); }, &t));
     out->mfma_f64_tflops = waves * it64 * (PK_CHAINS / 2) * (2.0 * 16 * 16 * 4) / t * 1e-12;
     KEVD_TRY(best_time(ctx, 3, [&] { k_peak_exp<<<nblk, PK_THREADS, 0, s>>>(itexp, sink); }, &t));
