@@ -79,13 +79,13 @@ __global__ __launch_bounds__(256) void k_rejchol(const double* __restrict__ H, A
     rejchol_body<T>(H, st, seed, tol, inject, Sprime, Si, LinvT, Linv64);
 }
 
-template <typename T>
+template <typename T, bool X6>
 __global__ __launch_bounds__(256) void k_gather_pivots(const T* __restrict__ Xsrc, const double* __restrict__ nrm,
                                                        int32_t dp, const T* __restrict__ F, int64_t ldf,
                                                        const int64_t* __restrict__ Si, const ArpcState* st,
                                                        T* XS, double* nrmS, T* FS)
 {
-    gather_pivots_body<T>(Xsrc, nrm, dp, F, ldf, Si, st, XS, nrmS, FS, blockIdx.x);
+    gather_pivots_body<T, X6>(Xsrc, nrm, dp, F, ldf, Si, st, XS, nrmS, FS, blockIdx.x);
 }
 
 template <typename T, bool X6>
@@ -222,6 +222,7 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
     for (int i = 0; i < 4; ++i) KEVD_HIP(ctx, hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
     kevd_status status = KEVD_OK;
     unsigned gdown = (unsigned)((n + Cfg<T>::BM - 1) / Cfg<T>::BM);
+    const bool x6 = std::is_same<T, float>::value && use_bf16x6();
     auto round = [&]() {
         {
             ProfScope ps(ctx, TC_SAMPLE);
@@ -236,12 +237,16 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
                                                                   Sprime, st, gs.neg_inv_eps, H);
             k_rejchol<T><<<1, 256, 0, s>>>(H, st, p->seed, p->tol, inject_dev != nullptr, Sprime, Si, LinvT,
                                            Linv64);
-            k_gather_pivots<T><<<BMAX, 256, 0, s>>>(Xsrc, nrm, ctx->dp, F, ctx->ldf, Si, st, XS,
-                                                    nrmS, FS);
+            if (x6)
+                k_gather_pivots<T, true><<<BMAX, 256, 0, s>>>(Xsrc, nrm, ctx->dp, F, ctx->ldf, Si, st, XS,
+                                                              nrmS, FS);
+            else
+                k_gather_pivots<T, false><<<BMAX, 256, 0, s>>>(Xsrc, nrm, ctx->dp, F, ctx->ldf, Si, st, XS,
+                                                               nrmS, FS);
         }
         {
             ProfScope ps(ctx, TC_DOWNDATE);
-            if (std::is_same<T, float>::value && use_bf16x6())
+            if (x6)
                 k_downdate<T, true><<<gdown, 256, 0, s>>>(Xsrc, ctx->dp, nrm, F, ctx->ldf, d, n, XS, nrmS, Si,
                                                           FS, LinvT, st, gs);
             else
@@ -509,7 +514,11 @@ kevd_status arpc_prepare(kevd_ctx* ctx, const kevd_arpc_params* p, int max_round
     KEVD_TRY(ctx_buf_t(ctx, "a_Linv64", BMAX * BMAX, &B->Linv64));
     KEVD_TRY(ctx_buf_t(ctx, "a_nrmS", BMAX, &B->nrmS));
     KEVD_TRY(ctx_buf(ctx, "a_XS", (size_t)BMAX * ctx->dp * esz, &B->XS));
-    KEVD_TRY(ctx_buf(ctx, "a_FS", (size_t)BMAX * (p->regenerate ? 64 : ldf) * esz, &B->FS));
+    // mixed F form: the pivot rows' bf16x6 parts follow them
+    KEVD_TRY(ctx_buf(ctx, "a_FS",
+                     p->regenerate ? (size_t)BMAX * 64 * esz
+                                   : (p->prec == KEVD_MIXED ? x6_fs_bytes(ldf) : (size_t)BMAX * ldf * esz),
+                     &B->FS));
     KEVD_TRY(ctx_buf(ctx, "a_LinvT", (size_t)BMAX * BMAX * esz, &B->LinvT));
 
     ArpcState init{};

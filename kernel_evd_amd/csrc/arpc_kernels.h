@@ -490,9 +490,19 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
     if (tid == 0) st->k = k;
 }
 
+// The bf16x6 parts of the pivot rows follow FS (BMAX x ldf FP32) in the
+// same buffer: part q at x6_fsx(FS, ldf) + q * BMAX * ldf.
+__host__ __device__ inline size_t x6_fs_bytes(int64_t ldf) { return (size_t)BMAX * ldf * (4 + 3 * 2); }
+__device__ inline __bf16* x6_fsx(float* FS, int64_t ldf) { return reinterpret_cast<__bf16*>(FS + (int64_t)BMAX * ldf); }
+__device__ inline const __bf16* x6_fsx(const float* FS, int64_t ldf)
+{
+    return reinterpret_cast<const __bf16*>(FS + (int64_t)BMAX * ldf);
+}
+
 // Pivot rows in the GEMM precision: X(S_i,:), |x|^2, F(S_i, 0:Nr).  Rows
-// past k are zero.
-template <typename T>
+// past k are zero.  X6: also F(S_i,:)'s bf16x6 parts, zero from Nr up to a
+// multiple of 32.
+template <typename T, bool X6 = false>
 __device__ inline void gather_pivots_body(const T* __restrict__ Xsrc, const double* __restrict__ nrm,
                                           int32_t dp, const T* __restrict__ F, int64_t ldf,
                                           const int64_t* __restrict__ Si, const ArpcState* st, T* XS,
@@ -505,6 +515,18 @@ __device__ inline void gather_pivots_body(const T* __restrict__ Xsrc, const doub
     for (int j = threadIdx.x; j < dp; j += 256) XS[c * dp + j] = s >= 0 ? Xsrc[s * dp + j] : (T)0;
     if (threadIdx.x == 0) nrmS[c] = s >= 0 ? nrm[s] : 0.0;
     for (int64_t m = threadIdx.x; m < Nr; m += 256) FS[c * ldf + m] = s >= 0 ? F[s * ldf + m] : (T)0;
+    if constexpr (X6 && std::is_same<T, float>::value) {
+        __bf16* fx = x6_fsx(FS, ldf);
+        const int64_t ps = (int64_t)BMAX * ldf, kp = (Nr + 31) / 32 * 32;
+        for (int64_t m = threadIdx.x; m < kp; m += 256) {
+            float v = s >= 0 && m < Nr ? F[s * ldf + m] : 0.0f;
+            __bf16 h, md, l;
+            bf16_split3(v, h, md, l);
+            fx[c * ldf + m] = h;
+            fx[ps + c * ldf + m] = md;
+            fx[2 * ps + c * ldf + m] = l;
+        }
+    }
 }
 
 // ----------------------------------------------------------------- downdate
@@ -706,9 +728,9 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
     // overlap with the current factor (ends with a barrier)
     if (Nr > 0) {
         if constexpr (X6 && std::is_same<T, float>::value) {
-            static_assert(X6Stage<G>::BYTES <= DowndateLds<T>::ELEMS * (int)sizeof(T), "x6 stage fits");
-            x6_nt_mainloop_live<G>(g, reinterpret_cast<unsigned char*>(lds), F + r0 * ldf, ldf, n - r0, FS, ldf,
-                                   BN, Nr);
+            static_assert(X6Stage<G>::BYTES <= DowndateLds<T>::ELEMS * (int)sizeof(T), "the x6 stage fits");
+            x6_nt_mainloop_live<G>(g, reinterpret_cast<unsigned char*>(lds), F + r0 * ldf, ldf, n - r0,
+                                   x6_fsx(FS, ldf), ldf, (int64_t)BMAX * ldf, Nr);
         } else {
             g.mainloop_live(lds, F + r0 * ldf, ldf, n - r0, FS, ldf, BN, Nr);
         }

@@ -54,67 +54,90 @@ __device__ inline void mfma_bf16_16(floatx4& c, const bf16x8& a, const bf16x8& b
     c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// acc += A[0:BM, 0:K] B[0:BN, 0:K]^T for FP32 operands whose rows are
-// K-contiguous (as NTGemm::mainloop), with bf16x6 products on
+// acc += A[0:BM, 0:K] B[0:BN, 0:K]^T with bf16x6 products on
 // v_mfma_f32_16x16x32_bf16, into the accumulators of an FP32 NTGemm G with
 // 16 x 16 tiles, one wave column (WN = 1) and KC = 32 (same accumulator
 // layout, so the caller's epilogues are unchanged).  NACT live column tiles.
+// A is FP32 with K-contiguous rows (as NTGemm::mainloop); B comes already
+// split: part q at Bx + q * part_stride, row j at + j * ldb, K-contiguous
+// and zero from K up to a multiple of 32 (the downdate's pivot rows, split
+// once per round when they are gathered).
 //
 // One LDS stage [part][row][32] bf16 for the BM + BN rows.  A row's four
 // 16-byte slots (8 K values each) are XOR-swizzled by (row >> 2) & 3, which
-// makes both accesses conflict-free: a fragment read (16 rows, one slot
-// each) and a split store (4 rows x 8 eight-byte halves per 32 lanes).
-// Each thread splits the float4s it loaded into the three parts.  The next
-// chunk's loads are in flight in registers while the current one
-// multiplies; two barriers per chunk, the second workgroup on the CU
-// covering them.  Rows past arows / brows read as zero through buffer
-// resources, columns past K are zeroed.  Ends with a barrier.
+// makes every access conflict-free: a fragment read (16 rows, one slot
+// each), an A split store (4 rows x 8 eight-byte halves per 32 lanes) and a
+// B copy (4 rows x 4 slots per 16 lanes).  The next chunk's loads are in
+// flight in registers while the current one multiplies; two barriers per
+// chunk, the second workgroup on the CU covering them.  A's rows past arows
+// read as zero through a buffer resource and its columns past K are zeroed.
+// Ends with a barrier.  (B's fragments read straight from L2 by every wave
+// instead of through LDS moved four times the bytes and took 60 ms against
+// 37.5 at N = 1M, r = 2000.)
 template <typename G>
 struct X6Stage {
     static constexpr int BM = G::NBM, BN = G::NBN, KC = 32, MT = 16;
     static constexpr int ROWB = KC * 2;  // bytes per row of one part
     static constexpr int PART = (BM + BN) * ROWB;
     static constexpr int BYTES = 3 * PART;
-    static constexpr int NV = (BM + BN) * (KC / 4) / 256;  // float4 loads per thread
+    static constexpr int NVA = BM * (KC / 4) / 256;      // A float4 loads per thread
+    static constexpr int NVB = 3 * BN * (KC / 8) / 256;  // B 16-byte part loads per thread
     static_assert(G::NKC == KC && G::M::MT == MT && G::NBM == BM, "16x16 tiles, KC = 32");
-    static_assert(((BM + BN) * (KC / 4)) % 256 == 0 && (BM * (KC / 4)) % 256 == 0, "load shape");
+    static_assert((BM * (KC / 4)) % 256 == 0 && (3 * BN * (KC / 8)) % 256 == 0, "load shape");
 };
 
 template <typename G, int NACT>
 __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A, int64_t lda, int64_t arows,
-                                        const float* B, int64_t ldb, int64_t brows, int64_t K)
+                                        const __bf16* Bx, int64_t ldb, int64_t part_stride, int64_t K)
 {
     typedef X6Stage<G> S;
-    constexpr int BM = S::BM, BN = S::BN, KC = S::KC, NV = S::NV, TM = G::NTM;
-    constexpr int NVA = BM * (KC / 4) / 256;  // the first NVA loads are A rows
+    constexpr int BM = S::BM, KC = S::KC, NVA = S::NVA, NVB = S::NVB, TM = G::NTM;
     typedef __attribute__((ext_vector_type(4))) float f4;
+    typedef __attribute__((ext_vector_type(4))) unsigned u4;
     const int64_t nk = (K + KC - 1) / KC, nfull = K / KC;
     if (nk <= 0) return;
-    const __amdgpu_buffer_rsrc_t rsA = G::rows_rsrc(A, lda, arows, BM), rsB = G::rows_rsrc(B, ldb, brows, BN);
-    f4 rg[NV];
+    const __amdgpu_buffer_rsrc_t rsA = G::rows_rsrc(A, lda, arows, BM);
+    const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<__bf16*>(Bx), (short)0, (int)(3 * part_stride * 2), 0x00020000);
+    f4 rg[NVA];
+    u4 rb[NVB];
     auto load = [&](int64_t c) {
         const int64_t k0 = c * KC;
 #pragma unroll
-        for (int j = 0; j < NV; ++j) {
+        for (int j = 0; j < NVA; ++j) {
             const int lin = threadIdx.x + 256 * j;
-            const bool isA = j < NVA;
-            const int row = (isA ? lin : lin - BM * (KC / 4)) / (KC / 4), cv = lin % (KC / 4);
+            const int row = lin / (KC / 4), cv = lin % (KC / 4);
             const int64_t kk = k0 + 4 * cv;
-            const uint32_t off = (uint32_t)(((int64_t)row * (isA ? lda : ldb) + kk) * 4);
-            rg[j] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(isA ? rsA : rsB, off, 0, 0));
+            rg[j] = __builtin_bit_cast(
+                f4, __builtin_amdgcn_raw_buffer_load_b128(rsA, (uint32_t)(((int64_t)row * lda + kk) * 4), 0, 0));
             if (c >= nfull) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     if (kk + q >= K) rg[j][q] = 0.0f;
             }
         }
+        // B: part q, row, 16-byte slot (8 K values); no mask (zero past K)
+#pragma unroll
+        for (int j = 0; j < NVB; ++j) {
+            const int lin = threadIdx.x + 256 * j;
+            const int q = lin / (S::BN * 4), rem = lin % (S::BN * 4), row = rem / 4, sl = rem % 4;
+            rb[j] = __builtin_amdgcn_raw_buffer_load_b128(
+                rsB, (uint32_t)((q * part_stride + (int64_t)row * ldb + k0 + 8 * sl) * 2), 0, 0);
+        }
     };
     auto store = [&]() {
         typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-        for (int j = 0; j < NV; ++j) {
+        for (int j = 0; j < NVB; ++j) {
             const int lin = threadIdx.x + 256 * j;
-            const int srow = lin / (KC / 4), cv = lin % (KC / 4);  // A rows then B rows
+            const int q = lin / (S::BN * 4), rem = lin % (S::BN * 4), row = rem / 4, sl = rem % 4;
+            const int srow = BM + row;
+            *reinterpret_cast<u4*>(lds + q * S::PART + srow * S::ROWB + ((sl ^ ((srow >> 2) & 3)) << 4)) = rb[j];
+        }
+#pragma unroll
+        for (int j = 0; j < NVA; ++j) {
+            const int lin = threadIdx.x + 256 * j;
+            const int srow = lin / (KC / 4), cv = lin % (KC / 4);  // A rows
             bf16x4 p0, p1, p2;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -176,10 +199,10 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
 
 template <typename G, int N = G::NTN>
 __device__ inline void x6_nt_mainloop_live(G& g, unsigned char* lds, const float* A, int64_t lda, int64_t arows,
-                                           const float* B, int64_t ldb, int64_t brows, int64_t K)
+                                           const __bf16* Bx, int64_t ldb, int64_t part_stride, int64_t K)
 {
     if (g.nact == N)
-        x6_nt_mainloop_n<G, N>(g, lds, A, lda, arows, B, ldb, brows, K);
+        x6_nt_mainloop_n<G, N>(g, lds, A, lda, arows, Bx, ldb, part_stride, K);
     else if constexpr (N > 1)
-        x6_nt_mainloop_live<G, N - 1>(g, lds, A, lda, arows, B, ldb, brows, K);
+        x6_nt_mainloop_live<G, N - 1>(g, lds, A, lda, arows, Bx, ldb, part_stride, K);
 }

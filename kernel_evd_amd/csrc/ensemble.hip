@@ -103,11 +103,11 @@ __global__ __launch_bounds__(256) void k_ens_rejchol(const EnsProb<T>* P)
     rejchol_body<T>(q.H, q.st, q.seed, q.tol, 0, q.Sprime, q.Si, q.LinvT, q.Linv64);
 }
 
-template <typename T>
+template <typename T, bool X6>
 __global__ __launch_bounds__(256) void k_ens_gather(const EnsProb<T>* P)
 {
     const EnsProb<T>& q = P[blockIdx.y];
-    gather_pivots_body<T>(q.X, q.nrm, q.dp, q.F, q.ldf, q.Si, q.st, q.XS, q.nrmS, q.FS, blockIdx.x);
+    gather_pivots_body<T, X6>(q.X, q.nrm, q.dp, q.F, q.ldf, q.Si, q.st, q.XS, q.nrmS, q.FS, blockIdx.x);
 }
 
 __global__ void k_ens_zero(int* ctr)
@@ -243,18 +243,21 @@ kevd_status ens_rounds(kevd_ctx* m, kevd_ctx** ctxs, int nprob, const kevd_arpc_
     const unsigned nch = (unsigned)chunks.size();
     const int nit = (int)items.size();
     const unsigned gdown = (unsigned)std::min<int64_t>(nit, 2 * (int64_t)m->num_cu);
+    const bool x6 = std::is_same<T, float>::value && use_bf16x6();
     auto round = [&]() {
         k_ens_reduce_d<T><<<nch, 256, 0, s>>>(dP, dchunks, 0);
         k_ens_weights_scan<T><<<nch, 256, 0, s>>>(dP, dchunks);
         k_ens_draw<T><<<nprob, 256, 0, s>>>(dP);
         k_ens_form_H<T><<<dim3(BMAX / 16, BMAX / 16, nprob), FORMH_THREADS, 0, s>>>(dP);
         k_ens_rejchol<T><<<nprob, 256, 0, s>>>(dP);
-        k_ens_gather<T><<<dim3(BMAX, nprob), 256, 0, s>>>(dP);
         k_ens_zero<<<1, 64, 0, s>>>(ctr);
-        if (std::is_same<T, float>::value && use_bf16x6())
+        if (x6) {
+            k_ens_gather<T, true><<<dim3(BMAX, nprob), 256, 0, s>>>(dP);
             k_ens_downdate<T, true><<<gdown, 256, 0, s>>>(dP, ditems, nit, ctr);
-        else
+        } else {
+            k_ens_gather<T, false><<<dim3(BMAX, nprob), 256, 0, s>>>(dP);
             k_ens_downdate<T, false><<<gdown, 256, 0, s>>>(dP, ditems, nit, ctr);
+        }
         k_ens_commit<T><<<nprob, BMAX, 0, s>>>(dP);
         k_ens_all_done<T><<<1, 256, 0, s>>>(dP, nprob, flag);
     };
