@@ -287,6 +287,8 @@ typedef struct {
     int32_t eig_full_fallback; /* 1 if subspace iteration had not converged
                                   after its step limit and the full
                                   divide-and-conquer solve gave the pairs  */
+    int32_t eig_batched;      /* 1 if the pairs came from the ensemble's
+                                 batched solve                              */
 } kevd_timings;
 
 /* Enable per-kernel HIP event timing on the context's stream (adds one
@@ -311,8 +313,15 @@ kevd_status kevd_get_timings(const kevd_ctx* ctx, kevd_timings* out);
 kevd_status kevd_ensemble_arpc(kevd_ctx** ctxs, int32_t nprob, const kevd_arpc_params* params,
                                kevd_arpc_result* results, int32_t* used_graph);
 /* kevd_ensemble_evd: kevd_evd on every context with the same parameters,
- * on `workers` host threads (<= 0: 8), each problem on its own context
- * stream.  results[i] as kevd_evd's; errors are reported on ctxs[0]. */
+ * each problem on its own context stream.  With leading eigenpairs only
+ * (all_eigenvalues = 0) and 2..128 problems, every problem runs on its own
+ * host thread and the leading-eigenpair solves of all problems run as one
+ * batched subspace iteration (strided-batched GEMMs, one-workgroup Cholesky
+ * per problem, batched Jacobi Rayleigh-Ritz, over W zero-padded to the
+ * largest rank); a problem the batch does not converge solves alone.
+ * Otherwise, or with KEVD_ENS_EIG_BATCH=0 in the environment, the problems
+ * run on `workers` host threads (<= 0: 8) with their own solves.
+ * results[i] as kevd_evd's; errors are reported on ctxs[0]. */
 kevd_status kevd_ensemble_evd(kevd_ctx** ctxs, int32_t nprob, const kevd_evd_params* p,
                               kevd_evd_result* results, int32_t workers);
 /* kevd_autotune_block: the block size among candidates[0..ncand) with the

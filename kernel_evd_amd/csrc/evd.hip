@@ -20,6 +20,8 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
@@ -38,17 +40,18 @@ namespace {
 constexpr int COLSUM_ROWS = 512;  // rows of F per column-sum block
 
 // part[b][j] = sum_{rows of block b} w_i F(i, j), FP64 (w = 1 if null).
-// Each thread owns 16-byte column groups; a block streams COLSUM_ROWS
-// contiguous rows, so every wave load is one contiguous 4 KiB segment.
+// Each thread owns 16-byte column groups; a block streams `rows`
+// contiguous rows (COLSUM_ROWS, fewer for a small n so that the grid still
+// fills the device), so every wave load is one contiguous 4 KiB segment.
 template <typename T>
 __global__ __launch_bounds__(256) void k_colsum(const T* __restrict__ F, int64_t ldf, int64_t n,
                                                 int64_t r, const double* __restrict__ w,
-                                                double* part, int64_t rpad)
+                                                double* part, int64_t rpad, int rows)
 {
     constexpr int V = 16 / (int)sizeof(T);
     typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type vec_t;
-    int64_t r0 = (int64_t)blockIdx.x * COLSUM_ROWS;
-    int64_t r1 = r0 + COLSUM_ROWS < n ? r0 + COLSUM_ROWS : n;
+    int64_t r0 = (int64_t)blockIdx.x * rows;
+    int64_t r1 = r0 + rows < n ? r0 + rows : n;
     for (int64_t cg = (int64_t)threadIdx.x * V; cg < r; cg += 256 * V) {
         double acc[V];
 #pragma unroll
@@ -551,6 +554,12 @@ template <typename T, int NG> struct GramCfg;
 #endif
 // workgroups per tile in the split reduction
 constexpr int GRAM_REDUCE_SLICES = 8;
+// slices for ntiles tiles: at least 8, and enough workgroups for 4 per CU
+inline int gram_reduce_slices(kevd_ctx* ctx, int ntiles)
+{
+    int s = (4 * ctx->num_cu + ntiles - 1) / ntiles;
+    return std::max(GRAM_REDUCE_SLICES, std::min(s, 64));
+}
 template <int NG> struct GramCfg<float, NG> {
     static constexpr int BI = 128, BJ = 128, KC = 32, TM = 2, TN = 2, PROMOTE = KEVD_GRAM_PROMOTE;
     static constexpr int NT = 256, WGM = 2, WGN = 2, WG_PER_CU = 2, WPE = 2, PB = 16;
@@ -1291,21 +1300,25 @@ template <typename T> struct Rows {
 // part[b][j] for the COLSUM_ROWS-row blocks of src (block b of src is
 // part row b): column partials of w^T src (w null: ones).
 template <typename T>
-void colsum_part(kevd_ctx* ctx, const Rows<T>& src, const double* w, double* part, int64_t rpad)
+void colsum_part(kevd_ctx* ctx, const Rows<T>& src, const double* w, double* part, int64_t rpad,
+                 int rows = COLSUM_ROWS)
 {
-    int nb = (int)((src.n + COLSUM_ROWS - 1) / COLSUM_ROWS);
-    k_colsum<T><<<nb, 256, 0, ctx->stream>>>(src.F, src.ldf, src.n, src.r, w, part, rpad);
+    int nb = (int)((src.n + rows - 1) / rows);
+    k_colsum<T><<<nb, 256, 0, ctx->stream>>>(src.F, src.ldf, src.n, src.r, w, part, rpad, rows);
 }
 
 template <typename T>
 kevd_status colsum_rows(kevd_ctx* ctx, const Rows<T>& src, const double* w, double* c)
 {
-    int nb = (int)((src.n + COLSUM_ROWS - 1) / COLSUM_ROWS);
+    // at least 4 blocks per CU (rows a multiple of 32, at most COLSUM_ROWS)
+    int64_t rows = KEVD_ROUND_UP((src.n + 4 * (int64_t)ctx->num_cu - 1) / (4 * (int64_t)ctx->num_cu), 32);
+    rows = std::max<int64_t>(32, std::min<int64_t>(COLSUM_ROWS, rows));
+    int nb = (int)((src.n + rows - 1) / rows);
     int64_t rpad = KEVD_ROUND_UP(src.r, 4);
     double* part = nullptr;
     KEVD_TRY(ctx_buf_t(ctx, "e_cpart", (size_t)nb * rpad, &part));
     ProfScope ps(ctx, TC_MATVEC);
-    colsum_part<T>(ctx, src, w, part, rpad);
+    colsum_part<T>(ctx, src, w, part, rpad, (int)rows);
     k_colsum_reduce<<<(unsigned)((src.r + 63) / 64), 1024, 0, ctx->stream>>>(part, nb, src.r, rpad, c);
     KEVD_HIP(ctx, hipGetLastError());
     return KEVD_OK;
@@ -1403,7 +1416,7 @@ kevd_status gram_x6(kevd_ctx* ctx, const Rows<float>& src, const double* w, doub
         k_gram_x6<<<nitems, C::NT, 0, ctx->stream>>>((const float*)src.F, src.ldf, n, r, wt, dTI, dTJ, ntiles,
                                                       nitems, rps, part);
     }
-    k_gram_reduce<C::BI, C::BJ, 1><<<dim3(ntiles, GRAM_REDUCE_SLICES), 256, 0, ctx->stream>>>(
+    k_gram_reduce<C::BI, C::BJ, 1><<<dim3(ntiles, gram_reduce_slices(ctx, ntiles)), 256, 0, ctx->stream>>>(
         part, nsplit, ntiles, dTI, dTJ, r, G, nullptr);
     KEVD_HIP(ctx, hipGetLastError());
     ctx->tm.gram_flop += (double)n * (double)r * (double)(r + 1);
@@ -1475,7 +1488,7 @@ kevd_status gram_rows(kevd_ctx* ctx, const Rows<T>& src, const double* w1, const
         k_gram<T, NG><<<nitems, C::NT, 0, ctx->stream>>>(src.F, src.ldf, n, r, wt1, wt2, dTI, dTJ, ntiles, nitems,
                                                       rps, part);
     }
-    k_gram_reduce<C::BI, C::BJ, NG><<<dim3(ntiles, GRAM_REDUCE_SLICES), 256, 0, ctx->stream>>>(
+    k_gram_reduce<C::BI, C::BJ, NG><<<dim3(ntiles, gram_reduce_slices(ctx, ntiles)), 256, 0, ctx->stream>>>(
         part, nsplit, ntiles, dTI, dTJ, r, G1, G2);
     KEVD_HIP(ctx, hipGetLastError());
     ctx->tm.gram_flop += (double)NG * (double)n * (double)r * (double)(r + 1);
@@ -1930,8 +1943,11 @@ __device__ inline double readlane_f64(double v, int lane)
     int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
     return __hiloint2double(hi, lo);
 }
+// A batched launch factors matrix b = blockIdx.x at A + b l^2 (info + b).
 __global__ __launch_bounds__(1024) void k_potrf_upper(double* A, int l, int* info)
 {
+    A += (int64_t)blockIdx.x * l * l;
+    info += blockIdx.x;
     __shared__ double D[32][33];
     __shared__ double P[32][CHOL_LMAX - 32];
     __shared__ int bad;
@@ -2278,6 +2294,9 @@ kevd_status orth_basis(kevd_ctx* ctx, double* Y, int64_t r, int l, bool househol
 // ||W x - theta x|| <= 1e-6 theta + 1e-14 theta_max.  The error of a Ritz
 // value is of the order of the squared relative residual, so the stopping
 // rule gives ~1e-12 relative eigenvalues.
+kevd_status eig_batch_join(kevd_ctx* ctx, double* W, int64_t r, int k, double* lam, double** Vk, int* iters,
+                           bool* handled);
+
 kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, double* lam,
                         double** Vk, int* iters)
 {
@@ -2288,6 +2307,11 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
         KEVD_TRY(eigh(ctx, W, r, lam));
         *Vk = W + (r - k) * r;
         return KEVD_OK;
+    }
+    if (ctx->eig_batch) {
+        bool handled = false;
+        kevd_status bs = eig_batch_join(ctx, W, r, k, lam, Vk, iters, &handled);
+        if (handled) return bs;
     }
     rocblas_handle h = blas_handle(ctx);
     double *Q, *Y, *H, *G, *GS, *th, *tau, *res2, *E, *X;
@@ -2389,10 +2413,13 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
                 need = std::max(need, std::min(m, 6));
             }
             if (conv || it == maxit) break;
-            next_check = it + need;
+            // the last step is always a check; a check on the next step needs
+            // the basis orthonormal to O(u) (two passes), as before the first
+            next_check = std::min(it + need, maxit);
             KEVD_HIP(ctx, hipMemsetAsync(info, 0, (1 + 3 * max_orth) * sizeof(int), ctx->stream));
             north = 0;
-            KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++, 1));
+            KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++,
+                                next_check == it + 1 ? 2 : 1));
             std::swap(Q, Y);
         }
         *iters = it;
@@ -2420,6 +2447,338 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
     }
     return kevd_fail(ctx, KEVD_EEIG, "subspace iteration: orthonormalisation failed");
 }
+
+
+// ------------------------------------------- batched leading eigenpairs
+// The ensemble's problems each solve an r_p x r_p leading-eigenpair problem
+// whose cost is latency: 6-9 subspace steps of small GEMMs, Cholesky QR
+// passes and l x l Rayleigh-Ritz eigensolves, 3.4-4.3 ms whatever r_p
+// (48 problems: 182 ms in sequence, about 45 ms on 4 hardware queues).
+// Here the problems that reach the subspace iteration meet, and one solve
+// runs the same iteration for all of them with strided-batched rocBLAS /
+// rocSOLVER calls on copies of W_p zero-padded to the largest order R (the
+// padding adds eigenvalue 0, below every leading eigenvalue of the
+// positive semidefinite W_p, and leaves the Ritz vectors' padded rows 0).
+// The stopping rule is eig_leading's, applied to every problem; problems
+// not converged at the step limit, or a Cholesky QR that loses
+// definiteness, fall back to each problem's own solve.
+
+__global__ void k_pad_copy(const double* const* __restrict__ Wp, const int64_t* __restrict__ rs, int64_t R,
+                           double* __restrict__ Wb)
+{
+    const int p = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= R * R) return;
+    const int64_t r = rs[p], i = e % R, j = e / R;
+    Wb[(int64_t)p * R * R + e] = i < r && j < r ? Wp[p][i + j * r] : 0.0;
+}
+
+// strict lower triangle of each l x l matrix := 0 (grid: tiles, problems)
+__global__ void k_zero_lower_b(double* G, int l)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)l * l) return;
+    if (e % l > e / l) G[(int64_t)blockIdx.y * l * l + e] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_chol_shift_b(double* G, int l, double c)
+{
+    double* g = G + (int64_t)blockIdx.x * l * l;
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < l; i += 256) acc += g[i + (int64_t)i * l];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    double sh = c * red[0];
+    for (int i = threadIdx.x; i < l; i += 256) g[i + (int64_t)i * l] += sh;
+}
+
+__global__ void k_symmetrize_b(double* H, int l)
+{
+    double* h = H + (int64_t)blockIdx.z * l * l;
+    int i = blockIdx.x * 16 + (threadIdx.x >> 4), j = blockIdx.y * 16 + (threadIdx.x & 15);
+    if (i < l && j < l && i < j) {
+        double a = 0.5 * (h[i + (int64_t)j * l] + h[j + (int64_t)i * l]);
+        h[i + (int64_t)j * l] = a;
+        h[j + (int64_t)i * l] = a;
+    }
+}
+
+__global__ void k_ritz_res2_b(const double* S, const double* GS, const double* th, int l, int k, double* res2)
+{
+    const int p = blockIdx.y, c = blockIdx.x, i = l - k + c;
+    const double* s = S + (int64_t)p * l * l;
+    const double* gs = GS + (int64_t)p * l * l;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < l; j += 256) acc += s[j + (int64_t)i * l] * gs[j + (int64_t)i * l];
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) res2[(int64_t)p * k + c] = red[0] - th[(int64_t)p * l + i] * th[(int64_t)p * l + i];
+}
+
+}  // namespace
+
+struct EigBatch {
+    kevd_ctx* m;
+    int k, l;
+    int expected;  // slots that have neither joined nor left yet, plus those joined
+    int arrived = 0;
+    bool done = false;
+    std::mutex mu;
+    std::condition_variable cv;
+    struct Slot {
+        kevd_ctx* ctx = nullptr;
+        double* W = nullptr;
+        int64_t r = 0;
+        double* lam = nullptr;
+        double* X = nullptr;
+        bool joined = false, left = false, ok = false;
+        int iters = 0;
+    };
+    std::vector<Slot> slots;
+};
+
+namespace {
+
+// The batched solve over the joined slots (called with the batch's mutex
+// held, every joined problem waiting).  Marks slot.ok for the problems it
+// solved; the others run their own solve.
+kevd_status eig_batch_run(EigBatch& B)
+{
+    kevd_ctx* m = B.m;
+    kevd_ctx* ctx = m;  // for BLAS()
+    std::vector<int> J;
+    for (int i = 0; i < (int)B.slots.size(); ++i)
+        if (B.slots[i].joined) J.push_back(i);
+    const int P = (int)J.size();
+    if (P < 2) return KEVD_OK;  // nothing to batch: the one problem solves alone
+    TraceRange tr("kEVD batched eigensolve");
+    const int l = B.l, k = B.k;
+    int64_t R = 0;
+    for (int i : J) R = std::max(R, B.slots[i].r);
+    KEVD_HIP(m, hipSetDevice(m->device));
+    rocblas_handle h = blas_handle(m);
+    if (!h) return kevd_fail(m, KEVD_EHIP, "rocblas handle creation failed");
+    const int max_orth = 8;
+    double *Wb, *Q, *Y, *H, *G, *GS, *th, *E, *res2, *Xb;
+    int64_t* rs;
+    const double** Wp;
+    int* info;
+    const int ninfo = (1 + 3 * max_orth) * P;
+    KEVD_TRY(ctx_buf_t(m, "eb_W", (size_t)P * R * R, &Wb));
+    KEVD_TRY(ctx_buf_t(m, "eb_Q", (size_t)P * R * l, &Q));
+    KEVD_TRY(ctx_buf_t(m, "eb_Y", (size_t)P * R * l, &Y));
+    KEVD_TRY(ctx_buf_t(m, "eb_H", (size_t)P * l * l, &H));
+    KEVD_TRY(ctx_buf_t(m, "eb_G", (size_t)P * l * l, &G));
+    KEVD_TRY(ctx_buf_t(m, "eb_GS", (size_t)P * l * l, &GS));
+    KEVD_TRY(ctx_buf_t(m, "eb_th", (size_t)P * l, &th));
+    KEVD_TRY(ctx_buf_t(m, "eb_E", (size_t)P * l, &E));
+    KEVD_TRY(ctx_buf_t(m, "eb_res2", (size_t)P * k, &res2));
+    KEVD_TRY(ctx_buf_t(m, "eb_X", (size_t)P * R * k, &Xb));
+    KEVD_TRY(ctx_buf_t(m, "eb_rs", P, &rs));
+    KEVD_TRY(ctx_buf(m, "eb_Wp", P * sizeof(double*), (void**)&Wp));
+    KEVD_TRY(ctx_buf_t(m, "eb_info", ninfo, &info));
+    double* T;
+    int *tinfo, *nsw;
+    KEVD_TRY(ctx_buf_t(m, "eb_T", (size_t)P * R * l, &T));
+    KEVD_TRY(ctx_buf_t(m, "eb_tinfo", P, &tinfo));
+    KEVD_TRY(ctx_buf_t(m, "eb_nsw", P, &nsw));
+    std::vector<int64_t> hr(P);
+    std::vector<const double*> hw(P);
+    for (int q = 0; q < P; ++q) hr[q] = B.slots[J[q]].r, hw[q] = B.slots[J[q]].W;
+    hipStream_t s = m->stream;
+    KEVD_HIP(m, hipMemcpyAsync(rs, hr.data(), P * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    KEVD_HIP(m, hipMemcpyAsync(Wp, hw.data(), P * sizeof(double*), hipMemcpyHostToDevice, s));
+    k_pad_copy<<<dim3((unsigned)((R * R + 255) / 256), P), 256, 0, s>>>(Wp, rs, R, Wb);
+    k_randn<<<(unsigned)((P * R * l + 255) / 256), 256, 0, s>>>(Q, P * R * l, 0x6b657664ull);
+    KEVD_HIP(m, hipGetLastError());
+    const rocblas_int Rr = (rocblas_int)R, Lb = (rocblas_int)l, Pb = (rocblas_int)P;
+    const rocblas_stride sRR = R * R, sRL = R * l, sLL = (rocblas_stride)l * l, sL = l;
+    double one = 1.0, zero = 0.0;
+    const double cshift = 11.0 * ((double)R * l + (double)l * (l + 1)) * 0x1.0p-53;
+    // passes shifted (first) then plain Cholesky QR; info of pass j at info + j * P
+    auto orth = [&](int slot0, int passes) -> kevd_status {
+        for (int pass = 0; pass < passes; ++pass) {
+            BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, Rr,
+                                               &one, Y, Rr, sRL, Y, Rr, sRL, &zero, G, Lb, sLL, Pb));
+            if (pass == 0) k_chol_shift_b<<<P, 256, 0, s>>>(G, l, cshift);
+            // one workgroup per problem (rocSOLVER's batched potrf and rocBLAS's
+            // batched trsm run as hundreds of per-matrix kernels), then
+            // Y <- Y R^-1 as a batched GEMM with the inverted factor
+            k_potrf_upper<<<P, 1024, 0, s>>>(G, l, info + (int64_t)(slot0 + pass) * P);
+            BLAS(rocsolver_dtrtri_strided_batched(h, rocblas_fill_upper, rocblas_diagonal_non_unit, Lb, G, Lb, sLL,
+                                                  tinfo, Pb));
+            // the GEMM reads the whole matrix: the strict lower triangle still
+            // holds the Gram
+            k_zero_lower_b<<<dim3((unsigned)(((int64_t)l * l + 255) / 256), P), 256, 0, s>>>(G, l);
+            BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, Rr, Lb, Lb, &one, Y,
+                                               Rr, sRL, G, Lb, sLL, &zero, T, Rr, sRL, Pb));
+            std::swap(Y, T);
+        }
+        KEVD_HIP(m, hipGetLastError());
+        return KEVD_OK;
+    };
+    std::vector<int> hinfo(ninfo);
+    std::vector<double> hres((size_t)P * k), hth((size_t)P * l);
+    // dead: a Cholesky QR pass or the Rayleigh-Ritz eigensolve of that
+    // problem failed; the batch elements are independent, so the others go on
+    std::vector<char> conv(P, 0), dead(P, 0);
+    KEVD_HIP(m, hipMemsetAsync(info, 0, ninfo * sizeof(int), s));
+    const int maxit = 16;
+    int next_check = 4, north = 0, it = 0;
+    bool all_conv = false;
+    for (it = 1; it <= maxit; ++it) {
+        BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, Rr, Lb, Rr, &one, Wb,
+                                           Rr, sRR, Q, Rr, sRL, &zero, Y, Rr, sRL, Pb));
+        if (it < next_check && north < max_orth) {
+            bool before_check = it + 1 >= next_check || north + 1 >= max_orth;
+            KEVD_TRY(orth(1 + 3 * north++, before_check ? 2 : 1));
+            std::swap(Q, Y);
+            continue;
+        }
+        BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, Rr, &one,
+                                           Q, Rr, sRL, Y, Rr, sRL, &zero, H, Lb, sLL, Pb));
+        BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, Rr, &one,
+                                           Y, Rr, sRL, Y, Rr, sRL, &zero, G, Lb, sLL, Pb));
+        k_symmetrize_b<<<dim3((l + 15) / 16, (l + 15) / 16, P), 256, 0, s>>>(H, l);
+        // batched Jacobi (eigenvalues ascending; its batched divide and
+        // conquer runs as thousands of per-matrix kernels), converged once
+        // the off-diagonal norm is 1e-14 of the matrix's (a tolerance of
+        // 0 asks for machine precision, which it need not reach)
+        BLAS(rocsolver_dsyevj_strided_batched(h, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_lower,
+                                              Lb, H, Lb, sLL, 1e-14, E, 100, nsw, th, sL, info, Pb));
+        BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, Lb, Lb, Lb, &one, G, Lb,
+                                           sLL, H, Lb, sLL, &zero, GS, Lb, sLL, Pb));
+        k_ritz_res2_b<<<dim3(k, P), 256, 0, s>>>(H, GS, th, l, k, res2);
+        KEVD_HIP(m, hipGetLastError());
+        KEVD_HIP(m, hipMemcpyAsync(hinfo.data(), info, ninfo * sizeof(int), hipMemcpyDeviceToHost, s));
+        KEVD_HIP(m, hipMemcpyAsync(hres.data(), res2, hres.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        KEVD_HIP(m, hipMemcpyAsync(hth.data(), th, hth.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        KEVD_HIP(m, hipStreamSynchronize(s));
+        for (int j = 0; j < (1 + 3 * north) * P; ++j)
+            if (hinfo[j] != 0) dead[j % P] = 1;
+        all_conv = true;
+        int need = 1;
+        for (int q = 0; q < P; ++q) {
+            if (dead[q]) {
+                conv[q] = 0;
+                continue;
+            }
+            const double* tq = hth.data() + (size_t)q * l;
+            const double thmax = std::fabs(tq[l - 1]);
+            bool cq = true;
+            for (int c = 0; c < k; ++c) {
+                double t = tq[l - k + c], res = std::sqrt(std::max(hres[(size_t)q * k + c], 0.0));
+                double tolc = 1e-6 * std::fabs(t) + 1e-14 * thmax;
+                if (res <= tolc) continue;
+                cq = false;
+                double rho = std::fabs(tq[0]) / std::max(std::fabs(t), 1e-300);
+                int mm = rho > 0.0 && rho < 0.9 ? (int)std::ceil(std::log(tolc / res) / std::log(rho)) : 1;
+                need = std::max(need, std::min(mm, 6));
+            }
+            conv[q] = cq;
+            all_conv &= cq;
+        }
+        if (all_conv || it == maxit) break;
+        next_check = std::min(it + need, maxit);
+        KEVD_HIP(m, hipMemsetAsync(info, 0, ninfo * sizeof(int), s));
+        north = 0;
+        KEVD_TRY(orth(1 + 3 * north++, next_check == it + 1 ? 2 : 1));
+        std::swap(Q, Y);
+    }
+    // Ritz vectors of the leading k: X = Q S(:, l-k:l), then each problem's
+    // first r_p rows and its eigenvalue tail
+    BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, Rr, (rocblas_int)k, Lb,
+                                       &one, Q, Rr, sRL, H + (int64_t)(l - k) * l, Lb, sLL, &zero, Xb, Rr,
+                                       (rocblas_stride)R * k, Pb));
+    for (int q = 0; q < P; ++q) {
+        if (!conv[q]) continue;
+        EigBatch::Slot& sl = B.slots[J[q]];
+        KEVD_HIP(m, hipMemcpy2DAsync(sl.X, sl.r * sizeof(double), Xb + (int64_t)q * R * k, R * sizeof(double),
+                                     sl.r * sizeof(double), k, hipMemcpyDeviceToDevice, s));
+        k_lam_tail<<<(unsigned)((sl.r + 255) / 256), 256, 0, s>>>(th + (int64_t)q * l, l, sl.r, sl.lam);
+    }
+    KEVD_HIP(m, hipGetLastError());
+    KEVD_HIP(m, hipStreamSynchronize(s));
+    for (int q = 0; q < P; ++q) {
+        B.slots[J[q]].ok = conv[q] != 0;
+        B.slots[J[q]].iters = it;
+    }
+    return KEVD_OK;
+}
+
+// with the mutex held: run once every slot has joined or left
+void eig_batch_maybe_run(EigBatch& B)
+{
+    if (B.done || B.arrived < B.expected) return;
+    if (B.arrived > 0 && eig_batch_run(B) != KEVD_OK)
+        for (auto& sl : B.slots) sl.ok = false;  // the batch failed: every problem solves alone
+    B.done = true;
+    B.cv.notify_all();
+}
+
+kevd_status eig_batch_join(kevd_ctx* ctx, double* W, int64_t r, int k, double* lam, double** Vk, int* iters,
+                           bool* handled)
+{
+    EigBatch& B = *ctx->eig_batch;
+    *handled = false;
+    const int slot = ctx->eig_batch_slot;
+    if (slot < 0 || slot >= (int)B.slots.size() || k != B.k) return KEVD_OK;
+    double* X;
+    KEVD_TRY(ctx_buf_t(ctx, "q_X", (size_t)r * k, &X));
+    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));  // W complete before another stream reads it
+    std::unique_lock<std::mutex> g(B.mu);
+    if (B.done) return KEVD_OK;  // too late for the batch: solve alone
+    EigBatch::Slot& sl = B.slots[slot];
+    sl.ctx = ctx, sl.W = W, sl.r = r, sl.lam = lam, sl.X = X;
+    sl.joined = true;
+    ++B.arrived;
+    eig_batch_maybe_run(B);
+    B.cv.wait(g, [&] { return B.done; });
+    if (!sl.ok) return KEVD_OK;  // not converged, or the batch broke down: solve alone
+    *handled = true;
+    *Vk = X;
+    *iters = sl.iters;
+    ctx->tm.eig_iters = sl.iters;
+    ctx->tm.eig_batched = 1;
+    return KEVD_OK;
+}
+
+}  // namespace
+
+EigBatch* eig_batch_create(kevd_ctx* master, int nslots, int k_out)
+{
+    EigBatch* b = new EigBatch();
+    b->m = master;
+    b->k = k_out;
+    b->l = (int)KEVD_ROUND_UP(std::max(k_out + 64, std::min(4 * k_out, 256)), 32);
+    b->expected = nslots;
+    b->slots.resize(nslots);
+    return b;
+}
+
+void eig_batch_leave(EigBatch* b, int slot)
+{
+    std::lock_guard<std::mutex> g(b->mu);
+    EigBatch::Slot& sl = b->slots[slot];
+    if (sl.joined || sl.left) return;
+    sl.left = true;
+    --b->expected;
+    eig_batch_maybe_run(*b);
+}
+
+void eig_batch_destroy(EigBatch* b) { delete b; }
+
+namespace {
 
 // ------------------------------------------------ intermediate record
 // The reference's dump datasets (ks_kevd.cu:531-542, 809-815, 935-941,
@@ -2922,6 +3281,7 @@ extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_ev
     ctx->tm.eig_iters = 0;
     ctx->tm.eig_householder = 0;
     ctx->tm.eig_full_fallback = 0;
+    ctx->tm.eig_batched = 0;
     prof_reset(ctx);
     size_t ubytes = (size_t)ctx->n * p->k_out * sizeof(float);
     if (ctx->U_bytes < ubytes) {

@@ -64,6 +64,11 @@ struct kevd_ctx {
     // Rayleigh-Ritz check there.
     int eig_converged_at = 0;
 
+    // Set by kevd_ensemble_evd: the leading-eigenpair solve joins one batched
+    // solve over the ensemble's problems (slot = this problem's index).
+    struct EigBatch* eig_batch = nullptr;
+    int eig_batch_slot = -1;
+
     // Device address of this device's fixed scratch array (KEVD_SCRATCH_BYTES),
     // resolved when the context is created; shared by every context on the
     // device.
@@ -139,5 +144,16 @@ bool use_bf16x6();
 
 // Cross-unit entry points.
 kevd_status pivot_cholesky(kevd_ctx* ctx, double* L);
+
+// Batched leading-eigenpair solve for an ensemble (evd.hip): every problem
+// whose kEVD reaches the subspace iteration joins, the last one to arrive
+// (or to leave without joining) runs one batched solve for all of them on
+// the master context, and each continues with its own eigenvectors.
+// nslots problems, k_out leading pairs each.  A problem that ends without
+// joining must leave (eig_batch_leave is a no-op for one that joined).
+struct EigBatch;
+EigBatch* eig_batch_create(kevd_ctx* master, int nslots, int k_out);
+void eig_batch_leave(EigBatch* b, int slot);
+void eig_batch_destroy(EigBatch* b);
 kevd_status kblock_gather(kevd_ctx* ctx, const int64_t* idx_dev, int64_t count,
                           kevd_precision prec, void* Xout, double* nrm_out);

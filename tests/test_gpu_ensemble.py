@@ -5,8 +5,14 @@ Bars: every problem of a batched mixed run gives the same pivots, accepted
 counts, trace errors and factor bits as kevd_arpc on the same points and
 parameters (both run the same round bodies); the FP64 ensemble matches the
 CPU oracle's pivots exactly and its factor to 1e-9; the concurrent EVDs
-match single-context EVDs to 1e-12 relative; the autotuner returns one of
-its candidates with positive measured rates.
+match single-context EVDs to 1e-10 relative (the ensemble's batched
+leading-eigenpair solve and a problem's own solve stop at different
+steps); the batched solve's eigenvalues match the problems' own solves
+(KEVD_ENS_EIG_BATCH=0) to 1e-10 relative (both stop at Ritz residuals of
+1e-6 theta, so eigenvalues agree to about 1e-12) and the leading
+eigenvectors with a relative gap above 1e-4 to 1e-5 of their scale, up to
+sign; the autotuner returns one
+of its candidates with positive measured rates.
 """
 import numpy as np
 import pytest
@@ -57,8 +63,49 @@ def test_ensemble_mixed_matches_single_problem_runs():
             Fr = c.get_factor(ref.rank, "mixed")
             assert np.array_equal(Fe, Fr), "problem %d factor differs" % i
             er = c.evd(ref.rank, k_out=12, prec="mixed", norm="sinkhorn")
-            np.testing.assert_allclose(evs[i].lam[-12:], er.lam[-12:], rtol=1e-12, atol=0)
+            np.testing.assert_allclose(evs[i].lam[-12:], er.lam[-12:], rtol=1e-10, atol=0)
             assert evs[i].rowsum_dev <= 1e-12
+    for c in ctxs:
+        c.close()
+
+
+def test_ensemble_batched_eigensolve_matches_own_solves(monkeypatch):
+    from kernel_evd_amd import ensemble_arpc, ensemble_evd
+    ctxs, Xs, eps = _contexts(PROBLEMS, seed0=10)
+    ranks = [p[2] for p in PROBLEMS]
+    outs, _ = ensemble_arpc(ctxs, eps, ranks, [p[3] for p in PROBLEMS], seeds=list(range(len(PROBLEMS))),
+                            prec="mixed")
+    rk = [o.rank for o in outs]
+    monkeypatch.setenv("KEVD_ENS_EIG_BATCH", "0")
+    own = ensemble_evd(ctxs, rk, k_out=12, norm="sinkhorn", prec="mixed", workers=3, want_U=True)
+    monkeypatch.delenv("KEVD_ENS_EIG_BATCH", raising=False)
+    bat = ensemble_evd(ctxs, rk, k_out=12, norm="sinkhorn", prec="mixed", workers=3, want_U=True)
+    tms = [c.timings() for c in ctxs]
+    print("ranks", rk, "eig iters", [t["eig_iters"] for t in tms], "batched", [t["eig_batched"] for t in tms],
+          "full fallback", [t["eig_full_fallback"] for t in tms], flush=True)
+    # k_out 12: block l = 96, so ranks >= 192 join the batch and the smaller
+    # ones take the full solve; a joined problem the batch did not converge
+    # by the step limit solves alone (its own solve then falls back to the
+    # full one)
+    for i, r in enumerate(rk):
+        if r < 192:
+            assert tms[i]["eig_batched"] == 0
+        else:
+            assert tms[i]["eig_batched"] == 1 or tms[i]["eig_full_fallback"] == 1
+    assert sum(t["eig_batched"] for t in tms) >= 1
+    for i in range(len(PROBLEMS)):
+        np.testing.assert_allclose(bat[i].lam[-12:], own[i].lam[-12:], rtol=1e-10, atol=0)
+        assert bat[i].rowsum_dev <= 1e-12
+        w = own[i].lam[-12:]
+        gap = np.abs(np.diff(w)) / np.abs(w[-1])
+        ok = np.ones(12, dtype=bool)
+        ok[0] = False
+        ok[:-1] &= gap > 1e-4
+        ok[1:] &= gap > 1e-4
+        Ub, Uo = bat[i].U.astype(np.float64), own[i].U.astype(np.float64)
+        for j in np.flatnonzero(ok):
+            e = min(np.abs(Ub[:, j] - Uo[:, j]).max(), np.abs(Ub[:, j] + Uo[:, j]).max()) / np.abs(Uo[:, j]).max()
+            assert e < 1e-5, (i, j, e)
     for c in ctxs:
         c.close()
 
