@@ -178,6 +178,11 @@ def ensemble_leg(nprob=48, kout=32, seed=1, workers=8, candidates=(32, 64, 128),
     seeds = list(range(nprob))
     # batched: warm once, then time
     ensemble_arpc(ctxs, eps, ranks, blocks, seeds=seeds, prec="mixed")
+    # each problem's Gram tile, measured on its own factor (kept by its
+    # context for the kEVDs below)
+    t = time.time()
+    tiles = [c.autotune_tiles() for c in ctxs]
+    t_tiles = time.time() - t
     ensemble_evd(ctxs, ranks, k_out=kout, norm=norm, prec="mixed", workers=workers)
     t0 = time.perf_counter()
     outs, graph = ensemble_arpc(ctxs, eps, ranks, blocks, seeds=seeds, prec="mixed")
@@ -215,11 +220,13 @@ def ensemble_leg(nprob=48, kout=32, seed=1, workers=8, candidates=(32, 64, 128),
            "graph_replay": bool(graph), "evd_workers": workers, "k_out": kout, "norm": norm,
            "blocks_chosen": {str(b): int(blocks.count(b)) for b in sorted(set(blocks))},
            "autotune_s": round(t_tune, 2),
+           "gram_tiles_chosen": {str(v): int(tiles.count(v)) for v in sorted(set(tiles))},
+           "gram_tile_tune_s": round(t_tiles, 2),
            "eig_rel_err_vs_fp64_max": max(errs), "eig_rel_err_vs_fp64_median": float(np.median(errs)),
            "eig_rel_err_vs_fp64_same_pivots_max": max(errs_same),
            "eig_rel_err_vs_fp64_same_pivots_median": float(np.median(errs_same))}
-    log("ensemble: %d problems, %d columns: ARPC %.3f s + EVD %.3f s (sequential %.3f s); max eig err %.2e"
-        % (nprob, cols, t1 - t0, t2 - t1, t4 - t3, max(errs)))
+    log("ensemble: %d problems, %d columns: ARPC %.3f s + EVD %.3f s (sequential %.3f s); max eig err %.2e;"
+        " Gram tiles %s" % (nprob, cols, t1 - t0, t2 - t1, t4 - t3, max(errs), res["gram_tiles_chosen"]))
     return res
 
 

@@ -324,6 +324,22 @@ kevd_status kevd_ensemble_arpc(kevd_ctx** ctxs, int32_t nprob, const kevd_arpc_p
  * results[i] as kevd_evd's; errors are reported on ctxs[0]. */
 kevd_status kevd_ensemble_evd(kevd_ctx** ctxs, int32_t nprob, const kevd_evd_params* p,
                               kevd_evd_result* results, int32_t workers);
+/* kevd_autotune_tiles: the output tile of the bf16x6 Gram (256 x 256 on 8
+ * waves, one workgroup per CU, or 128 x 128 on 4 waves, two per CU) for
+ * this context's FP32 factor, by timing both on the factor itself (one
+ * warm-up and two timed runs each, the faster counts).  Later kevd_evd
+ * calls on the context use the choice.  Without gram_ms the choice is
+ * cached per (device, n, rank, leading dimension) for the process and a
+ * cached answer is returned without measuring; with it (2 entries), both
+ * are measured and their times written (256 first).  An FP64 factor, or
+ * KEVD_FP32_MFMA=1, has no choice: *gram_tile = 0.  The tuning tile shapes
+ * have no counterpart in the reference, which sizes launches with the
+ * occupancy API (ks_bandwidth.cu:436-474). */
+kevd_status kevd_autotune_tiles(kevd_ctx* ctx, int32_t* gram_tile, double* gram_ms);
+/* Set the bf16x6 Gram's tile for this context without measuring: 0 (the
+ * default, 256), 128 or 256; anything else is KEVD_EINVAL. */
+kevd_status kevd_set_gram_tile(kevd_ctx* ctx, int32_t gram_tile);
+
 /* kevd_autotune_block: the block size among candidates[0..ncand) with the
  * highest factor-columns-per-second of kevd_arpc on this context's points
  * (base supplies eps, rank, seed, tol, precision; two timed runs per

@@ -87,6 +87,8 @@ SIGNATURES = {
     "kevd_ensemble_arpc": (C.c_int, [P(C.c_void_p), i32, P(ArpcParams), P(ArpcResult), P(i32)]),
     "kevd_ensemble_evd": (C.c_int, [P(C.c_void_p), i32, P(EvdParams), P(EvdResult), i32]),
     "kevd_autotune_block": (C.c_int, [C.c_void_p, P(ArpcParams), P(i32), i32, P(i32), P(dbl)]),
+    "kevd_autotune_tiles": (C.c_int, [C.c_void_p, P(i32), P(dbl)]),
+    "kevd_set_gram_tile": (C.c_int, [C.c_void_p, i32]),
     "kevd_measure_peaks": (C.c_int, [C.c_void_p, i64, P(Peaks)]),
 }
 

@@ -245,6 +245,19 @@ class Context:
         self._rank = rank  # the context holds the last timed run's factor
         return (best.value, rates) if measure else best.value
 
+    def autotune_tiles(self, measure=False):
+        """The bf16x6 Gram's output tile for this context's factor
+        (kevd_autotune_tiles): 256, 128, or 0 when there is no choice.
+        measure=True times both (and returns their ms as well)."""
+        t = C.c_int32(0)
+        ms = np.zeros(2) if measure else None
+        self._check(self._lib.kevd_autotune_tiles(self._h, C.byref(t), _p(ms, C.c_double) if measure else None))
+        return (t.value, ms) if measure else t.value
+
+    def set_gram_tile(self, tile):
+        """Use the given bf16x6 Gram tile (0, 128 or 256) without measuring."""
+        self._check(self._lib.kevd_set_gram_tile(self._h, int(tile)))
+
     def measure_peaks(self, copy_bytes=4 << 30):
         """Sustained MFMA (FP32 16x16 / 32x32, FP64), v_exp_f32 and HBM copy
         rates of this context's device (kevd_measure_peaks)."""

@@ -21,9 +21,11 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <map>
 #include <mutex>
 #include <cstdlib>
 #include <cstring>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -894,31 +896,38 @@ void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T*
 // ---------------------------------------------- weighted Gram, bf16x6
 // G = F^T diag(w) F for FP32 F on the bf16 matrix cores (bf16x6 products,
 // FP32 accumulation, FP64 promotion every PROMOTE rows as in k_gram).
-// 256 x 256 output tiles of the lower triangle on 8 waves (2 x 4) of
-// 128 x 64 each; 16 rows of F per chunk, one workgroup per CU.  Thread t
-// stages one column of one panel (threads 0..255 the weighted A panel,
-// 256..511 the B panel): it reads the column's 16 rows (one dword per row;
-// a wave reads 256 contiguous bytes of each row), splits each value into
-// bf16 hi, mid, lo and writes each part as two 16-byte halves of 8 rows.
-// LDS per stage: [panel][part][half][column][8 rows], so an MFMA lane's
-// fragment (8 rows of one column) is one 16-byte read and 32 lanes read or
-// write 512 contiguous bytes.
-struct GramX6 {
-    static constexpr int BI = 256, BJ = 256, KC = 16, NT = 512, WGM = 2, WGN = 4;
-    static constexpr int TM = 4, TN = 2, WR = 128, WC = 64, PB = 16, PROMOTE = KEVD_GRAM_PROMOTE;
-    static constexpr int HALF = 256 * 16;  // bytes of one half slab (256 columns x 8 rows)
+// Two tile shapes, chosen per problem by kevd_autotune_tiles:
+//   256 x 256 on 8 waves (2 x 4) of 128 x 64, one workgroup per CU (the
+//   default: half the panel re-reads of 128-wide tiles at large r);
+//   128 x 128 on 4 waves (2 x 2) of 64 x 64, two workgroups per CU (less
+//   waste in the partial edge and diagonal tiles at small r).
+// 16 rows of F per chunk.  Thread t stages one column of one panel (threads
+// 0..BI-1 the weighted A panel, BI..2BI-1 the B panel): it reads the
+// column's 16 rows (one dword per row; a wave reads 256 contiguous bytes of
+// each row), splits each value into bf16 hi, mid, lo and writes each part
+// as two 16-byte halves of 8 rows.  LDS per stage: [panel][part][half]
+// [column][8 rows], so an MFMA lane's fragment (8 rows of one column) is one
+// 16-byte read and 32 lanes read or write 512 contiguous bytes.
+template <int B_, int WGM_, int WGN_, int TM_, int TN_, int WPC>
+struct GramX6Cfg {
+    static constexpr int BI = B_, BJ = B_, KC = 16, NT = 2 * B_, WGM = WGM_, WGN = WGN_;
+    static constexpr int TM = TM_, TN = TN_, WR = 32 * TM_, WC = 32 * TN_, PB = 16, PROMOTE = KEVD_GRAM_PROMOTE;
+    static constexpr int WG_PER_CU = WPC;
+    static constexpr int HALF = B_ * 16;  // bytes of one half slab (B_ columns x 8 rows)
     static constexpr int PART = 2 * HALF;
     static constexpr int PANEL = 3 * PART;
-    static constexpr int STAGE = 2 * PANEL;  // 48 KiB
+    static constexpr int STAGE = 2 * PANEL;  // 48 KiB (256) / 24 KiB (128)
     static_assert(WGM * WR == BI && WGN * WC == BJ && WGM * WGN * 64 == NT, "Gram x6 wave layout");
 };
+typedef GramX6Cfg<256, 2, 4, 4, 2, 1> GramX6;
+typedef GramX6Cfg<128, 2, 2, 2, 2, 2> GramX6s;
 
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+template <typename C>
+__global__ __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_gram_x6(const float* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const float* __restrict__ w,
                const int* __restrict__ tileI, const int* __restrict__ tileJ, int ntiles, int nitems,
                int64_t rows_per_split, double* part)
 {
-    typedef GramX6 C;
     __shared__ __attribute__((aligned(16))) unsigned char lds[2 * C::STAGE];
     // the split-major item order and XCD assignment of k_gram
     int id = blockIdx.x, xcd = id & 7, per = nitems >> 3, rem = nitems & 7;
@@ -935,8 +944,8 @@ void k_gram_x6(const float* __restrict__ F, int64_t ldf, int64_t n, int64_t r, c
     __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float*>(w + k_beg), (short)0, k_end > k_beg ? (int)((k_end - k_beg) * 4) : 0, 0x00020000);
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int panel = t >> 8;  // wave-uniform
-    const int scol = t & 255;
+    const int panel = t / C::BI;  // wave-uniform
+    const int scol = t % C::BI;
     const int64_t gcol = (panel ? cj : ci) + scol;
     const bool colok = gcol < r;
     const uint32_t coff = (uint32_t)(gcol * 4);
@@ -1365,9 +1374,9 @@ kevd_status rowdot_rows(kevd_ctx* ctx, const Rows<T>& src, const double* c, doub
 }
 
 // G = F^T diag(w) F for FP32 F by k_gram_x6 and the split reduction.
+template <typename C>
 kevd_status gram_x6(kevd_ctx* ctx, const Rows<float>& src, const double* w, double* G, const char* wbuf)
 {
-    typedef GramX6 C;
     int64_t n = src.n, r = src.r;
     int ti_n = (int)((r + C::BI - 1) / C::BI), tj_n = (int)((r + C::BJ - 1) / C::BJ);
     std::vector<int> TI, TJ;
@@ -1381,7 +1390,7 @@ kevd_status gram_x6(kevd_ctx* ctx, const Rows<float>& src, const double* w, doub
     // One workgroup per CU; at least ~2048 workgroups, the split count
     // chosen so that the grid fills whole waves of the device's CUs.  A
     // split's rows are addressed by 32-bit buffer offsets.
-    const int64_t slots = ctx->num_cu;
+    const int64_t slots = C::WG_PER_CU * (int64_t)ctx->num_cu;
     const int64_t rp_max = ((int64_t)INT32_MAX / (src.ldf * 4)) / C::KC * C::KC - C::KC;
     if (rp_max < C::KC) return kevd_fail(ctx, KEVD_EINVAL, "Gram: factor rows too long for 32-bit offsets");
     int ns0 = (int)std::max<int64_t>({(int64_t)1, (2048 + ntiles - 1) / ntiles, (n + rp_max - 1) / rp_max});
@@ -1413,7 +1422,7 @@ kevd_status gram_x6(kevd_ctx* ctx, const Rows<float>& src, const double* w, doub
     KEVD_HIP(ctx, hipMemcpyAsync(dTJ, TJ.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     {
         ProfScope ps(ctx, TC_GRAM);
-        k_gram_x6<<<nitems, C::NT, 0, ctx->stream>>>((const float*)src.F, src.ldf, n, r, wt, dTI, dTJ, ntiles,
+        k_gram_x6<C><<<nitems, C::NT, 0, ctx->stream>>>((const float*)src.F, src.ldf, n, r, wt, dTI, dTJ, ntiles,
                                                       nitems, rps, part);
     }
     k_gram_reduce<C::BI, C::BJ, 1><<<dim3(ntiles, gram_reduce_slices(ctx, ntiles)), 256, 0, ctx->stream>>>(
@@ -1428,8 +1437,13 @@ kevd_status gram_rows(kevd_ctx* ctx, const Rows<T>& src, const double* w1, const
 {
     if constexpr (std::is_same<T, float>::value) {
         if (use_bf16x6()) {
-            KEVD_TRY(gram_x6(ctx, src, w1, G1, "e_gw1"));
-            if (NG > 1) KEVD_TRY(gram_x6(ctx, src, w2, G2, "e_gw2"));
+            if (ctx->gram_tile == 128) {
+                KEVD_TRY(gram_x6<GramX6s>(ctx, src, w1, G1, "e_gw1"));
+                if (NG > 1) KEVD_TRY(gram_x6<GramX6s>(ctx, src, w2, G2, "e_gw2"));
+            } else {
+                KEVD_TRY(gram_x6<GramX6>(ctx, src, w1, G1, "e_gw1"));
+                if (NG > 1) KEVD_TRY(gram_x6<GramX6>(ctx, src, w2, G2, "e_gw2"));
+            }
             return KEVD_OK;
         }
     }
@@ -2803,6 +2817,12 @@ __global__ __launch_bounds__(256) void k_rec_upper(const double* Lc, int64_t r, 
     out[e] = i <= j ? Lc[j + i * r] : 0.0;
 }
 
+__global__ __launch_bounds__(256) void k_fill1(double* p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 1.0;
+}
+
 __global__ __launch_bounds__(256) void k_rec_tril(double* M, int64_t r)
 {
     int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -3341,5 +3361,83 @@ extern "C" kevd_status kevd_get_intermediate(kevd_ctx* ctx, const char* name, do
         std::memcpy(out, it->second.data(), (size_t)m * sizeof(double));
     }
     *count = m;
+    return KEVD_OK;
+}
+
+namespace {
+std::mutex tile_mu;
+std::map<std::tuple<int, int64_t, int64_t, int64_t>, int> tile_cache;
+}  // namespace
+
+extern "C" kevd_status kevd_autotune_tiles(kevd_ctx* ctx, int32_t* gram_tile, double* gram_ms)
+{
+    TraceRange trace_range("kevd_autotune_tiles");
+    if (!ctx || !gram_tile) return kevd_fail(ctx, KEVD_EINVAL, "autotune_tiles: null argument");
+    if (!ctx->have_factor || ctx->rank < 1) return kevd_fail(ctx, KEVD_EINVAL, "autotune_tiles: no factor");
+    if (ctx->fprec != KEVD_MIXED || !use_bf16x6()) {
+        // only the bf16x6 Gram of an FP32 factor has a tile choice
+        ctx->gram_tile = 0;
+        *gram_tile = 0;
+        return KEVD_OK;
+    }
+    KEVD_HIP(ctx, hipSetDevice(ctx->device));
+    const auto key = std::make_tuple(ctx->device, ctx->n, ctx->rank, ctx->ldf);
+    if (!gram_ms) {
+        std::lock_guard<std::mutex> g(tile_mu);
+        auto it = tile_cache.find(key);
+        if (it != tile_cache.end()) {
+            ctx->gram_tile = it->second;
+            *gram_tile = it->second;
+            return KEVD_OK;
+        }
+    }
+    const int64_t n = ctx->n, r = ctx->rank;
+    double *w, *G;
+    KEVD_TRY(ctx_buf_t(ctx, "t_w", n, &w));
+    KEVD_TRY(ctx_buf_t(ctx, "t_G", (size_t)r * r, &G));
+    k_fill1<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(w, n);
+    KEVD_HIP(ctx, hipGetLastError());
+    const Rows<float> src{(const float*)ctx->F, ctx->ldf, n, r};
+    hipEvent_t a, b;
+    KEVD_HIP(ctx, hipEventCreate(&a));
+    KEVD_HIP(ctx, hipEventCreate(&b));
+    double t[2] = {1e30, 1e30};
+    kevd_status st = KEVD_OK;
+    for (int c = 0; c < 2 && st == KEVD_OK; ++c) {
+        for (int rep = 0; rep < 3 && st == KEVD_OK; ++rep) {  // the first run warms up
+            if (hipEventRecord(a, ctx->stream) != hipSuccess) st = KEVD_EHIP;
+            if (st == KEVD_OK)
+                st = c == 0 ? gram_x6<GramX6>(ctx, src, w, G, "t_gw") : gram_x6<GramX6s>(ctx, src, w, G, "t_gw");
+            float ms = 0.f;
+            if (st == KEVD_OK && hipEventRecord(b, ctx->stream) != hipSuccess) st = KEVD_EHIP;
+            if (st == KEVD_OK && hipEventSynchronize(b) != hipSuccess) st = KEVD_EHIP;
+            if (st == KEVD_OK && hipEventElapsedTime(&ms, a, b) != hipSuccess) st = KEVD_EHIP;
+            if (st == KEVD_OK && rep > 0) t[c] = std::min(t[c], (double)ms);
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (st == KEVD_EHIP) return kevd_fail(ctx, st, "autotune_tiles: timing failed");
+    if (st != KEVD_OK) return st;
+    const int choice = t[1] < t[0] ? 128 : 256;
+    {
+        std::lock_guard<std::mutex> g(tile_mu);
+        tile_cache[key] = choice;
+    }
+    ctx->gram_tile = choice;
+    *gram_tile = choice;
+    if (gram_ms) {
+        gram_ms[0] = t[0];
+        gram_ms[1] = t[1];
+    }
+    return KEVD_OK;
+}
+
+extern "C" kevd_status kevd_set_gram_tile(kevd_ctx* ctx, int32_t gram_tile)
+{
+    if (!ctx) return KEVD_EINVAL;
+    if (gram_tile != 0 && gram_tile != 128 && gram_tile != 256)
+        return kevd_fail(ctx, KEVD_EINVAL, "set_gram_tile: 0, 128 or 256");
+    ctx->gram_tile = gram_tile;
     return KEVD_OK;
 }

@@ -64,6 +64,9 @@ struct kevd_ctx {
     // Rayleigh-Ritz check there.
     int eig_converged_at = 0;
 
+    // Tile of the bf16x6 Gram (256 or 128; 0: 256), set by kevd_autotune_tiles.
+    int gram_tile = 0;
+
     // Set by kevd_ensemble_evd: the leading-eigenpair solve joins one batched
     // solve over the ensemble's problems (slot = this problem's index).
     struct EigBatch* eig_batch = nullptr;

@@ -177,3 +177,31 @@ def test_autotune_block():
         assert np.all(rates > 0)
         assert best == (16, 64, 128)[int(np.argmax(rates))]
         assert c.autotune_block(eps, 300, candidates=(16, 64, 128)) == best  # cached
+
+
+def test_autotune_tiles_chooses_and_keeps_the_results():
+    """Both Gram tiles are measured on the factor and the faster one is
+    kept; the kEVD's eigenvalues do not depend on the tile beyond FP32
+    accumulation order (1e-6 relative)."""
+    from kernel_evd_amd import Context, KevdError
+    X = clustered_points(20000, 16, seed=40).astype(np.float32)
+    eps = orc.median_sqdist(X.astype(np.float64), seed=0, nsample=20001)
+    with Context(0) as c:
+        c.set_points(X)
+        a = c.arpc(eps, 300, block=64, seed=1, prec="mixed")
+        tile, ms = c.autotune_tiles(measure=True)
+        print("tile", tile, "ms (256, 128)", ms, flush=True)
+        assert tile in (128, 256) and np.all(ms > 0)
+        assert tile == (128 if ms[1] < ms[0] else 256)
+        assert c.autotune_tiles() == tile  # cached for this shape
+        lams = {}
+        for t in (256, 128):
+            c.set_gram_tile(t)
+            lams[t] = c.evd(a.rank, k_out=16, prec="mixed").lam[-16:]
+        np.testing.assert_allclose(lams[128], lams[256], rtol=1e-6)
+        with pytest.raises(KevdError):
+            c.set_gram_tile(64)
+    with Context(0) as c:
+        c.set_points(X)
+        c.arpc(eps, 300, block=64, seed=1, prec="fp64")
+        assert c.autotune_tiles() == 0  # FP64 factor: no choice
