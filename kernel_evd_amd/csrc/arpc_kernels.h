@@ -491,7 +491,10 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
 }
 
 // The bf16x6 parts of the pivot rows follow FS (BMAX x ldf FP32) in the
-// same buffer: part q at x6_fsx(FS, ldf) + q * BMAX * ldf.
+// same buffer, chunk-major for the downdate's loads: the parts of columns
+// [32 c, 32 c + 32) of all BMAX rows are one contiguous 24 KB block [part]
+// [row][32] (X6_CHUNK elements), so that every load of the downdate reads
+// whole cache lines.
 __host__ __device__ inline size_t x6_fs_bytes(int64_t ldf) { return (size_t)BMAX * ldf * (4 + 3 * 2); }
 __device__ inline __bf16* x6_fsx(float* FS, int64_t ldf) { return reinterpret_cast<__bf16*>(FS + (int64_t)BMAX * ldf); }
 __device__ inline const __bf16* x6_fsx(const float* FS, int64_t ldf)
@@ -517,14 +520,15 @@ __device__ inline void gather_pivots_body(const T* __restrict__ Xsrc, const doub
     for (int64_t m = threadIdx.x; m < Nr; m += 256) FS[c * ldf + m] = s >= 0 ? F[s * ldf + m] : (T)0;
     if constexpr (X6 && std::is_same<T, float>::value) {
         __bf16* fx = x6_fsx(FS, ldf);
-        const int64_t ps = (int64_t)BMAX * ldf, kp = (Nr + 31) / 32 * 32;
+        const int64_t kp = (Nr + 31) / 32 * 32;
         for (int64_t m = threadIdx.x; m < kp; m += 256) {
             float v = s >= 0 && m < Nr ? F[s * ldf + m] : 0.0f;
             __bf16 h, md, l;
             bf16_split3(v, h, md, l);
-            fx[c * ldf + m] = h;
-            fx[ps + c * ldf + m] = md;
-            fx[2 * ps + c * ldf + m] = l;
+            __bf16* e = fx + (m / 32) * X6_CHUNK + c * 32 + m % 32;
+            e[0] = h;
+            e[X6_CHUNK / 3] = md;
+            e[2 * X6_CHUNK / 3] = l;
         }
     }
 }
@@ -730,7 +734,7 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
         if constexpr (X6 && std::is_same<T, float>::value) {
             static_assert(X6Stage<G>::BYTES <= DowndateLds<T>::ELEMS * (int)sizeof(T), "the x6 stage fits");
             x6_nt_mainloop_live<G>(g, reinterpret_cast<unsigned char*>(lds), F + r0 * ldf, ldf, n - r0,
-                                   x6_fsx(FS, ldf), ldf, (int64_t)BMAX * ldf, Nr);
+                                   x6_fsx(FS, ldf), Nr);
         } else {
             g.mainloop_live(lds, F + r0 * ldf, ldf, n - r0, FS, ldf, BN, Nr);
         }

@@ -58,109 +58,106 @@ __device__ inline void mfma_bf16_16(floatx4& c, const bf16x8& a, const bf16x8& b
 // v_mfma_f32_16x16x32_bf16, into the accumulators of an FP32 NTGemm G with
 // 16 x 16 tiles, one wave column (WN = 1) and KC = 32 (same accumulator
 // layout, so the caller's epilogues are unchanged).  NACT live column tiles.
-// A is FP32 with K-contiguous rows (as NTGemm::mainloop); B comes already
-// split: part q at Bx + q * part_stride, row j at + j * ldb, K-contiguous
-// and zero from K up to a multiple of 32 (the downdate's pivot rows, split
-// once per round when they are gathered).
+// A is FP32 with K-contiguous rows (the factor's row block); B (BN = 128
+// rows) comes already split, chunk-major: the parts of K values
+// [32 c, 32 c + 32) are the contiguous block Bx + c X6_CHUNK, laid out
+// [part][row][32], zero from K up to a multiple of 32 (the downdate's pivot
+// rows, split once per round when they are gathered).
 //
-// One LDS stage [part][row][32] bf16 for the BM + BN rows.  A row's four
-// 16-byte slots (8 K values each) are XOR-swizzled by (row >> 2) & 3, which
-// makes every access conflict-free: a fragment read (16 rows, one slot
-// each), an A split store (4 rows x 8 eight-byte halves per 32 lanes) and a
-// B copy (4 rows x 4 slots per 16 lanes).  The next chunk's loads are in
-// flight in registers while the current one multiplies; two barriers per
-// chunk, the second workgroup on the CU covering them.  A's rows past arows
-// read as zero through a buffer resource and its columns past K are zeroed.
-// Ends with a barrier.  (B's fragments read straight from L2 by every wave
-// instead of through LDS moved four times the bytes and took 60 ms against
-// 37.5 at N = 1M, r = 2000.)
+// A row is used by one wave only (the waves split the rows), so A never
+// passes through LDS: each lane loads the 8 K values its MFMA fragment
+// needs (row 16 x + lane % 16, K group lane / 16; a wave reads whole
+// 128-byte row segments) straight from HBM into registers, two chunks
+// ahead, and splits them into the three bf16 fragments in registers.  Only
+// B, which every wave reads, is staged (its 24 KB chunk is one contiguous
+// block, so its loads read whole cache lines): two LDS stages [part][row][32] bf16
+// whose 16-byte slots are XOR-swizzled by (row >> 2) & 3 (fragment reads and
+// the copy are conflict-free), one barrier per chunk, the next chunk's B
+// rows (from L2) loaded during the current chunk's products.  A's rows past
+// arows read as zero through a buffer resource; its columns past K are
+// zeroed at the split.  Ends with a barrier.  (B's fragments read straight
+// from L2 by every wave instead of through LDS moved four times the bytes
+// and took 60 ms against 37.5 at N = 1M, r = 2000.)
+// bf16 elements of one chunk of split pivot rows: 3 parts x 128 rows x 32
+constexpr int X6_CHUNK = 3 * 128 * 32;
+
 template <typename G>
 struct X6Stage {
     static constexpr int BM = G::NBM, BN = G::NBN, KC = 32, MT = 16;
     static constexpr int ROWB = KC * 2;  // bytes per row of one part
-    static constexpr int PART = (BM + BN) * ROWB;
-    static constexpr int BYTES = 3 * PART;
-    static constexpr int NVA = BM * (KC / 4) / 256;      // A float4 loads per thread
+    static constexpr int PART = BN * ROWB;
+    static constexpr int STAGE = 3 * PART;
+    static constexpr int BYTES = 2 * STAGE;
     static constexpr int NVB = 3 * BN * (KC / 8) / 256;  // B 16-byte part loads per thread
     static_assert(G::NKC == KC && G::M::MT == MT && G::NBM == BM, "16x16 tiles, KC = 32");
-    static_assert((BM * (KC / 4)) % 256 == 0 && (3 * BN * (KC / 8)) % 256 == 0, "load shape");
+    static_assert((3 * BN * (KC / 8)) % 256 == 0, "load shape");
 };
 
 template <typename G, int NACT>
 __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A, int64_t lda, int64_t arows,
-                                        const __bf16* Bx, int64_t ldb, int64_t part_stride, int64_t K)
+                                        const __bf16* Bx, int64_t K)
 {
     typedef X6Stage<G> S;
-    constexpr int BM = S::BM, KC = S::KC, NVA = S::NVA, NVB = S::NVB, TM = G::NTM;
+    constexpr int BM = S::BM, KC = S::KC, NVB = S::NVB, TM = G::NTM;
+    static_assert(NVB == 6 && S::BN == 128, "B copy: 3 parts x 2 row halves per thread");
     typedef __attribute__((ext_vector_type(4))) float f4;
     typedef __attribute__((ext_vector_type(4))) unsigned u4;
-    const int64_t nk = (K + KC - 1) / KC, nfull = K / KC;
+    const int nk = (int)((K + KC - 1) / KC), nfull = (int)(K / KC);
     if (nk <= 0) return;
+    const int tid = threadIdx.x;
     const __amdgpu_buffer_rsrc_t rsA = G::rows_rsrc(A, lda, arows, BM);
+    // B: the chunks that exist (the loads past the last one read zero)
     const __amdgpu_buffer_rsrc_t rsB = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<__bf16*>(Bx), (short)0, (int)(3 * part_stride * 2), 0x00020000);
-    f4 rg[NVA];
-    u4 rb[NVB];
-    auto load = [&](int64_t c) {
-        const int64_t k0 = c * KC;
+        const_cast<__bf16*>(Bx), (short)0, nk * X6_CHUNK * 2, 0x00020000);
+    // A: byte offset of the lane's row of tile x and its K group; the chunk's
+    // K offset is the uniform soffset
+    uint32_t aoff[TM];
 #pragma unroll
-        for (int j = 0; j < NVA; ++j) {
-            const int lin = threadIdx.x + 256 * j;
-            const int row = lin / (KC / 4), cv = lin % (KC / 4);
-            const int64_t kk = k0 + 4 * cv;
-            rg[j] = __builtin_bit_cast(
-                f4, __builtin_amdgcn_raw_buffer_load_b128(rsA, (uint32_t)(((int64_t)row * lda + kk) * 4), 0, 0));
-            if (c >= nfull) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (kk + q >= K) rg[j][q] = 0.0f;
-            }
-        }
-        // B: part q, row, 16-byte slot (8 K values); no mask (zero past K)
-#pragma unroll
-        for (int j = 0; j < NVB; ++j) {
-            const int lin = threadIdx.x + 256 * j;
-            const int q = lin / (S::BN * 4), rem = lin % (S::BN * 4), row = rem / 4, sl = rem % 4;
-            rb[j] = __builtin_amdgcn_raw_buffer_load_b128(
-                rsB, (uint32_t)((q * part_stride + (int64_t)row * ldb + k0 + 8 * sl) * 2), 0, 0);
-        }
-    };
-    auto store = [&]() {
-        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-#pragma unroll
-        for (int j = 0; j < NVB; ++j) {
-            const int lin = threadIdx.x + 256 * j;
-            const int q = lin / (S::BN * 4), rem = lin % (S::BN * 4), row = rem / 4, sl = rem % 4;
-            const int srow = BM + row;
-            *reinterpret_cast<u4*>(lds + q * S::PART + srow * S::ROWB + ((sl ^ ((srow >> 2) & 3)) << 4)) = rb[j];
-        }
-#pragma unroll
-        for (int j = 0; j < NVA; ++j) {
-            const int lin = threadIdx.x + 256 * j;
-            const int srow = lin / (KC / 4), cv = lin % (KC / 4);  // A rows
-            bf16x4 p0, p1, p2;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                __bf16 a0, a1, a2;
-                bf16_split3(rg[j][q], a0, a1, a2);
-                p0[q] = a0, p1[q] = a1, p2[q] = a2;
-            }
-            unsigned char* dst = lds + srow * S::ROWB + (((cv >> 1) ^ ((srow >> 2) & 3)) << 4) + (cv & 1) * 8;
-            *reinterpret_cast<bf16x4*>(dst) = p0;
-            *reinterpret_cast<bf16x4*>(dst + S::PART) = p1;
-            *reinterpret_cast<bf16x4*>(dst + 2 * S::PART) = p2;
-        }
-    };
-    // a fragment row is 16 t + li: its slot swizzle depends on li only
-    const int slot = (g.h ^ ((g.li >> 2) & 3)) << 4;
-    auto compute = [&]() {
-        bf16x8 af[TM][3];
+    for (int x = 0; x < TM; ++x) aoff[x] = (uint32_t)(((g.wm * TM * 16 + x * 16 + g.li) * lda + 8 * g.h) * 4);
+    auto loadA = [&](f4 (&r)[TM][2], int c) {
+        const int kb = c * KC * 4;
 #pragma unroll
         for (int x = 0; x < TM; ++x)
 #pragma unroll
-            for (int q = 0; q < 3; ++q)
-                af[x][q] = *reinterpret_cast<const bf16x8*>(lds + q * S::PART +
-                                                            (g.wm * TM * 16 + x * 16 + g.li) * S::ROWB + slot);
+            for (int hh = 0; hh < 2; ++hh)
+                r[x][hh] = __builtin_bit_cast(
+                    f4, __builtin_amdgcn_raw_buffer_load_b128(rsA, aoff[x] + 16 * hh, kb, 0));
+    };
+    // B: load j of thread tid reads bytes 4096 j + 16 tid of the chunk, i.e.
+    // 16-byte slot tid % 4 of row tid / 4 + 64 (j % 2) of part j / 2; the
+    // row's slot swizzle (row >> 2) & 3 is the same for both row halves
+    const int bst = (tid >> 2) * S::ROWB + (((tid & 3) ^ ((tid >> 4) & 3)) << 4);
+    u4 rb[NVB];
+    auto loadB = [&](int c) {
+#pragma unroll
+        for (int j = 0; j < NVB; ++j)
+            rb[j] = __builtin_amdgcn_raw_buffer_load_b128(rsB, (uint32_t)(tid * 16), c * X6_CHUNK * 2 + j * 4096, 0);
+    };
+    auto storeB = [&](int stage) {
+        unsigned char* base = lds + stage * S::STAGE + bst;
+#pragma unroll
+        for (int j = 0; j < NVB; ++j)
+            *reinterpret_cast<u4*>(base + (j >> 1) * S::PART + (j & 1) * 64 * S::ROWB) = rb[j];
+    };
+    bf16x8 af[TM][3];
+    auto split = [&](const f4 (&r)[TM][2], int c) {
+        const bool tail = c >= nfull;
+        const int kl = (int)K - c * KC - 8 * g.h;  // K values of the lane's group still below K
+#pragma unroll
+        for (int x = 0; x < TM; ++x)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                float v = r[x][e / 4][e % 4];
+                v = tail && e >= kl ? 0.0f : v;
+                __bf16 a0, a1, a2;
+                bf16_split3(v, a0, a1, a2);
+                af[x][0][e] = a0, af[x][1][e] = a1, af[x][2][e] = a2;
+            }
+    };
+    // a fragment row is 16 t + li: its slot swizzle depends on li only
+    const int slot = (g.h ^ ((g.li >> 2) & 3)) << 4;
+    auto compute = [&](int stage) {
+        const unsigned char* base = lds + stage * S::STAGE + g.li * S::ROWB + slot;
         // two column tiles at a time: four independent accumulators between
         // two MFMAs into the same one
 #pragma unroll
@@ -171,8 +168,7 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
                 if (y + yo < NACT)
 #pragma unroll
                     for (int q = 0; q < 3; ++q)
-                        bf[yo][q] = *reinterpret_cast<const bf16x8*>(
-                            lds + q * S::PART + (BM + (y + yo) * 16 + g.li) * S::ROWB + slot);
+                        bf[yo][q] = *reinterpret_cast<const bf16x8*>(base + q * S::PART + (y + yo) * 16 * S::ROWB);
 #pragma unroll
             for (int p = 0; p < X6_NPROD; ++p)
 #pragma unroll
@@ -184,25 +180,53 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
             __builtin_amdgcn_sched_barrier(0);
         }
     };
-    // A second register set (loads two chunks ahead) does not fit beside
-    // the accumulators and fragments at two waves per SIMD (it spills).
-    load(0);
-    for (int64_t c = 0; c < nk; ++c) {
-        if (c > 0) __syncthreads();  // every wave is done with chunk c - 1
-        store();
-        __syncthreads();
-        if (c + 1 < nk) load(c + 1);
-        compute();
-    }
+    // Chunk c: B(c) is in stage c & 1, A(c) in registers.  Per chunk the
+    // loads go out as B(c + 1), then A(c + 2): the wait for B(c + 1) before
+    // its copy to LDS then leaves A(c + 2) in flight (the memory counter
+    // completes in order), so each A chunk has two chunks' time to arrive.
+    // Every step of the loop issues the same loads, also past the last
+    // chunk (the buffer resources keep them in range; those values are never
+    // multiplied), and the loop runs whole pairs of steps with an odd last
+    // chunk after it: a load or split on only some paths would leave the
+    // counter state at the loop head unknown or keep two register sets of
+    // one operand live, and the compiler would then wait for every load in
+    // flight.
+    auto step = [&](f4 (&r)[TM][2], int c) {
+        __builtin_amdgcn_sched_barrier(0);  // each step's code stays in its step
+        loadB(c + 1);
+        split(r, c);
+        // the next A loads reuse r's registers only if they stay behind
+        // the split (else the loop carries a copy that waits for them)
+        __builtin_amdgcn_sched_barrier(0);
+        loadA(r, c + 2);
+        compute(c & 1);
+        storeB((c + 1) & 1);
+        __syncthreads();  // stage (c + 1) & 1 written, stage c & 1 free
+    };
+    f4 ra0[TM][2], ra1[TM][2];
+    loadA(ra0, 0);
+    loadA(ra1, 1);
+    loadB(0);
+    storeB(0);
     __syncthreads();
+    int c = 0;
+    for (; c + 1 < nk; c += 2) {
+        step(ra0, c);
+        step(ra1, c + 1);
+    }
+    if (c < nk) {  // odd chunk count: the last chunk, in ra0
+        split(ra0, c);
+        compute(c & 1);
+        __syncthreads();
+    }
 }
 
 template <typename G, int N = G::NTN>
 __device__ inline void x6_nt_mainloop_live(G& g, unsigned char* lds, const float* A, int64_t lda, int64_t arows,
-                                           const __bf16* Bx, int64_t ldb, int64_t part_stride, int64_t K)
+                                           const __bf16* Bx, int64_t K)
 {
     if (g.nact == N)
-        x6_nt_mainloop_n<G, N>(g, lds, A, lda, arows, Bx, ldb, part_stride, K);
+        x6_nt_mainloop_n<G, N>(g, lds, A, lda, arows, Bx, K);
     else if constexpr (N > 1)
-        x6_nt_mainloop_live<G, N - 1>(g, lds, A, lda, arows, Bx, ldb, part_stride, K);
+        x6_nt_mainloop_live<G, N - 1>(g, lds, A, lda, arows, Bx, K);
 }
