@@ -513,8 +513,10 @@ kevd_status arpc_prepare(kevd_ctx* ctx, const kevd_arpc_params* p, int max_round
     KEVD_TRY(ctx_buf_t(ctx, "a_H", BMAX * BMAX, &B->H));
     KEVD_TRY(ctx_buf_t(ctx, "a_Linv64", BMAX * BMAX, &B->Linv64));
     KEVD_TRY(ctx_buf_t(ctx, "a_nrmS", BMAX, &B->nrmS));
-    KEVD_TRY(ctx_buf(ctx, "a_XS", (size_t)BMAX * ctx->dp * esz, &B->XS));
-    // mixed F form: the pivot rows' bf16x6 parts follow them
+    // mixed F form: the pivot points' and pivot rows' bf16x6 parts follow them
+    KEVD_TRY(ctx_buf(ctx, "a_XS",
+                     !p->regenerate && p->prec == KEVD_MIXED ? x6_xs_bytes(ctx->dp) : (size_t)BMAX * ctx->dp * esz,
+                     &B->XS));
     KEVD_TRY(ctx_buf(ctx, "a_FS",
                      p->regenerate ? (size_t)BMAX * 64 * esz
                                    : (p->prec == KEVD_MIXED ? x6_fs_bytes(ldf) : (size_t)BMAX * ldf * esz),

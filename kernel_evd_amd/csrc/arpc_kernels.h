@@ -502,9 +502,18 @@ __device__ inline const __bf16* x6_fsx(const float* FS, int64_t ldf)
     return reinterpret_cast<const __bf16*>(FS + (int64_t)BMAX * ldf);
 }
 
+// Likewise the parts of the pivot points X(S_i,:) follow XS (BMAX x dp FP32),
+// chunk-major: the downdate's distance product runs as bf16x6 too.
+__host__ __device__ inline size_t x6_xs_bytes(int32_t dp) { return (size_t)BMAX * dp * (4 + 3 * 2); }
+__device__ inline __bf16* x6_xsx(float* XS, int32_t dp) { return reinterpret_cast<__bf16*>(XS + (int64_t)BMAX * dp); }
+__device__ inline const __bf16* x6_xsx(const float* XS, int32_t dp)
+{
+    return reinterpret_cast<const __bf16*>(XS + (int64_t)BMAX * dp);
+}
+
 // Pivot rows in the GEMM precision: X(S_i,:), |x|^2, F(S_i, 0:Nr).  Rows
-// past k are zero.  X6: also F(S_i,:)'s bf16x6 parts, zero from Nr up to a
-// multiple of 32.
+// past k are zero.  X6: also the bf16x6 parts of X(S_i,:) and of F(S_i,:),
+// the latter zero from Nr up to a multiple of 32.
 template <typename T, bool X6 = false>
 __device__ inline void gather_pivots_body(const T* __restrict__ Xsrc, const double* __restrict__ nrm,
                                           int32_t dp, const T* __restrict__ F, int64_t ldf,
@@ -519,6 +528,15 @@ __device__ inline void gather_pivots_body(const T* __restrict__ Xsrc, const doub
     if (threadIdx.x == 0) nrmS[c] = s >= 0 ? nrm[s] : 0.0;
     for (int64_t m = threadIdx.x; m < Nr; m += 256) FS[c * ldf + m] = s >= 0 ? F[s * ldf + m] : (T)0;
     if constexpr (X6 && std::is_same<T, float>::value) {
+        __bf16* xx = x6_xsx(XS, dp);  // dp is a multiple of 32
+        for (int j = threadIdx.x; j < dp; j += 256) {
+            __bf16 h, md, l;
+            bf16_split3(s >= 0 ? Xsrc[s * dp + j] : 0.0f, h, md, l);
+            __bf16* e = xx + (j / 32) * X6_CHUNK + c * 32 + j % 32;
+            e[0] = h;
+            e[X6_CHUNK / 3] = md;
+            e[2 * X6_CHUNK / 3] = l;
+        }
         __bf16* fx = x6_fsx(FS, ldf);
         const int64_t kp = (Nr + 31) / 32 * 32;
         for (int64_t m = threadIdx.x; m < kp; m += 256) {
@@ -558,12 +576,14 @@ template <typename T> struct DowndateEp {
 // First stage of the Step-3 update of BM rows: the epilogue constants go to
 // LDS and acc = -K(R, S_i) (distance product on MFMA, K = dp, then norms
 // and the compensated exp; exactly 1 on a row's own pivot).
+// XSx (FP32 only, else null): the pivot points' bf16x6 parts, chunk-major;
+// the distance product then runs as bf16x6 on the bf16 matrix cores.
 template <typename T>
 __device__ inline void dd_prologue(typename Cfg<T>::G& g, const T* __restrict__ X, int32_t dp,
                                    const double* __restrict__ nrm, const double* __restrict__ d, int64_t n,
                                    const T* __restrict__ XS, const double* __restrict__ nrmS,
                                    const int64_t* __restrict__ Si, int k, GaussScale gs, int64_t r0, T* lds,
-                                   DowndateEp<T>* ep)
+                                   DowndateEp<T>* ep, const __bf16* XSx = nullptr)
 {
     typedef typename Cfg<T>::G G;
     constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
@@ -579,7 +599,14 @@ __device__ inline void dd_prologue(typename Cfg<T>::G& g, const T* __restrict__ 
     g.init();
     g.set_live_cols(k);
     // distance product (its barriers also publish ep)
-    g.mainloop(lds, X + r0 * dp, dp, rows, XS, dp, BN, dp);
+    if constexpr (std::is_same<T, float>::value) {
+        if (XSx)
+            x6_nt_mainloop_live<G>(g, reinterpret_cast<unsigned char*>(lds), X + r0 * dp, dp, rows, XSx, dp);
+        else
+            g.mainloop(lds, X + r0 * dp, dp, rows, XS, dp, BN, dp);
+    } else {
+        g.mainloop(lds, X + r0 * dp, dp, rows, XS, dp, BN, dp);
+    }
     // The lane's row and column constants are read once; the element loop
     // is then branch-free (out-of-range rows and columns are masked to 0).
     double nr[G::NTM][G::M::ACC], ns[G::NTN];
@@ -701,7 +728,7 @@ __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64
 
 // The fused Step-3 update for BM rows (ks_arpclm.cu:929-995 and
 // update_diag_block 455-478):
-//   acc  = X_R X_{S_i}^T                    (MFMA, K = dp)
+//   acc  = X_R X_{S_i}^T                    (MFMA, K = dp; bf16x6 with X6)
 //   acc  = -K(R, S_i)                       (epilogue: norms, exp)
 //   acc += F_R F_{S_i}^T                    (MFMA, K = Nr)
 //   T    = -acc -> LDS
@@ -710,8 +737,8 @@ __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64
 // This is row block blk (rows blk * BM ...) of one problem; it returns
 // early (before any barrier) when the problem is done or accepted nothing
 // this round.
-// X6: the overlap of an FP32 factor on the bf16 matrix cores (bf16x6
-// products, bf16x6.h) instead of FP32 MFMA.
+// X6: the distance product and the overlap of an FP32 factor on the bf16
+// matrix cores (bf16x6 products, bf16x6.h) instead of FP32 MFMA.
 template <typename T, bool X6 = false>
 __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const double* __restrict__ nrm,
                                      T* __restrict__ F, int64_t ldf, double* __restrict__ d, int64_t n,
@@ -728,7 +755,9 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
     int64_t Nr = st->Nr;
     int64_t r0 = blk * BM;
     G g;
-    dd_prologue<T>(g, X, dp, nrm, d, n, XS, nrmS, Si, k, gs, r0, lds, ep);
+    const __bf16* XSx = nullptr;
+    if constexpr (X6 && std::is_same<T, float>::value) XSx = x6_xsx(XS, dp);
+    dd_prologue<T>(g, X, dp, nrm, d, n, XS, nrmS, Si, k, gs, r0, lds, ep, XSx);
     // overlap with the current factor (ends with a barrier)
     if (Nr > 0) {
         if constexpr (X6 && std::is_same<T, float>::value) {
