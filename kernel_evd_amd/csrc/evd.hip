@@ -1189,6 +1189,52 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
 }
 
+// The U product with an FP32 factor on the bf16 matrix cores: k_ugemm<float>
+// with the bf16x6 main loop (bf16x6.h), for kc <= 128 output columns.  Yx
+// holds the B operand's parts chunk-major (k_pack_y_x6).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_ugemm_x6(
+    const float* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const __bf16* __restrict__ Yx, int kc, int kout,
+    const double* __restrict__ scale, float* U, double* P1)
+{
+    typedef Cfg<float>::G G;
+    constexpr int BM = Cfg<float>::BM;
+    static_assert(X6Stage<G>::BYTES <= G::LDS_ELEMS * (int)sizeof(float), "the x6 stage fits");
+    __shared__ __attribute__((aligned(16))) float lds[G::LDS_ELEMS];
+    __shared__ double scs[BM];  // the block's row scales
+    G g;
+    g.init();
+    int64_t r0 = (int64_t)blockIdx.x * BM;
+    for (int i = threadIdx.x; i < BM; i += 256) scs[i] = r0 + i < n ? scale[r0 + i] : 0.0;
+    g.set_live_cols(kc);
+    x6_nt_mainloop_live<G>(g, reinterpret_cast<unsigned char*>(lds), F + r0 * ldf, ldf, n - r0, Yx, r);
+    __syncthreads();  // scs
+    int64_t vrows = n - r0 < BM ? n - r0 : BM;
+    __amdgpu_buffer_rsrc_t rsU =
+        __builtin_amdgcn_make_buffer_rsrc(U + r0 * kout, (short)0, (int)(vrows * kout * 4), 0x00020000);
+    double sc[G::NTM][G::M::ACC];
+    int rl[G::NTM][G::M::ACC];
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int v = 0; v < G::M::ACC; ++v) {
+            rl[a][v] = g.row_of(a, v);
+            sc[a][v] = scs[rl[a][v]];
+        }
+#pragma unroll
+    for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+        for (int b = 0; b < G::NTN; ++b)
+#pragma unroll
+            for (int v = 0; v < G::M::ACC; ++v) {
+                int col = g.col_of(b);
+                double x = (double)g.acc[a][b][v] * sc[a][v];
+                if (col < kout)
+                    buffer_store((float)x, rsU, (uint32_t)((rl[a][v] * kout + col) * 4));
+                else if (P1 && col < kc && r0 + rl[a][v] < n)
+                    P1[r0 + rl[a][v]] = x;
+            }
+}
+
 __global__ void k_trace_shift(double* A, int64_t r, double* tr_out)
 {
     __shared__ double s[256];
@@ -1226,6 +1272,35 @@ __global__ void k_pack_y(const double* Y, int64_t r, int k, const double* colsca
     } else if (extra) {
         out[(int64_t)c * ldy + i] = (T)extra[i];
     }
+}
+
+// k_pack_y's B operand as bf16x6 parts for k_ugemm_x6: row c < 128 of the
+// operand (column c of Y scaled by colscale[c], or extra for c = k), split
+// into three bf16 parts, chunk-major (X6_CHUNK elements per 32 values of
+// r); values past r and rows past k (+1 with extra) are zero.  Thread per
+// (value, row).
+__global__ void k_pack_y_x6(const double* Y, int64_t r, int k, const double* colscale, const double* extra,
+                            int64_t nchunk, __bf16* out)
+{
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int c = blockIdx.y;
+    if (i >= nchunk * 32) return;
+    float v = 0.0f;
+    if (i < r) {
+        if (c < k) {
+            double x = Y[i + (int64_t)c * r];
+            if (colscale) x *= colscale[c];
+            v = (float)x;
+        } else if (c == k && extra) {
+            v = (float)extra[i];
+        }
+    }
+    __bf16 h, md, lo;
+    bf16_split3(v, h, md, lo);
+    __bf16* e = out + (i / 32) * X6_CHUNK + c * 32 + i % 32;
+    e[0] = h;
+    e[X6_CHUNK / 3] = md;
+    e[2 * X6_CHUNK / 3] = lo;
 }
 
 __global__ void k_rowsum_dev(const double* P1, int64_t n, double* bdev)
@@ -1624,6 +1699,24 @@ template <typename T> struct FactorOps {
     kevd_status ugemm(const double* Y, int k, const double* colscale, const double* extra, const double* rowscale,
                       float* U, double* P1)
     {
+        const int kc = k + (extra ? 1 : 0);
+        if constexpr (std::is_same<T, float>::value) {
+            if (use_bf16x6() && kc <= Cfg<float>::BN) {
+                // the product on the bf16 matrix cores
+                const int64_t nchunk = (src.r + 31) / 32;
+                __bf16* Yx;
+                KEVD_TRY(ctx_buf(ctx, "e_Ypackx6", (size_t)nchunk * X6_CHUNK * sizeof(__bf16), (void**)&Yx));
+                k_pack_y_x6<<<dim3((unsigned)((nchunk * 32 + 255) / 256), Cfg<float>::BN), 256, 0, ctx->stream>>>(
+                    Y, src.r, k, colscale, extra, nchunk, Yx);
+                {
+                    ProfScope ps(ctx, TC_UGEMM);
+                    k_ugemm_x6<<<(unsigned)((src.n + Cfg<float>::BM - 1) / Cfg<float>::BM), 256, 0, ctx->stream>>>(
+                        src.F, src.ldf, src.n, src.r, Yx, kc, k, rowscale, U, P1);
+                }
+                KEVD_HIP(ctx, hipGetLastError());
+                return KEVD_OK;
+            }
+        }
         T* Yp;
         int64_t ldy;
         KEVD_TRY(pack_y<T>(ctx, src.r, Y, k, colscale, extra, &Yp, &ldy));
