@@ -492,9 +492,9 @@ __device__ __attribute__((always_inline)) inline void rejchol_body(const double*
 
 // The bf16x6 parts of the pivot rows follow FS (BMAX x ldf FP32) in the
 // same buffer, chunk-major for the downdate's loads: the parts of columns
-// [32 c, 32 c + 32) of all BMAX rows are one contiguous 24 KB block [part]
-// [row][32] (X6_CHUNK elements), so that every load of the downdate reads
-// whole cache lines.
+// [32 c, 32 c + 32) of all BMAX rows are one contiguous 24 KB block in MFMA
+// fragment order (X6_CHUNK elements, x6_idx), so that every load of the
+// downdate reads whole cache lines.
 __host__ __device__ inline size_t x6_fs_bytes(int64_t ldf) { return (size_t)BMAX * ldf * (4 + 3 * 2); }
 __device__ inline __bf16* x6_fsx(float* FS, int64_t ldf) { return reinterpret_cast<__bf16*>(FS + (int64_t)BMAX * ldf); }
 __device__ inline const __bf16* x6_fsx(const float* FS, int64_t ldf)
@@ -532,7 +532,7 @@ __device__ inline void gather_pivots_body(const T* __restrict__ Xsrc, const doub
         for (int j = threadIdx.x; j < dp; j += 256) {
             __bf16 h, md, l;
             bf16_split3(s >= 0 ? Xsrc[s * dp + j] : 0.0f, h, md, l);
-            __bf16* e = xx + (j / 32) * X6_CHUNK + c * 32 + j % 32;
+            __bf16* e = xx + (j / 32) * X6_CHUNK + x6_idx(c, j % 32);
             e[0] = h;
             e[X6_CHUNK / 3] = md;
             e[2 * X6_CHUNK / 3] = l;
@@ -543,7 +543,7 @@ __device__ inline void gather_pivots_body(const T* __restrict__ Xsrc, const doub
             float v = s >= 0 && m < Nr ? F[s * ldf + m] : 0.0f;
             __bf16 h, md, l;
             bf16_split3(v, h, md, l);
-            __bf16* e = fx + (m / 32) * X6_CHUNK + c * 32 + m % 32;
+            __bf16* e = fx + (m / 32) * X6_CHUNK + x6_idx(c, (int)(m % 32));
             e[0] = h;
             e[X6_CHUNK / 3] = md;
             e[2 * X6_CHUNK / 3] = l;

@@ -60,8 +60,8 @@ __device__ inline void mfma_bf16_16(floatx4& c, const bf16x8& a, const bf16x8& b
 // layout, so the caller's epilogues are unchanged).  NACT live column tiles.
 // A is FP32 with K-contiguous rows (the factor's row block); B (BN = 128
 // rows) comes already split, chunk-major: the parts of K values
-// [32 c, 32 c + 32) are the contiguous block Bx + c X6_CHUNK, laid out
-// [part][row][32], zero from K up to a multiple of 32 (the downdate's pivot
+// [32 c, 32 c + 32) are the contiguous block Bx + c X6_CHUNK in fragment
+// order (x6_idx), zero from K up to a multiple of 32 (the downdate's pivot
 // rows, split once per round when they are gathered).
 //
 // A row is used by one wave only (the waves split the rows), so A never
@@ -70,16 +70,25 @@ __device__ inline void mfma_bf16_16(floatx4& c, const bf16x8& a, const bf16x8& b
 // 128-byte row segments) straight from HBM into registers, two chunks
 // ahead, and splits them into the three bf16 fragments in registers.  Only
 // B, which every wave reads, is staged (its 24 KB chunk is one contiguous
-// block, so its loads read whole cache lines): two LDS stages [part][row][32] bf16
-// whose 16-byte slots are XOR-swizzled by (row >> 2) & 3 (fragment reads and
-// the copy are conflict-free), one barrier per chunk, the next chunk's B
+// block, so its loads read whole cache lines): two LDS stages holding the
+// chunk as it lies in memory, read lane-linearly, one barrier per chunk, the next chunk's B
 // rows (from L2) loaded during the current chunk's products.  A's rows past
 // arows read as zero through a buffer resource; its columns past K are
 // zeroed at the split.  Ends with a barrier.  (B's fragments read straight
 // from L2 by every wave instead of through LDS moved four times the bytes
 // and took 60 ms against 37.5 at N = 1M, r = 2000.)
-// bf16 elements of one chunk of split pivot rows: 3 parts x 128 rows x 32
+// bf16 elements of one chunk of split B rows: 3 parts x 128 rows x 32.
+// Within a chunk the layout is the MFMA fragment order: part, then the
+// 16-row tile y, the 8-value K group h, the row i in the tile and the 8
+// values, so the fragment of (part, y) is 1 KB read by lane 16 h + i at
+// 16 (16 h + i), lane-linear as LDS reads serve fastest, and the copy into
+// LDS is linear too.
 constexpr int X6_CHUNK = 3 * 128 * 32;
+// element index of (row, K value kk) of part 0 in a chunk (kk < 32)
+__host__ __device__ constexpr int x6_idx(int row, int kk)
+{
+    return (((row >> 4) * 4 + (kk >> 3)) * 16 + (row & 15)) * 8 + (kk & 7);
+}
 
 template <typename G>
 struct X6Stage {
@@ -115,7 +124,9 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
 #pragma unroll
     for (int x = 0; x < TM; ++x) aoff[x] = (uint32_t)(((g.wm * TM * 16 + x * 16 + g.li) * lda + 8 * g.h) * 4);
     auto loadA = [&](f4 (&r)[TM][2], int c) {
-        const int kb = c * KC * 4;
+        // past the last chunk: the last chunk again (an L2 hit, not a
+        // second read of the row from HBM)
+        const int kb = (c < nk ? c : nk - 1) * KC * 4;
 #pragma unroll
         for (int x = 0; x < TM; ++x)
 #pragma unroll
@@ -123,10 +134,7 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
                 r[x][hh] = __builtin_bit_cast(
                     f4, __builtin_amdgcn_raw_buffer_load_b128(rsA, aoff[x] + 16 * hh, kb, 0));
     };
-    // B: load j of thread tid reads bytes 4096 j + 16 tid of the chunk, i.e.
-    // 16-byte slot tid % 4 of row tid / 4 + 64 (j % 2) of part j / 2; the
-    // row's slot swizzle (row >> 2) & 3 is the same for both row halves
-    const int bst = (tid >> 2) * S::ROWB + (((tid & 3) ^ ((tid >> 4) & 3)) << 4);
+    // B: load j of thread tid moves bytes 4096 j + 16 tid of the chunk
     u4 rb[NVB];
     auto loadB = [&](int c) {
 #pragma unroll
@@ -134,10 +142,9 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
             rb[j] = __builtin_amdgcn_raw_buffer_load_b128(rsB, (uint32_t)(tid * 16), c * X6_CHUNK * 2 + j * 4096, 0);
     };
     auto storeB = [&](int stage) {
-        unsigned char* base = lds + stage * S::STAGE + bst;
+        unsigned char* base = lds + stage * S::STAGE + tid * 16;
 #pragma unroll
-        for (int j = 0; j < NVB; ++j)
-            *reinterpret_cast<u4*>(base + (j >> 1) * S::PART + (j & 1) * 64 * S::ROWB) = rb[j];
+        for (int j = 0; j < NVB; ++j) *reinterpret_cast<u4*>(base + j * 4096) = rb[j];
     };
     bf16x8 af[TM][3];
     auto split = [&](const f4 (&r)[TM][2], int c) {
@@ -154,10 +161,8 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
                 af[x][0][e] = a0, af[x][1][e] = a1, af[x][2][e] = a2;
             }
     };
-    // a fragment row is 16 t + li: its slot swizzle depends on li only
-    const int slot = (g.h ^ ((g.li >> 2) & 3)) << 4;
     auto compute = [&](int stage) {
-        const unsigned char* base = lds + stage * S::STAGE + g.li * S::ROWB + slot;
+        const unsigned char* base = lds + stage * S::STAGE + (threadIdx.x & 63) * 16;
         // two column tiles at a time: four independent accumulators between
         // two MFMAs into the same one
 #pragma unroll
@@ -168,7 +173,7 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
                 if (y + yo < NACT)
 #pragma unroll
                     for (int q = 0; q < 3; ++q)
-                        bf[yo][q] = *reinterpret_cast<const bf16x8*>(base + q * S::PART + (y + yo) * 16 * S::ROWB);
+                        bf[yo][q] = *reinterpret_cast<const bf16x8*>(base + q * S::PART + (y + yo) * 1024);
 #pragma unroll
             for (int p = 0; p < X6_NPROD; ++p)
 #pragma unroll

@@ -1276,9 +1276,9 @@ __global__ void k_pack_y(const double* Y, int64_t r, int k, const double* colsca
 
 // k_pack_y's B operand as bf16x6 parts for k_ugemm_x6: row c < 128 of the
 // operand (column c of Y scaled by colscale[c], or extra for c = k), split
-// into three bf16 parts, chunk-major (X6_CHUNK elements per 32 values of
-// r); values past r and rows past k (+1 with extra) are zero.  Thread per
-// (value, row).
+// into three bf16 parts, chunk-major in fragment order (X6_CHUNK elements
+// per 32 values of r, x6_idx); values past r and rows past k (+1 with
+// extra) are zero.  Thread per (value, row).
 __global__ void k_pack_y_x6(const double* Y, int64_t r, int k, const double* colscale, const double* extra,
                             int64_t nchunk, __bf16* out)
 {
@@ -1297,7 +1297,7 @@ __global__ void k_pack_y_x6(const double* Y, int64_t r, int k, const double* col
     }
     __bf16 h, md, lo;
     bf16_split3(v, h, md, lo);
-    __bf16* e = out + (i / 32) * X6_CHUNK + c * 32 + i % 32;
+    __bf16* e = out + (i / 32) * X6_CHUNK + x6_idx(c, (int)(i % 32));
     e[0] = h;
     e[X6_CHUNK / 3] = md;
     e[2 * X6_CHUNK / 3] = lo;
