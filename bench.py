@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # profiles/traffic_k_downdate_x6_k_gram_x6_k_sinkhorn_ks_n1M_r2000.md (bf16x6
 # kernels) and profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md
 # (the FP32-MFMA kernels, KEVD_FP32_MFMA=1).
-TRAFFIC_BYTES = {"k_downdate<float, true>": 4.290e9, "k_gram_x6": 47.68e9,
+TRAFFIC_BYTES = {"k_downdate<float, true>": 4.375e9, "k_gram_x6": 33.55e9,
                  "k_downdate<float, false>": 4.276e9, "k_gram<float,1>": 110.89e9,
                  "k_sinkhorn_rows<float,8>": 8.508e9}
 FP32_MFMA_PEAK_TF = 157.3
@@ -567,14 +567,15 @@ def main():
                           "algorithmic_bytes": 4.0 * args.n * a_mix.rank},
         # the step's three GEMMs on the factor together: the downdate that
         # builds F, the Gram F^T S^2 F (or both two-step Grams) and U = S F V
-        "factor_gemms": {"kernels": [dd_kernel, gr_kernel, "k_ugemm<float>"],
+        "factor_gemms": {"kernels": [dd_kernel, gr_kernel, "k_ugemm_x6" if X6 else "k_ugemm<float>"],
                          "flop": dd_flop + gr_flop + 2.0 * args.n * a_mix.rank * (args.kout + 1),
                          "ms": round(dd_ms + gr_ms + agg["u_ms"] / steps, 3),
                          "achieved": round((dd_flop + gr_flop + 2.0 * args.n * a_mix.rank * (args.kout + 1))
                                            / ((dd_ms + gr_ms + agg["u_ms"] / steps) * 1e-3) / 1e12, 3)
                          if dd_ms + gr_ms > 0 else None,
                          "unit": "TFLOP/s", "peak": FP32_MFMA_PEAK_TF,
-                         "peak_note": "FP32 MFMA peak, for comparison across builds (the U product stays FP32)"},
+                         "peak_note": "FP32 MFMA peak, for comparison across builds (all three run as bf16x6 "
+                                      "unless KEVD_FP32_MFMA=1)"},
         "measured_peaks": {"mfma_f32_16x16x4_tflops": round(peaks["mfma_f32_tflops"], 2),
                            "mfma_bf16_16x16x32_tflops": round(peaks["mfma_bf16_tflops"], 2),
                            "mfma_bf16_32x32x16_tflops": round(peaks["mfma_bf16_32_tflops"], 2),
