@@ -93,13 +93,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const T* __restrict__ X, int32_t dp, const double* __restrict__ nrm, T* __restrict__ F, int64_t ldf,
     double* __restrict__ d, int64_t n, const T* __restrict__ XS, const double* __restrict__ nrmS,
     const int64_t* __restrict__ Si, const T* __restrict__ FS, const T* __restrict__ LinvT, const ArpcState* st,
-    GaussScale gs)
+    GaussScale gs, double* __restrict__ cspart)
 {
     typedef DowndateLds<T> DL;
     __shared__ __attribute__((aligned(16))) T lds[DL::ELEMS];
     __shared__ double rs[DL::WN][DL::BM];
     __shared__ DowndateEp<T> ep;
-    downdate_body<T, X6>(X, dp, nrm, F, ldf, d, n, XS, nrmS, Si, FS, LinvT, st, gs, blockIdx.x, lds, rs, &ep);
+    downdate_body<T, X6>(X, dp, nrm, F, ldf, d, n, XS, nrmS, Si, FS, LinvT, st, gs, blockIdx.x, lds, rs, &ep,
+                         cspart);
 }
 
 __global__ __launch_bounds__(BMAX) void k_commit(ArpcState* st, const int64_t* __restrict__ Si, int64_t* pivots,
@@ -248,10 +249,10 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
             ProfScope ps(ctx, TC_DOWNDATE);
             if (x6)
                 k_downdate<T, true><<<gdown, 256, 0, s>>>(Xsrc, ctx->dp, nrm, F, ctx->ldf, d, n, XS, nrmS, Si,
-                                                          FS, LinvT, st, gs);
+                                                          FS, LinvT, st, gs, B.cspart);
             else
                 k_downdate<T, false><<<gdown, 256, 0, s>>>(Xsrc, ctx->dp, nrm, F, ctx->ldf, d, n, XS, nrmS, Si,
-                                                           FS, LinvT, st, gs);
+                                                           FS, LinvT, st, gs, B.cspart);
         }
         k_commit<<<1, BMAX, 0, s>>>(st, Si, pivots_dev, acc_dev);
     };
@@ -483,6 +484,7 @@ kevd_status arpc_prepare(kevd_ctx* ctx, const kevd_arpc_params* p, int max_round
         }
     }
     ctx->have_factor = false;
+    ctx->cs_valid = false;
     ctx->lm = false;
     ctx->lm_linv = nullptr;
     ctx->ldf = ldf;
@@ -522,6 +524,13 @@ kevd_status arpc_prepare(kevd_ctx* ctx, const kevd_arpc_params* p, int max_round
                                    : (p->prec == KEVD_MIXED ? x6_fs_bytes(ldf) : (size_t)BMAX * ldf * esz),
                      &B->FS));
     KEVD_TRY(ctx_buf(ctx, "a_LinvT", (size_t)BMAX * BMAX * esz, &B->LinvT));
+    // F form: the downdate leaves F^T 1 as per-row-block partials
+    B->cspart = nullptr;
+    if (!p->regenerate) {
+        const int64_t bm = p->prec == KEVD_MIXED ? Cfg<float>::BM : Cfg<double>::BM;
+        KEVD_TRY(ctx_buf_t(ctx, "a_cspart", (size_t)((n + bm - 1) / bm) * ldf, &B->cspart));
+        ctx->cs_nblk = (n + bm - 1) / bm;
+    }
 
     ArpcState init{};
     init.rank = p->rank;
@@ -546,6 +555,8 @@ kevd_status arpc_finish(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds
     ctx->rank = fin.Nr;
     ctx->pivots.assign(res->pivots, res->pivots + fin.Nr);
     ctx->have_factor = fin.Nr > 0;
+    ctx->cs_part = B.cspart;
+    ctx->cs_valid = B.cspart != nullptr && fin.Nr > 0;
     ctx->factor_eps = p->eps;
     // Algorithmic flop of the downdate: per round 2 n k Nr (overlap) +
     // 2 n k dim (distances) + n k^2 (triangular solve) + 2 n k (norms).

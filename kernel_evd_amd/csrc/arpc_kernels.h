@@ -647,10 +647,14 @@ __device__ inline void dd_prologue(typename Cfg<T>::G& g, const T* __restrict__ 
 // the caller, which leaves the LDS free behind a barrier), T goes to LDS,
 // G = T L_c^-T (L_c^-T is triangular), G is written to F(R, Nr:Nr+k) when F
 // is given, and d(R) = max(d(R) - rowsq(G), 0) (the old d(R) from ep).
+// csrow (optional): this block's row of the F^T 1 partials; its entries
+// Nr .. Nr+k-1 get the column sums of G over the block's rows (FP64 sums of
+// the FP32 or FP64 values written to F), so the kEVD's degree pass F^T 1
+// need not read F again.
 template <typename T>
 __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64_t ldf, double* __restrict__ d,
                                  int64_t n, const T* __restrict__ LinvT, int k, int64_t Nr, int64_t r0, T* lds,
-                                 double (*rs)[Cfg<T>::BM], const DowndateEp<T>* ep)
+                                 double (*rs)[Cfg<T>::BM], const DowndateEp<T>* ep, double* __restrict__ csrow = nullptr)
 {
     typedef typename Cfg<T>::G G;
     typedef DowndateLds<T> DL;
@@ -713,7 +717,38 @@ __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64
             for (int o = 1; o < MT; o <<= 1) s += __shfl_xor(s, o, 64);
             if (g.li == 0) rs[g.wn][g.row_of(a, v)] = s;
         }
-    __syncthreads();
+    // column sums of G: a lane's rows, then the MT-lane groups that share a
+    // column (lane % MT), in a fixed order
+    double cs[G::NTN];
+    if (csrow) {
+#pragma unroll
+        for (int b = 0; b < G::NTN; ++b) {
+            double c = 0.0;
+#pragma unroll
+            for (int a = 0; a < G::NTM; ++a)
+#pragma unroll
+                for (int v = 0; v < G::M::ACC; ++v) c += (double)g.acc[a][b][v];
+#pragma unroll
+            for (int o = MT; o < 64; o <<= 1) c += __shfl_xor(c, o, 64);
+            cs[b] = c;
+        }
+    }
+    __syncthreads();  // every wave is past its product: the LDS tile is free
+    if (csrow) {
+        // the waves' column sums through LDS, added in wave order
+        constexpr int BN = Cfg<T>::BN, NW = 4 / WN;
+        double* cw = reinterpret_cast<double*>(lds);
+        if (g.h == 0)
+#pragma unroll
+            for (int b = 0; b < G::NTN; ++b) cw[g.wm * BN + g.col_of(b)] = cs[b];
+        __syncthreads();
+        for (int c = threadIdx.x; c < k; c += 256) {
+            double sum = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) sum += cw[w * BN + c];
+            csrow[Nr + c] = sum;
+        }
+    }
     for (int r = threadIdx.x; r < BM; r += 256) {
         int64_t row = r0 + r;
         if (row < n) {
@@ -736,7 +771,8 @@ __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64
 //   F(R, Nr:Nr+k) = acc;  d(R) = max(d(R) - rowsq(acc), 0)
 // This is row block blk (rows blk * BM ...) of one problem; it returns
 // early (before any barrier) when the problem is done or accepted nothing
-// this round.
+// this round.  cspart (optional): the F^T 1 partials, row blk at
+// cspart + blk * ldf (dd_finish).
 // X6: the distance product and the overlap of an FP32 factor on the bf16
 // matrix cores (bf16x6 products, bf16x6.h) instead of FP32 MFMA.
 template <typename T, bool X6 = false>
@@ -745,7 +781,8 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
                                      const T* __restrict__ XS, const double* __restrict__ nrmS,
                                      const int64_t* __restrict__ Si, const T* __restrict__ FS,
                                      const T* __restrict__ LinvT, const ArpcState* st, GaussScale gs,
-                                     int64_t blk, T* lds, double (*rs)[Cfg<T>::BM], DowndateEp<T>* ep)
+                                     int64_t blk, T* lds, double (*rs)[Cfg<T>::BM], DowndateEp<T>* ep,
+                                     double* __restrict__ cspart)
 {
     typedef typename Cfg<T>::G G;
     constexpr int BM = Cfg<T>::BM, BN = Cfg<T>::BN;
@@ -768,7 +805,7 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
             g.mainloop_live(lds, F + r0 * ldf, ldf, n - r0, FS, ldf, BN, Nr);
         }
     }
-    dd_finish<T>(g, F, ldf, d, n, LinvT, k, Nr, r0, lds, rs, ep);
+    dd_finish<T>(g, F, ldf, d, n, LinvT, k, Nr, r0, lds, rs, ep, cspart ? cspart + blk * ldf : nullptr);
 }
 
 // ------------------------------------------- regenerating (LM) variant
@@ -989,6 +1026,7 @@ struct ArpcBufs {
     uint64_t *prefix = nullptr, *btot = nullptr;
     int64_t *Sprime = nullptr, *Si = nullptr;
     void *XS = nullptr, *FS = nullptr, *LinvT = nullptr;  // GEMM precision
+    double* cspart = nullptr;     // F^T 1 partials, ceil(n / BM) x ldf (F form)
 };
 
 // Checks the parameters and result arrays as kevd_arpc does; returns the

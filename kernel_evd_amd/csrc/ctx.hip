@@ -313,6 +313,7 @@ static kevd_status set_points_impl(kevd_ctx* ctx, const double* X64h, const floa
     ctx->X32 = nullptr; ctx->X64 = nullptr; ctx->nrm32 = nullptr; ctx->nrm64 = nullptr;
     ctx->F = nullptr; ctx->F_bytes = 0;
     ctx->have_factor = false;
+    ctx->cs_valid = false;
     ctx->lm = false;
     ctx->rank = 0;
     ctx->pivots.clear();

@@ -52,6 +52,7 @@ struct EnsProb {
     int64_t *Sprime, *Si, *piv;
     int32_t* acc;
     T *XS, *FS, *LinvT;
+    double* cspart;  // F^T 1 partials
     int64_t n, ldf;
     double trK, tol;
     uint64_t seed;
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         int2 w = items[it];
         const EnsProb<T>& q = P[w.x];
         downdate_body<T, X6>(q.X, q.dp, q.nrm, q.F, q.ldf, q.d, q.n, q.XS, q.nrmS, q.Si, q.FS, q.LinvT, q.st,
-                             q.gs, w.y, lds, rs, &ep);
+                             q.gs, w.y, lds, rs, &ep, q.cspart);
     }
 }
 
@@ -199,6 +200,7 @@ kevd_status ens_rounds(kevd_ctx* m, kevd_ctx** ctxs, int nprob, const kevd_arpc_
         q.piv = B[i].piv;
         q.acc = B[i].acc;
         q.XS = (T*)B[i].XS;
+        q.cspart = B[i].cspart;
         q.FS = (T*)B[i].FS;
         q.LinvT = (T*)B[i].LinvT;
         q.n = c->n;

@@ -1683,8 +1683,19 @@ template <typename T> struct FactorOps {
     Rows<T> src;
     explicit FactorOps(kevd_ctx* c) : ctx(c), src{(const T*)c->F, c->ldf, c->n, c->rank} {}
     kevd_status begin() { return KEVD_OK; }
-    // c = F^T w (w null: ones)
-    kevd_status colsum(const double* w, double* c) { return colsum_rows<T>(ctx, src, w, c); }
+    // c = F^T w (w null: ones; from the downdate's partials when F is the
+    // factor kevd_arpc left)
+    kevd_status colsum(const double* w, double* c)
+    {
+        if (!w && ctx->cs_valid && ctx->cs_part && ctx->cs_nblk > 0) {
+            ProfScope ps(ctx, TC_MATVEC);
+            k_colsum_reduce<<<(unsigned)((src.r + 63) / 64), 1024, 0, ctx->stream>>>(ctx->cs_part, (int)ctx->cs_nblk,
+                                                                                    src.r, ctx->ldf, c);
+            KEVD_HIP(ctx, hipGetLastError());
+            return KEVD_OK;
+        }
+        return colsum_rows<T>(ctx, src, w, c);
+    }
     // y = F c (Sinkhorn mode 1 as k_rowdot); min(y), max dev to host
     kevd_status rowdot(const double* c, double* y, int mode, const double* s, double* s_new, double mm[2])
     {

@@ -100,6 +100,7 @@ extern "C" kevd_status kevd_set_factor(kevd_ctx* ctx, const void* F_host, int64_
     ctx->rank = r;
     ctx->fprec = prec;
     ctx->have_factor = true;
+    ctx->cs_valid = false;  // F^T 1 partials belong to kevd_arpc's factors only
     ctx->lm = false;
     ctx->pivots.clear();
     return KEVD_OK;

@@ -52,6 +52,12 @@ struct kevd_ctx {
     int64_t ldf = 0;
     bool have_factor = false;
     std::vector<int64_t> pivots;  // acceptance order, when F came from kevd_arpc
+    // F^T 1 of the factor kevd_arpc built, as the downdate's per-row-block
+    // partials (cs_nblk x ldf FP64, a named buffer); cs_valid while F is
+    // that factor.
+    double* cs_part = nullptr;
+    int64_t cs_nblk = 0;
+    bool cs_valid = false;
     double factor_eps = 0.0;
     // The last kevd_arpc ran in the LM form: no F, L^-1 kept instead
     // (row-major lower, FP64, leading dimension lm_ldl; a named buffer).
