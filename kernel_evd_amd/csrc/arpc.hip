@@ -468,7 +468,7 @@ kevd_status arpc_prepare(kevd_ctx* ctx, const kevd_arpc_params* p, int max_round
 {
     KEVD_HIP(ctx, hipSetDevice(ctx->device));
     // factor storage (none in the LM form: that is its point)
-    int64_t ldf = KEVD_ROUND_UP(p->rank, 64);
+    int64_t ldf = kevd_factor_ld(p->rank);
     size_t esz = p->prec == KEVD_MIXED ? sizeof(float) : sizeof(double);
     size_t fbytes = (size_t)ctx->n * ldf * esz;
     KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));

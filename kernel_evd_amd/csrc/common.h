@@ -165,6 +165,12 @@ __device__ inline double wave_max(double v)
 
 #define KEVD_ROUND_UP(a, b) ((((a) + (b) - 1) / (b)) * (b))
 
+#ifndef KEVD_LDF_PAD
+#define KEVD_LDF_PAD 32
+#endif
+// leading dimension of a resident rank-r factor (row-major n x ldf)
+inline int64_t kevd_factor_ld(int64_t r) { return KEVD_ROUND_UP(r, 64) + KEVD_LDF_PAD; }
+
 struct kevd_ctx;
 kevd_status kevd_fail(kevd_ctx* ctx, kevd_status s, const std::string& msg);
 kevd_status kevd_hip_fail(kevd_ctx* ctx, hipError_t e, const char* what, const char* file,

@@ -555,6 +555,9 @@ template <typename T, int NG> struct GramCfg;
 #ifndef KEVD_GRAM_X6_PROMOTE
 #define KEVD_GRAM_X6_PROMOTE 4096
 #endif
+#ifndef KEVD_GRAM_X6_PINGPONG
+#define KEVD_GRAM_X6_PINGPONG 1
+#endif
 #ifndef KEVD_GRAM1_CFG
 #define KEVD_GRAM1_CFG 0
 #endif
@@ -901,6 +904,9 @@ void k_gram(const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const T*
     if (since > 0 || first) g.promote(out, first);
 }
 
+// a register set chosen at compile time
+template <int N> struct RegSet { static constexpr int value = N; };
+
 // ---------------------------------------------- weighted Gram, bf16x6
 // G = F^T diag(w) F for FP32 F on the bf16 matrix cores (bf16x6 products,
 // FP32 accumulation, FP64 promotion every PROMOTE rows as in k_gram, here
@@ -965,7 +971,7 @@ void k_gram_x6(const float* __restrict__ F, int64_t ldf, int64_t n, int64_t r, c
     const bool live = ci + (int64_t)(wm + 1) * C::WR - 1 >= cj + (int64_t)wn * C::WC;
 
     floatx16 acc[C::TM][C::TN];
-    auto zero = [&]() {
+    auto zero = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int a = 0; a < C::TM; ++a)
 #pragma unroll
@@ -976,13 +982,18 @@ void k_gram_x6(const float* __restrict__ F, int64_t ldf, int64_t n, int64_t r, c
     zero();
 
     // The whole offset goes in the VGPR operand: the range check that turns
-    // rows at or past k_end into zeros applies to it.
-    float v[C::KC], wv[C::KC];
-    auto gload = [&](int64_t kro) {
+    // rows at or past k_end into zeros applies to it.  Rows of F are loaded
+    // two chunks ahead into two register sets (a load has two chunks'
+    // products to land), the weights one chunk ahead.
+    // (the register set is a constant, so v stays in registers)
+    float v[2][C::KC], wv[C::KC];
+    auto gload = [&](auto set, int64_t kro) __attribute__((always_inline)) {
         const uint32_t base = (uint32_t)kro * rowb + coff;
 #pragma unroll
         for (int k = 0; k < C::KC; ++k)
-            v[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, base + k * rowb, 0, 0));
+            v[decltype(set)::value][k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, base + k * rowb, 0, 0));
+    };
+    auto wload = [&](int64_t kro) __attribute__((always_inline)) {
         if (panel == 0) {
             // the whole vector is bit-cast: clang reads element 0 for a
             // __builtin_bit_cast of a single ext-vector element (q.y)
@@ -997,14 +1008,14 @@ void k_gram_x6(const float* __restrict__ F, int64_t ldf, int64_t n, int64_t r, c
             }
         }
     };
-    auto sstore = [&](unsigned char* st) {
+    auto sstore = [&](auto set, unsigned char* st) __attribute__((always_inline)) {
         unsigned char* dst = st + panel * C::PANEL + scol * 16;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             bf16x8 p0, p1, p2;
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                float x = colok ? v[8 * h + e] : 0.0f;
+                float x = colok ? v[decltype(set)::value][8 * h + e] : 0.0f;
                 if (panel == 0) x *= wv[8 * h + e];
                 __bf16 a0, a1, a2;
                 bf16_split3(x, a0, a1, a2);
@@ -1015,7 +1026,7 @@ void k_gram_x6(const float* __restrict__ F, int64_t ldf, int64_t n, int64_t r, c
             *reinterpret_cast<bf16x8*>(dst + 2 * C::PART + h * C::HALF) = p2;
         }
     };
-    auto compute = [&](const unsigned char* st) {
+    auto compute = [&](const unsigned char* st) __attribute__((always_inline)) {
         if (!live) return;
         bf16x8 af[C::TM][3], bf[C::TN][3];
 #pragma unroll
@@ -1041,7 +1052,7 @@ void k_gram_x6(const float* __restrict__ F, int64_t ldf, int64_t n, int64_t r, c
     };
     // value v of tile (a, b): row wm*WR + 32 a + 8 (v / 4) + 4 hh + v % 4,
     // column wn*WC + 32 b + li; the partial tile is row-major BI x BJ FP64
-    auto promote = [&](bool first) {
+    auto promote = [&](bool first) __attribute__((always_inline)) {
         __amdgpu_buffer_rsrc_t ro =
             __builtin_amdgcn_make_buffer_rsrc(out, (short)0, C::BI * C::BJ * (int)sizeof(double), 0x00020000);
         const int voff = ((wm * C::WR + 4 * hh) * C::BJ + wn * C::WC + li) * (int)sizeof(double);
@@ -1080,28 +1091,62 @@ void k_gram_x6(const float* __restrict__ F, int64_t ldf, int64_t n, int64_t r, c
     const int64_t nk = k_end > k_beg ? (k_end - k_beg + C::KC - 1) / C::KC : 0;
     bool first = true;
     int since = 0;
+    // chunk j's rows sit in v[j & 1]; loads past the last chunk read zeros
+    // (the range check) and are stored to the idle stage, never multiplied
     if (nk > 0) {
-        gload(0);
-        sstore(lds);
-        if (nk > 1) gload(C::KC);
+        wload(0);
+        gload(RegSet<0>(), 0);
+        gload(RegSet<1>(), C::KC);
+        sstore(RegSet<0>(), lds);
+        wload(C::KC);
+        gload(RegSet<0>(), 2 * C::KC);
     }
     __syncthreads();
-    for (int64_t c = 0; c < nk; ++c) {
-        unsigned char* cur = lds + (c & 1) * C::STAGE;
-        unsigned char* nxt = lds + ((c + 1) & 1) * C::STAGE;
-        compute(cur);
-        // the registers hold chunk c + 1: store it, then load c + 2 behind
-        // the next chunk's products
-        if (c + 1 < nk) sstore(nxt);
-        if (c + 2 < nk) gload((c + 2) * C::KC);
-        since += C::KC;
-        if (since >= C::PROMOTE) {
-            promote(first);
-            first = false;
-            since = 0;
+    // A workgroup's waves are placed on the CU's four SIMDs in turn, so
+    // waves w and w + 4 share one.  With one workgroup per CU the barrier
+    // lines up their phases (both multiply, then both split and store, and
+    // the matrix core idles through the split); the second wave of each
+    // pair splits and stores first, so each wave's split runs under the
+    // other's products.
+    const bool store_first = C::WG_PER_CU == 1 && KEVD_GRAM_X6_PINGPONG && wave >= C::NT / 128;
+    // chunk c: multiply it, store chunk c + 1 (in vn), then load the
+    // weights of c + 2 and the rows of c + 3 into vn.  Each order has a loop
+    // of its own: in one loop with both orders the register-reuse waits
+    // would take the other order's loads as outstanding and wait for all.
+    auto run = [&](auto order) __attribute__((always_inline)) {
+        auto iter = [&](int64_t c, auto vn) __attribute__((always_inline)) {
+            unsigned char* cur = lds + (c & 1) * C::STAGE;
+            unsigned char* nxt = lds + ((c + 1) & 1) * C::STAGE;
+            if constexpr (decltype(order)::value) {
+                sstore(vn, nxt);
+                wload((c + 2) * C::KC);
+                gload(vn, (c + 3) * C::KC);
+                compute(cur);
+            } else {
+                compute(cur);
+                sstore(vn, nxt);
+                wload((c + 2) * C::KC);
+                gload(vn, (c + 3) * C::KC);
+            }
+            since += C::KC;
+            if (since >= C::PROMOTE) {
+                promote(first);
+                first = false;
+                since = 0;
+            }
+            __syncthreads();
+        };
+        int64_t c = 0;
+        for (; c + 1 < nk; c += 2) {
+            iter(c, RegSet<1>());
+            iter(c + 1, RegSet<0>());
         }
-        __syncthreads();
-    }
+        if (c < nk) iter(c, RegSet<1>());
+    };
+    if (store_first)
+        run(RegSet<1>());
+    else
+        run(RegSet<0>());
     if (since > 0 || first) promote(first);
 }
 

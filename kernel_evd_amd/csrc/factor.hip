@@ -81,7 +81,7 @@ extern "C" kevd_status kevd_set_factor(kevd_ctx* ctx, const void* F_host, int64_
     KEVD_HIP(ctx, hipSetDevice(ctx->device));
     KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     size_t esz = prec == KEVD_MIXED ? sizeof(float) : sizeof(double);
-    int64_t ldf = KEVD_ROUND_UP(r, 64);
+    int64_t ldf = kevd_factor_ld(r);
     size_t bytes = (size_t)n * ldf * esz;
     if (ctx->F && ctx->F_bytes < bytes) {
         KEVD_HIP(ctx, hipFree(ctx->F));
