@@ -483,7 +483,8 @@ def main():
         _, _, ta_c, te_c = one_step(cc, eps, args.rank, args.block, args.kout, "mixed", args.norm)
         tmc = cc.timings()
     cold = {"arpc_ms": round(1e3 * ta_c, 3), "evd_ms": round(1e3 * te_c, 3), "step_ms": round(1e3 * (ta_c + te_c), 3),
-            "eig_ms": round(tmc["eig_ms"], 3), "eig_iters": int(tmc["eig_iters"])}
+            "eig_ms": round(tmc["eig_ms"], 3), "eig_iters": int(tmc["eig_iters"]),
+            "eig_orth_passes": int(tmc["eig_orth_passes"])}
     log("cold step: %.1f ms (arpc %.1f, evd %.1f, eig %.1f ms in %d iterations)"
         % (cold["step_ms"], cold["arpc_ms"], cold["evd_ms"], cold["eig_ms"], cold["eig_iters"]))
     ms_step = 1e3 * ttotal / args.steps
@@ -527,7 +528,8 @@ def main():
                       "rejection": round(agg["rejection_ms"] / steps, 3),
                       "matvec": round(agg["matvec_ms"] / steps, 3),
                       "dense": round(agg["dense_ms"] / steps, 3), "u_gemm": round(agg["u_ms"] / steps, 3)},
-        "eig_iters": int(tm["eig_iters"]), "eig_householder_fallback": bool(tm["eig_householder"]),
+        "eig_iters": int(tm["eig_iters"]), "eig_orth_passes": int(tm["eig_orth_passes"]),
+        "eig_householder_fallback": bool(tm["eig_householder"]),
         "eig_full_fallback": bool(tm["eig_full_fallback"]),
         "cold_step": cold,
         "rank": int(a_mix.rank), "rounds": int(a_mix.rounds),

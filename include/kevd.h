@@ -289,6 +289,8 @@ typedef struct {
                                   divide-and-conquer solve gave the pairs  */
     int32_t eig_batched;      /* 1 if the pairs came from the ensemble's
                                  batched solve                              */
+    int32_t eig_orth_passes;  /* Cholesky QR passes of the subspace
+                                 iteration (0 for the other solves)         */
 } kevd_timings;
 
 /* Enable per-kernel HIP event timing on the context's stream (adds one

@@ -50,7 +50,8 @@ class Timings(C.Structure):
                 ("gram_launches", i64), ("gram_flop", dbl), ("eig_ms", dbl),
                 ("sample_ms", dbl), ("rejection_ms", dbl), ("matvec_ms", dbl),
                 ("dense_ms", dbl), ("u_ms", dbl), ("eig_iters", i32), ("eig_householder", i32),
-                ("eig_full_fallback", i32), ("eig_batched", i32)]
+                ("eig_full_fallback", i32), ("eig_batched", i32),
+                ("eig_orth_passes", i32)]
 
 
 class Peaks(C.Structure):

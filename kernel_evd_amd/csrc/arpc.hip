@@ -83,9 +83,9 @@ template <typename T, bool X6>
 __global__ __launch_bounds__(256) void k_gather_pivots(const T* __restrict__ Xsrc, const double* __restrict__ nrm,
                                                        int32_t dp, const T* __restrict__ F, int64_t ldf,
                                                        const int64_t* __restrict__ Si, const ArpcState* st,
-                                                       T* XS, double* nrmS, T* FS)
+                                                       T* XS, double* nrmS, T* FS, T* LinvT)
 {
-    gather_pivots_body<T, X6>(Xsrc, nrm, dp, F, ldf, Si, st, XS, nrmS, FS, blockIdx.x);
+    gather_pivots_body<T, X6>(Xsrc, nrm, dp, F, ldf, Si, st, XS, nrmS, FS, LinvT, blockIdx.x);
 }
 
 template <typename T, bool X6>
@@ -240,10 +240,10 @@ kevd_status run_rounds(kevd_ctx* ctx, const kevd_arpc_params* p, int max_rounds,
                                            Linv64);
             if (x6)
                 k_gather_pivots<T, true><<<BMAX, 256, 0, s>>>(Xsrc, nrm, ctx->dp, F, ctx->ldf, Si, st, XS,
-                                                              nrmS, FS);
+                                                              nrmS, FS, LinvT);
             else
                 k_gather_pivots<T, false><<<BMAX, 256, 0, s>>>(Xsrc, nrm, ctx->dp, F, ctx->ldf, Si, st, XS,
-                                                               nrmS, FS);
+                                                               nrmS, FS, LinvT);
         }
         {
             ProfScope ps(ctx, TC_DOWNDATE);
@@ -523,7 +523,9 @@ kevd_status arpc_prepare(kevd_ctx* ctx, const kevd_arpc_params* p, int max_round
                      p->regenerate ? (size_t)BMAX * 64 * esz
                                    : (p->prec == KEVD_MIXED ? x6_fs_bytes(ldf) : (size_t)BMAX * ldf * esz),
                      &B->FS));
-    KEVD_TRY(ctx_buf(ctx, "a_LinvT", (size_t)BMAX * BMAX * esz, &B->LinvT));
+    KEVD_TRY(ctx_buf(ctx, "a_LinvT",
+                     !p->regenerate && p->prec == KEVD_MIXED ? x6_linv_bytes() : (size_t)BMAX * BMAX * esz,
+                     &B->LinvT));
     // F form: the downdate leaves F^T 1 as per-row-block partials
     B->cspart = nullptr;
     if (!p->regenerate) {

@@ -511,14 +511,24 @@ __device__ inline const __bf16* x6_xsx(const float* XS, int32_t dp)
     return reinterpret_cast<const __bf16*>(XS + (int64_t)BMAX * dp);
 }
 
+// And the parts of L_c^-1 follow LinvT (BMAX x BMAX FP32, row-major, lower
+// triangular), chunk-major in the same fragment order: the downdate's
+// product with L_c^-T runs as bf16x6.
+__host__ __device__ inline size_t x6_linv_bytes() { return (size_t)BMAX * BMAX * (4 + 3 * 2); }
+__device__ inline __bf16* x6_lx(float* LinvT) { return reinterpret_cast<__bf16*>(LinvT + BMAX * BMAX); }
+__device__ inline const __bf16* x6_lx(const float* LinvT)
+{
+    return reinterpret_cast<const __bf16*>(LinvT + BMAX * BMAX);
+}
+
 // Pivot rows in the GEMM precision: X(S_i,:), |x|^2, F(S_i, 0:Nr).  Rows
 // past k are zero.  X6: also the bf16x6 parts of X(S_i,:) and of F(S_i,:),
-// the latter zero from Nr up to a multiple of 32.
+// the latter zero from Nr up to a multiple of 32, and of row c of L_c^-1.
 template <typename T, bool X6 = false>
 __device__ inline void gather_pivots_body(const T* __restrict__ Xsrc, const double* __restrict__ nrm,
                                           int32_t dp, const T* __restrict__ F, int64_t ldf,
                                           const int64_t* __restrict__ Si, const ArpcState* st, T* XS,
-                                          double* nrmS, T* FS, int c)
+                                          double* nrmS, T* FS, T* LinvT, int c)
 {
     if (st->done) return;
     int k = st->k;
@@ -544,6 +554,16 @@ __device__ inline void gather_pivots_body(const T* __restrict__ Xsrc, const doub
             __bf16 h, md, l;
             bf16_split3(v, h, md, l);
             __bf16* e = fx + (m / 32) * X6_CHUNK + x6_idx(c, (int)(m % 32));
+            e[0] = h;
+            e[X6_CHUNK / 3] = md;
+            e[2 * X6_CHUNK / 3] = l;
+        }
+        // row c of L_c^-1 (zero past the diagonal and for c >= k)
+        __bf16* lx = x6_lx(LinvT);
+        for (int m = threadIdx.x; m < BMAX; m += 256) {
+            __bf16 h, md, l;
+            bf16_split3(LinvT[c * BMAX + m], h, md, l);
+            __bf16* e = lx + (m / 32) * X6_CHUNK + x6_idx(c, m % 32);
             e[0] = h;
             e[X6_CHUNK / 3] = md;
             e[2 * X6_CHUNK / 3] = l;
@@ -654,7 +674,8 @@ __device__ inline void dd_prologue(typename Cfg<T>::G& g, const T* __restrict__ 
 template <typename T>
 __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64_t ldf, double* __restrict__ d,
                                  int64_t n, const T* __restrict__ LinvT, int k, int64_t Nr, int64_t r0, T* lds,
-                                 double (*rs)[Cfg<T>::BM], const DowndateEp<T>* ep, double* __restrict__ csrow = nullptr)
+                                 double (*rs)[Cfg<T>::BM], const DowndateEp<T>* ep, double* __restrict__ csrow = nullptr,
+                                 const __bf16* Lx = nullptr)
 {
     typedef typename Cfg<T>::G G;
     typedef DowndateLds<T> DL;
@@ -669,7 +690,16 @@ __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64
                 lds[g.row_of(a, v) * TP + g.col_of(b)] = -g.acc[a][b][v];
     __syncthreads();
     g.zero();
-    g.lds_global_tri_live(lds, TP, LinvT, BMAX, (k + G::KSUB - 1) / G::KSUB * G::KSUB);  // L_c^-T is triangular
+    // L_c^-T is triangular; Lx (FP32 only): its bf16x6 parts, the product
+    // then runs on the bf16 matrix cores
+    if constexpr (std::is_same<T, float>::value) {
+        if (Lx)
+            x6_tri_live<G>(g, lds, TP, Lx);
+        else
+            g.lds_global_tri_live(lds, TP, LinvT, BMAX, (k + G::KSUB - 1) / G::KSUB * G::KSUB);
+    } else {
+        g.lds_global_tri_live(lds, TP, LinvT, BMAX, (k + G::KSUB - 1) / G::KSUB * G::KSUB);
+    }
     // write the new columns and the squared row norms
     double part[G::NTM][G::M::ACC];
     int ro[G::NTM][G::M::ACC];  // element offset of the lane's rows from F(r0, Nr)
@@ -767,7 +797,7 @@ __device__ inline void dd_finish(typename Cfg<T>::G& g, T* __restrict__ F, int64
 //   acc  = -K(R, S_i)                       (epilogue: norms, exp)
 //   acc += F_R F_{S_i}^T                    (MFMA, K = Nr)
 //   T    = -acc -> LDS
-//   acc  = T L_c^-T                         (MFMA, K = k)
+//   acc  = T L_c^-T                         (MFMA, K = k; bf16x6 with X6)
 //   F(R, Nr:Nr+k) = acc;  d(R) = max(d(R) - rowsq(acc), 0)
 // This is row block blk (rows blk * BM ...) of one problem; it returns
 // early (before any barrier) when the problem is done or accepted nothing
@@ -805,7 +835,9 @@ __device__ inline void downdate_body(const T* __restrict__ X, int32_t dp, const 
             g.mainloop_live(lds, F + r0 * ldf, ldf, n - r0, FS, ldf, BN, Nr);
         }
     }
-    dd_finish<T>(g, F, ldf, d, n, LinvT, k, Nr, r0, lds, rs, ep, cspart ? cspart + blk * ldf : nullptr);
+    const __bf16* Lx = nullptr;
+    if constexpr (X6 && std::is_same<T, float>::value) Lx = x6_lx(LinvT);
+    dd_finish<T>(g, F, ldf, d, n, LinvT, k, Nr, r0, lds, rs, ep, cspart ? cspart + blk * ldf : nullptr, Lx);
 }
 
 // ------------------------------------------- regenerating (LM) variant

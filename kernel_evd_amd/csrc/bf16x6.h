@@ -226,6 +226,80 @@ __device__ inline void x6_nt_mainloop_n(G& g, unsigned char* lds, const float* A
     }
 }
 
+// acc += Ts[0:BM, 0:K] L^T for a lower-triangular L (BN = 128 rows,
+// L(n, m) = 0 for m > n; K <= 16 NACT), as bf16x6 on
+// v_mfma_f32_16x16x32_bf16 into the accumulators of the FP32 NTGemm G.  Ts is
+// FP32 in LDS (row pitch tp), each wave reads only its own rows, and its A
+// fragments are split in registers per 32-value chunk.  L comes already
+// split, chunk-major in fragment order like the pivot rows (Lx, X6_CHUNK
+// elements per chunk of 32 columns), and is read from global memory (a small
+// operand every block shares, served from L2), one column-tile pair ahead of
+// its products.  Column tile y needs only the chunks c <= y / 2: about half
+// the products of the full square.
+template <typename G, int NACT>
+__device__ inline void x6_tri_n(G& g, const float* Ts, int tp, const __bf16* Lx)
+{
+    typedef __attribute__((ext_vector_type(4))) float f4;
+    constexpr int TM = G::NTM, NC = (NACT + 1) / 2;
+    static_assert(G::M::MT == 16 && G::NBN == 128, "16x16 tiles, 128 columns");
+    const int lane = threadIdx.x & 63;
+    const __bf16* lb = Lx + lane * 8;
+    // the three parts of column tiles y0, y0 + 1 of chunk c
+    auto loadB = [&](bf16x8 (&b)[2][3], int c, int y0) {
+#pragma unroll
+        for (int yo = 0; yo < 2; ++yo)
+            if (y0 + yo < NACT)
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    b[yo][q] = *reinterpret_cast<const bf16x8*>(lb + c * X6_CHUNK + q * (X6_CHUNK / 3) +
+                                                                (y0 + yo) * 512);
+    };
+    bf16x8 bfr[2][2][3];
+    int par = 0;
+    loadB(bfr[0], 0, 0);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        bf16x8 af[TM][3];
+#pragma unroll
+        for (int x = 0; x < TM; ++x) {
+            const float* src = Ts + (g.wm * TM * 16 + x * 16 + g.li) * tp + 32 * c + 8 * g.h;
+            const f4 v[2] = {*reinterpret_cast<const f4*>(src), *reinterpret_cast<const f4*>(src + 4)};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                __bf16 a0, a1, a2;
+                bf16_split3(v[e / 4][e % 4], a0, a1, a2);
+                af[x][0][e] = a0, af[x][1][e] = a1, af[x][2][e] = a2;
+            }
+        }
+#pragma unroll
+        for (int y0 = 2 * c; y0 < NACT; y0 += 2) {
+            // the next pair: the following tiles of this chunk, else the
+            // first pair of the next chunk
+            const bool last = y0 + 2 >= NACT;
+            const int nc = last ? c + 1 : c, ny = last ? 2 * (c + 1) : y0 + 2;
+            if (nc < NC) loadB(bfr[par ^ 1], nc, ny);
+#pragma unroll
+            for (int p = 0; p < X6_NPROD; ++p)
+#pragma unroll
+                for (int x = 0; x < TM; ++x)
+#pragma unroll
+                    for (int yo = 0; yo < 2; ++yo)
+                        if (y0 + yo < NACT)
+                            mfma_bf16_16(g.acc[x][y0 + yo], af[x][x6_a(p)], bfr[par][yo][x6_b(p)]);
+            par ^= 1;
+        }
+    }
+}
+
+template <typename G, int N = G::NTN>
+__device__ inline void x6_tri_live(G& g, const float* Ts, int tp, const __bf16* Lx)
+{
+    if (g.nact == N)
+        x6_tri_n<G, N>(g, Ts, tp, Lx);
+    else if constexpr (N > 1)
+        x6_tri_live<G, N - 1>(g, Ts, tp, Lx);
+}
+
 template <typename G, int N = G::NTN>
 __device__ inline void x6_nt_mainloop_live(G& g, unsigned char* lds, const float* A, int64_t lda, int64_t arows,
                                            const __bf16* Bx, int64_t K)

@@ -109,7 +109,7 @@ template <typename T, bool X6>
 __global__ __launch_bounds__(256) void k_ens_gather(const EnsProb<T>* P)
 {
     const EnsProb<T>& q = P[blockIdx.y];
-    gather_pivots_body<T, X6>(q.X, q.nrm, q.dp, q.F, q.ldf, q.Si, q.st, q.XS, q.nrmS, q.FS, blockIdx.x);
+    gather_pivots_body<T, X6>(q.X, q.nrm, q.dp, q.F, q.ldf, q.Si, q.st, q.XS, q.nrmS, q.FS, q.LinvT, blockIdx.x);
 }
 
 __global__ void k_ens_zero(int* ctr)

@@ -2536,6 +2536,7 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
                 bool before_check = it + 1 >= next_check || north + 1 >= max_orth;
                 KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++,
                                     before_check ? 2 : 1));
+                ctx->tm.eig_orth_passes += before_check ? 2 : 1;
                 std::swap(Q, Y);
                 continue;
             }
@@ -2592,6 +2593,7 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
             north = 0;
             KEVD_TRY(orth_basis(ctx, Y, r, l, householder, G, tau, info + 1 + 3 * north++,
                                 next_check == it + 1 ? 2 : 1));
+            ctx->tm.eig_orth_passes += next_check == it + 1 ? 2 : 1;
             std::swap(Q, Y);
         }
         *iters = it;
@@ -3459,6 +3461,7 @@ extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_ev
     ctx->tm.eig_iters = 0;
     ctx->tm.eig_householder = 0;
     ctx->tm.eig_full_fallback = 0;
+    ctx->tm.eig_orth_passes = 0;
     ctx->tm.eig_batched = 0;
     prof_reset(ctx);
     size_t ubytes = (size_t)ctx->n * p->k_out * sizeof(float);
