@@ -2228,6 +2228,178 @@ __global__ __launch_bounds__(1024) void k_potrf_upper(double* A, int l, int* inf
     if (tid == 0) *info = bad;
 }
 
+// The same factorisation for l <= CHOLR_LMAX with the whole upper triangle
+// in registers: right-looking and unblocked, one barrier per column.  The
+// 32 x 32 blocks (bi <= bj) of the upper triangle are dealt to the eight
+// waves round-robin in row-major order (block b to wave b % 8); lane
+// (pr, pc) of a wave holds the 4 x 4 patch rows 4 pr .. 4 pr + 3, columns
+// 4 pc .. 4 pc + 3 of each of its blocks.  Step k reads row k of the partly
+// updated matrix from LDS, published during step k - 1 as u_j = A(k, j) for
+// j > k and 0 for j <= k (from the first column of k's block on, which
+// covers every row and column a live block reads), with A(k, k) and its
+// reciprocal (formed by the publishing lane) in their own slots.  Row k goes
+// straight to A (A(k, k) and u_j, scaled into R(k, k) = sqrt(A(k, k)) and
+// R(k, j) = u_j / R(k, k) after the last step), and every entry takes
+// A(i, j) -= u_i (u_j / A(k, k)) (8 LDS reads, 4 multiplies and 16 FMAs per
+// patch; rows and columns <= k see u = 0).  Then the owners of row k + 1
+// publish it into the other LDS buffer.  A block is final once k passes its
+// last row, so a wave's blocks (in order of their last row) retire in
+// phases: phase p runs the steps up to block p's last row on blocks p ..,
+// and every step is straight-line code (a branch around a block would make
+// the compiler keep a second copy of its registers).  Entries outside l
+// read as 0; the strict lower triangle of A is left as it was.  *info as
+// k_potrf_upper (rows from the failing one on keep A).
+constexpr int CHOLR_LMAX = 256, CHOLR_NT = 512, CHOLR_MAXB = 5;
+__global__ __launch_bounds__(CHOLR_NT) void k_potrf_reg(double* A, int l, int* info)
+{
+    typedef __attribute__((ext_vector_type(2))) double d2;
+    A += (int64_t)blockIdx.x * l * l;
+    info += blockIdx.x;
+    __shared__ __attribute__((aligned(16))) double buf[2][CHOLR_LMAX + 2];  // + a spare slot
+    __shared__ double dbuf[2][2];
+    const int tid = threadIdx.x, lane = tid & 63;
+    // the wave index as a scalar: the block bookkeeping is wave-uniform
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int pr = lane >> 3, pc = lane & 7;
+    const int nb = (l + 31) / 32, nblk = nb * (nb + 1) / 2;
+    const int nq = nblk > w ? (nblk - w + 7) / 8 : 0;  // blocks of this wave
+    // slot s holds the wave's block s - (MAXB - nq) (none below 0, so the
+    // slots run in order of retirement: empty ones first)
+    int rb[CHOLR_MAXB], cb[CHOLR_MAXB], kend[CHOLR_MAXB];
+    bool have[CHOLR_MAXB];
+    double e[CHOLR_MAXB][4][4];
+#pragma unroll
+    for (int sl = 0; sl < CHOLR_MAXB; ++sl) {
+        const int q = sl - (CHOLR_MAXB - nq);
+        int b = w + 8 * q, bi = 0;
+        have[sl] = q >= 0;
+        if (!have[sl]) b = 0;
+        while (bi < nb && b >= nb - bi) b -= nb - bi++;  // row-major over bi <= bj
+        rb[sl] = have[sl] ? 32 * bi : 0;  // an empty slot is never stepped
+        cb[sl] = have[sl] ? 32 * (bi + b) : 0;
+        kend[sl] = have[sl] ? min(l, 32 * bi + 32) : 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int i = rb[sl] + 4 * pr + a, j = cb[sl] + 4 * pc + c;
+                e[sl][a][c] = (have[sl] && i < l && j < l) ? A[i + (int64_t)j * l] : 0.0;
+            }
+    }
+    for (int j = tid; j < 2 * (CHOLR_LMAX + 2); j += CHOLR_NT) (&buf[0][0])[j] = 0.0;
+    __syncthreads();
+    // row 0 into buffer 0 (its block row is the first one)
+#pragma unroll
+    for (int sl = 0; sl < CHOLR_MAXB; ++sl)
+        if (have[sl] && rb[sl] == 0 && pr == 0)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int j = cb[sl] + 4 * pc + c;
+                if (j == 0) {
+                    dbuf[0][0] = e[sl][0][c];
+                    dbuf[0][1] = 1.0 / e[sl][0][c];
+                }
+                if (j < l) buf[0][j] = j > 0 ? e[sl][0][c] : 0.0;
+            }
+    __syncthreads();
+    int bad = 0, k = 0;
+    // phase p < MAXB: the steps up to block p's last row on slots p ..;
+    // phase MAXB: the rest of the steps (every wave takes every barrier)
+#pragma unroll
+    for (int p = 0; p <= CHOLR_MAXB; ++p) {
+        const int k_end = p < CHOLR_MAXB ? kend[p] : l;
+        for (; k < k_end; ++k) {
+            const double* u = buf[k & 1];
+            const double dk = dbuf[k & 1][0], invd = dbuf[k & 1][1];
+            if (!(dk > 0.0)) {  // every thread reads the same pivot: a uniform exit
+                bad = k + 1;
+                goto done;
+            }
+            // row k of R before its scaling by 1 / R(k, k) (after the steps)
+            if (tid < l - k) A[k + (int64_t)(k + tid) * l] = tid == 0 ? dk : u[k + tid];
+            double* nxt = buf[(k + 1) & 1];
+#pragma unroll
+            for (int sl = p; sl < CHOLR_MAXB; ++sl) {
+                const int r0 = rb[sl] + 4 * pr, c0 = cb[sl] + 4 * pc;
+                const d2 rv0 = *reinterpret_cast<const d2*>(u + r0), rv1 = *reinterpret_cast<const d2*>(u + r0 + 2);
+                const d2 cv0 = *reinterpret_cast<const d2*>(u + c0), cv1 = *reinterpret_cast<const d2*>(u + c0 + 2);
+                const double ur[4] = {rv0.x, rv0.y, rv1.x, rv1.y};
+                const double uc[4] = {cv0.x * invd, cv0.y * invd, cv1.x * invd, cv1.y * invd};
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) e[sl][a][c] -= ur[a] * uc[c];
+                // row k + 1 (patch row dr): published, 0 up to its diagonal
+                // (a store past l goes to the spare slot CHOLR_LMAX)
+                // (a store past l goes to the spare slot CHOLR_LMAX).  The
+                // patch row holding it, (k + 1) % 4, is the same for the whole
+                // wave: a scalar switch picks it.
+                if (have[sl] && rb[sl] <= k + 1 && k + 1 < rb[sl] + 32) {  // uniform
+                    if (pr == ((k + 1 - rb[sl]) >> 2)) {
+                        double v[4];
+                        switch ((k + 1) & 3) {
+                        case 0:
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) v[c] = e[sl][0][c];
+                            break;
+                        case 1:
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) v[c] = e[sl][1][c];
+                            break;
+                        case 2:
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) v[c] = e[sl][2][c];
+                            break;
+                        default:
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) v[c] = e[sl][3][c];
+                        }
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            const int j = c0 + c;
+                            nxt[j < l ? j : CHOLR_LMAX] = j > k + 1 ? v[c] : 0.0;
+                            if (j == k + 1) {  // the pivot and its reciprocal
+                                dbuf[(k + 1) & 1][0] = v[c];
+                                dbuf[(k + 1) & 1][1] = 1.0 / v[c];
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+done:
+    // R(i, i) = sqrt(A(i, i)) and R(i, j) = A(i, j) / R(i, i) on the final
+    // rows (all, or those above a failing pivot)
+    // rows (all, or those above a failing pivot): thread (i, j0) scales row
+    // i in the columns j >= i of parity j0, the column-major loads of a wave
+    // contiguous
+    const int kf = bad ? bad - 1 : l;
+    double* rs = &buf[0][0];
+    __syncthreads();
+    for (int i = tid; i < kf; i += CHOLR_NT) rs[i] = sqrt(A[i + (int64_t)i * l]);
+    __syncthreads();
+    const int i = tid & (CHOLR_LMAX - 1), j0 = tid >> 8;
+    if (i < kf) {
+        const double d = rs[i], ri = 1.0 / d;
+#pragma unroll 8
+        for (int j = i + ((i ^ j0) & 1); j < l; j += 2) A[i + (int64_t)j * l] = j == i ? d : A[i + (int64_t)j * l] * ri;
+    }
+    if (tid == 0) *info = bad;
+}
+
+// KEVD_CHOL_PANEL=1 in the environment keeps the panel Cholesky
+// (k_potrf_upper) for every block width, for comparison.
+bool chol_panel_only()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("KEVD_CHOL_PANEL");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
 // Householder tridiagonalisation of the symmetric l x l FP64 matrix M
 // (column-major in A, lda = l, l <= SYTRD_LMAX; both triangles read) in one
 // workgroup, with LAPACK dsytrd's lower-storage output (Golub and Van Loan,
@@ -2441,7 +2613,9 @@ kevd_status orth_basis(kevd_ctx* ctx, double* Y, int64_t r, int l, bool househol
     for (int pass = 0; pass < passes; ++pass) {
         KEVD_TRY(gram_tn(ctx, Y, Y, r, l, G));
         if (pass == 0) k_chol_shift<<<1, 256, 0, ctx->stream>>>(G, l, c);
-        if (l <= CHOL_LMAX)
+        if (l <= CHOLR_LMAX && !chol_panel_only())
+            k_potrf_reg<<<1, CHOLR_NT, 0, ctx->stream>>>(G, l, info + pass);
+        else if (l <= CHOL_LMAX)
             k_potrf_upper<<<1, 1024, 0, ctx->stream>>>(G, l, info + pass);
         else
             BLAS(rocsolver_dpotrf(h, rocblas_fill_upper, Lb, G, Lb, info + pass));
@@ -2788,7 +2962,10 @@ kevd_status eig_batch_run(EigBatch& B)
             // one workgroup per problem (rocSOLVER's batched potrf and rocBLAS's
             // batched trsm run as hundreds of per-matrix kernels), then
             // Y <- Y R^-1 as a batched GEMM with the inverted factor
-            k_potrf_upper<<<P, 1024, 0, s>>>(G, l, info + (int64_t)(slot0 + pass) * P);
+            if (l <= CHOLR_LMAX && !chol_panel_only())
+                k_potrf_reg<<<P, CHOLR_NT, 0, s>>>(G, l, info + (int64_t)(slot0 + pass) * P);
+            else
+                k_potrf_upper<<<P, 1024, 0, s>>>(G, l, info + (int64_t)(slot0 + pass) * P);
             BLAS(rocsolver_dtrtri_strided_batched(h, rocblas_fill_upper, rocblas_diagonal_non_unit, Lb, G, Lb, sLL,
                                                   tinfo, Pb));
             // the GEMM reads the whole matrix: the strict lower triangle still

@@ -167,6 +167,47 @@ def test_leading_solver_matches_full_eigensolve(gpu_ctx):
     assert np.max(np.abs(Ua[:, -16:] - U[:, -16:])) < 1e-5
 
 
+def _known_spectrum_factor(n, r, lam, seed):
+    """F = [1, Q diag(sqrt(n lam))] with Q orthonormal and orthogonal to 1:
+    K^ 1 = n 1, so the Sinkhorn scaling is 1/sqrt(n) and the bistochastic
+    matrix P = 11^T/n + Q diag(lam) Q^T has eigenvalues 1 and lam."""
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((n, r - 1))
+    G -= G.mean(axis=0)
+    Q, _ = np.linalg.qr(G)
+    F = np.empty((n, r))
+    F[:, 0] = 1.0
+    F[:, 1:] = Q * np.sqrt(lam * n)
+    return F
+
+
+@pytest.mark.parametrize("k", [32, 100, 200])
+def test_leading_solve_known_spectrum(k):
+    """Block subspace iteration on a spectrum lam_j = 0.5 * 0.97^j that it
+    resolves in a few steps.  The blocks have l = 128, 256 and 288 columns:
+    the first two orthonormalise with the register-resident Cholesky
+    (l <= 256), the last with the panel one.  Each must converge in its own
+    iteration (no Householder restart, no full-solve fallback) to the exact
+    eigenvalues."""
+    from kernel_evd_amd import Context
+    n, r = 4096, 600
+    lam = 0.5 * 0.97 ** np.arange(r - 1)
+    F = _known_spectrum_factor(n, r, lam, seed=23)
+    exact = np.concatenate([[1.0], lam])[:k]
+    with Context(0) as c:
+        c.set_points(gauss_points(n, 3, seed=24))
+        c.set_factor(F)
+        e = c.evd(r, k_out=k, norm="sinkhorn", prec="fp64", want_U=False)
+        tm = c.timings()
+    top = e.lam[::-1][:k]
+    print("k", k, "iters", tm["eig_iters"], "passes", tm["eig_orth_passes"], "householder",
+          tm["eig_householder"], "fallback", tm["eig_full_fallback"], "max rel err",
+          np.max(np.abs(top - exact) / exact), flush=True)
+    assert tm["eig_iters"] > 0 and tm["eig_orth_passes"] > 0
+    assert tm["eig_householder"] == 0 and tm["eig_full_fallback"] == 0
+    np.testing.assert_allclose(top, exact, rtol=1e-10)
+
+
 def test_leading_solve_falls_back_on_slowly_decaying_spectrum(gpu_ctx):
     """A factor whose bistochastic spectrum is known: F = [1, Q diag(sqrt(n
     lam))] with Q orthonormal and orthogonal to 1 gives K^ 1 = n 1, so the
@@ -177,14 +218,8 @@ def test_leading_solve_falls_back_on_slowly_decaying_spectrum(gpu_ctx):
     supply the leading pairs."""
     from kernel_evd_amd import Context
     n, r, k = 4096, 600, 100
-    rng = np.random.default_rng(21)
-    G = rng.standard_normal((n, r - 1))
-    G -= G.mean(axis=0)
-    Q, _ = np.linalg.qr(G)
     lam = 0.5 * 0.9997 ** np.arange(r - 1)
-    F = np.empty((n, r))
-    F[:, 0] = 1.0
-    F[:, 1:] = Q * np.sqrt(lam * n)
+    F = _known_spectrum_factor(n, r, lam, seed=21)
     exact = np.concatenate([[1.0], lam])[:k]
     with Context(0) as c:
         c.set_points(gauss_points(n, 3, seed=22))
