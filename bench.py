@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # profiles/traffic_k_downdate_x6_k_gram_x6_k_sinkhorn_ks_n1M_r2000.md (bf16x6
 # kernels) and profiles/traffic_k_downdate_k_gram_k_sinkhorn_ks_n1M_r2000.md
 # (the FP32-MFMA kernels, KEVD_FP32_MFMA=1).
-TRAFFIC_BYTES = {"k_downdate<float, true>": 4.351e9, "k_gram_x6": 26.71e9,
+TRAFFIC_BYTES = {"k_downdate<float, true>": 4.360e9, "k_gram_x6": 26.63e9,
                  "k_downdate<float, false>": 4.276e9, "k_gram<float,1>": 110.89e9,
                  "k_sinkhorn_rows<float,8>": 8.507e9}
 FP32_MFMA_PEAK_TF = 157.3
