@@ -189,6 +189,9 @@ def ensemble_leg(nprob=48, kout=32, seed=1, workers=8, candidates=(32, 64, 128),
     t1 = time.perf_counter()
     evs = ensemble_evd(ctxs, [o.rank for o in outs], k_out=kout, norm=norm, prec="mixed", workers=workers)
     t2 = time.perf_counter()
+    tms = [c.timings() for c in ctxs]
+    sk_batched = int(sum(t["sinkhorn_batched"] for t in tms))
+    eig_batched = int(sum(t["eig_batched"] for t in tms))
     # the same problems one after another through the single-problem calls
     t3 = time.perf_counter()
     for i, c in enumerate(ctxs):
@@ -218,6 +221,7 @@ def ensemble_leg(nprob=48, kout=32, seed=1, workers=8, candidates=(32, 64, 128),
            "columns_per_s_incl_evd": round(cols / (t2 - t0), 1),
            "sequential_total_s": round(t4 - t3, 4), "batched_speedup": round((t4 - t3) / (t2 - t0), 3),
            "graph_replay": bool(graph), "evd_workers": workers, "k_out": kout, "norm": norm,
+           "sinkhorn_batched": sk_batched, "eigensolve_batched": eig_batched,
            "blocks_chosen": {str(b): int(blocks.count(b)) for b in sorted(set(blocks))},
            "autotune_s": round(t_tune, 2),
            "gram_tiles_chosen": {str(v): int(tiles.count(v)) for v in sorted(set(tiles))},

@@ -291,6 +291,8 @@ typedef struct {
                                  batched solve                              */
     int32_t eig_orth_passes;  /* Cholesky QR passes of the subspace
                                  iteration (0 for the other solves)         */
+    int32_t sinkhorn_batched; /* 1 if the Sinkhorn iteration ran in the
+                                 ensemble's batched launches               */
 } kevd_timings;
 
 /* Enable per-kernel HIP event timing on the context's stream (adds one

@@ -51,7 +51,7 @@ class Timings(C.Structure):
                 ("sample_ms", dbl), ("rejection_ms", dbl), ("matvec_ms", dbl),
                 ("dense_ms", dbl), ("u_ms", dbl), ("eig_iters", i32), ("eig_householder", i32),
                 ("eig_full_fallback", i32), ("eig_batched", i32),
-                ("eig_orth_passes", i32)]
+                ("eig_orth_passes", i32), ("sinkhorn_batched", i32)]
 
 
 class Peaks(C.Structure):

@@ -404,6 +404,25 @@ extern "C" kevd_status kevd_ensemble_evd(kevd_ctx** ctxs, int32_t nprob, const k
             ctxs[i]->eig_batch_slot = i;
         }
     }
+    // The Sinkhorn iterations first, batched over the problems on this
+    // thread (KEVD_ENS_SK_BATCH=0 keeps each problem's own iteration); each
+    // problem's kevd_evd then starts from its scaling.
+    const char* want_sk = std::getenv("KEVD_ENS_SK_BATCH");
+    int sk_batched = 0;
+    if (!(want_sk && want_sk[0] == '0')) {
+        kevd_status bs = ens_sinkhorn(ctxs, nprob, p, &sk_batched);
+        if (bs != KEVD_OK) {
+            for (int i = 0; i < nprob; ++i) ctxs[i]->sk_pre.valid = false;
+            if (eb) {
+                for (int i = 0; i < nprob; ++i) {
+                    ctxs[i]->eig_batch = nullptr;
+                    ctxs[i]->eig_batch_slot = -1;
+                }
+                eig_batch_destroy(eb);
+            }
+            return bs;
+        }
+    }
     std::vector<kevd_status> st((size_t)nprob, KEVD_OK);
     std::atomic<int> next{0};
     auto work = [&]() {
@@ -418,6 +437,9 @@ extern "C" kevd_status kevd_ensemble_evd(kevd_ctx** ctxs, int32_t nprob, const k
     for (int w = 1; w < nw; ++w) pool.emplace_back(work);
     work();
     for (auto& t : pool) t.join();
+    // a scaling its problem's kevd_evd did not take (it failed before) is
+    // not left for a later call
+    for (int i = 0; i < nprob; ++i) ctxs[i]->sk_pre.valid = false;
     if (eb) {
         for (int i = 0; i < nprob; ++i) {
             ctxs[i]->eig_batch = nullptr;

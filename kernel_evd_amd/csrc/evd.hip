@@ -74,12 +74,11 @@ __global__ __launch_bounds__(256) void k_colsum(const T* __restrict__ F, int64_t
 // c[j] = sum_b part[b][j], deterministic: a block owns 64 columns; its 16
 // row groups sum the blocks b = g (mod 16) in order, then group 0 adds the
 // 16 group sums in order.
-__global__ __launch_bounds__(1024) void k_colsum_reduce(const double* part, int nb, int64_t r,
-                                                        int64_t rpad, double* c)
+__device__ inline void colsum_reduce_body(const double* part, int nb, int64_t r, int64_t rpad, double* c, int bx)
 {
     __shared__ double red[16][64];
     int col = threadIdx.x & 63, g = threadIdx.x >> 6;
-    int64_t j = (int64_t)blockIdx.x * 64 + col;
+    int64_t j = (int64_t)bx * 64 + col;
     double s = 0.0;
     if (j < r) {
         int b = g;
@@ -101,6 +100,11 @@ __global__ __launch_bounds__(1024) void k_colsum_reduce(const double* part, int 
         for (int q = 0; q < 16; ++q) t += red[q][col];
         c[j] = t;
     }
+}
+__global__ __launch_bounds__(1024) void k_colsum_reduce(const double* part, int nb, int64_t r,
+                                                        int64_t rpad, double* c)
+{
+    colsum_reduce_body(part, nb, r, rpad, c, blockIdx.x);
 }
 
 constexpr int ROWDOT_LDS_MAX = 8192;  // c staged in LDS up to this many columns
@@ -319,17 +323,17 @@ __global__ __launch_bounds__(256) void k_sinkhorn_pass(const T* __restrict__ F, 
 // needs ~310 VGPRs, one wave per SIMD; capping it at two waves spills (and
 // moving u to LDS does not lower the count), so it runs at one.  FP32 only:
 // the FP64 factor uses k_sinkhorn_pass.
+// (body: row block blk; the batched ensemble pass runs it too)
 template <typename T, int VPQ>
-__global__ __launch_bounds__(256) void k_sinkhorn_rows(const T* __restrict__ F, int64_t ldf, int64_t n,
-                                                       int64_t r, const double* __restrict__ u,
-                                                       const double* __restrict__ s, int damped, double* q_out,
-                                                       double* part, int64_t rpad, double* bstat)
+__device__ __attribute__((always_inline)) inline void sinkhorn_rows_body(
+    const T* __restrict__ F, int64_t ldf, int64_t n, int64_t r, const double* __restrict__ u,
+    const double* __restrict__ s, int damped, double* q_out, double* part, int64_t rpad, double* bstat, int blk)
 {
     constexpr int V = 16 / (int)sizeof(T);
     typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type vec_t;
     __shared__ double red[4][SK_NSTAT];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t r0 = (int64_t)blockIdx.x * SK_ROWS;
+    const int64_t r0 = (int64_t)blk * SK_ROWS;
     const int64_t r1 = r0 + SK_ROWS < n ? r0 + SK_ROWS : n;
     double uv[VPQ][V], acc[VPQ][V];
 #pragma unroll
@@ -414,20 +418,28 @@ __global__ __launch_bounds__(256) void k_sinkhorn_rows(const T* __restrict__ F, 
     for (int idx = threadIdx.x; idx < 64 * VPQ * V; idx += 256) {
         int q = idx / (64 * V), rem = idx % (64 * V), ln = rem / V, k = rem % V;
         int64_t c = ((int64_t)ln + 64 * q) * V + k;
-        if (c < r) part[(int64_t)blockIdx.x * rpad + c] = cpart[idx];
+        if (c < r) part[(int64_t)blk * rpad + c] = cpart[idx];
     }
     if (threadIdx.x == 0) {
-        bstat[SK_NSTAT * blockIdx.x + 0] = fmin(fmin(red[0][0], red[1][0]), fmin(red[2][0], red[3][0]));
-        bstat[SK_NSTAT * blockIdx.x + 1] = fmax(fmax(red[0][1], red[1][1]), fmax(red[2][1], red[3][1]));
-        bstat[SK_NSTAT * blockIdx.x + 2] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
-        bstat[SK_NSTAT * blockIdx.x + 3] = fmin(fmin(red[0][3], red[1][3]), fmin(red[2][3], red[3][3]));
-        bstat[SK_NSTAT * blockIdx.x + 4] = (red[0][4] + red[1][4]) + (red[2][4] + red[3][4]);
+        bstat[SK_NSTAT * blk + 0] = fmin(fmin(red[0][0], red[1][0]), fmin(red[2][0], red[3][0]));
+        bstat[SK_NSTAT * blk + 1] = fmax(fmax(red[0][1], red[1][1]), fmax(red[2][1], red[3][1]));
+        bstat[SK_NSTAT * blk + 2] = (red[0][2] + red[1][2]) + (red[2][2] + red[3][2]);
+        bstat[SK_NSTAT * blk + 3] = fmin(fmin(red[0][3], red[1][3]), fmin(red[2][3], red[3][3]));
+        bstat[SK_NSTAT * blk + 4] = (red[0][4] + red[1][4]) + (red[2][4] + red[3][4]);
     }
+}
+template <typename T, int VPQ>
+__global__ __launch_bounds__(256) void k_sinkhorn_rows(const T* __restrict__ F, int64_t ldf, int64_t n,
+                                                       int64_t r, const double* __restrict__ u,
+                                                       const double* __restrict__ s, int damped, double* q_out,
+                                                       double* part, int64_t rpad, double* bstat)
+{
+    sinkhorn_rows_body<T, VPQ>(F, ldf, n, r, u, s, damped, q_out, part, rpad, bstat, blockIdx.x);
 }
 
 // [min, max, sum, min, sum] over the nb per-block Sinkhorn statistics, in
 // block order.
-__global__ __launch_bounds__(256) void k_stat_reduce(const double* bstat, int nb, double* out)
+__device__ inline void stat_reduce_body(const double* bstat, int nb, double* out)
 {
     __shared__ double smn[256], sdv[256], ssm[256], ssn[256], se2[256];
     double mn = INFINITY, dv = 0.0, sm = 0.0, sn = INFINITY, e2 = 0.0;
@@ -461,6 +473,10 @@ __global__ __launch_bounds__(256) void k_stat_reduce(const double* bstat, int nb
         out[3] = ssn[0];
         out[4] = se2[0];
     }
+}
+__global__ __launch_bounds__(256) void k_stat_reduce(const double* bstat, int nb, double* out)
+{
+    stat_reduce_body(bstat, nb, out);
 }
 
 // out = a x + b y + c z over r entries (a Sinkhorn iterate over the n
@@ -3424,6 +3440,62 @@ struct SkCheb {
     }
 };
 
+// ------------------------------------------- batched Sinkhorn (ensembles)
+// One Sinkhorn iteration of many problems as one launch per stage: the
+// fused pass over every problem's row blocks (one launch per register
+// tiling VPQ), the column-partial and statistics reductions, and the next
+// iterates.  Each block runs the single-problem kernels' bodies on its
+// problem, so every problem's iterates are those of its own kevd_evd.
+struct SkDesc {
+    const float* F;
+    int64_t ldf, n, r, rpad;
+    const double *u, *s;
+    double *q, *part, *bstat, *v, *st;
+    int damped, nb;
+};
+// out = a x + b y + c z over count entries (k_lincomb3's arithmetic)
+struct LcDesc {
+    double a, b, c;
+    const double *x, *y, *z;
+    double* out;
+    int64_t count;
+};
+
+template <int VPQ>
+__global__ __launch_bounds__(256) void k_ens_skpass(const SkDesc* __restrict__ D, const int2* __restrict__ items)
+{
+    const int2 it = items[blockIdx.x];
+    const SkDesc& d = D[it.x];
+    sinkhorn_rows_body<float, VPQ>(d.F, d.ldf, d.n, d.r, d.u, d.s, d.damped, d.q, d.part, d.rpad, d.bstat, it.y);
+}
+
+// items: (problem, 64-column block) for the partials; block b < nprob of
+// the second launch shape reduces problem b's statistics
+__global__ __launch_bounds__(1024) void k_ens_colsum_reduce(const SkDesc* __restrict__ D, const int2* __restrict__ items)
+{
+    const int2 it = items[blockIdx.x];
+    const SkDesc& d = D[it.x];
+    colsum_reduce_body(d.part, d.nb, d.r, d.rpad, d.v, it.y);
+}
+
+__global__ __launch_bounds__(256) void k_ens_stat_reduce(const SkDesc* __restrict__ D)
+{
+    const SkDesc& d = D[blockIdx.x];
+    stat_reduce_body(d.bstat, d.nb, d.st);
+}
+
+// blockIdx.y: the descriptor; a grid-stride loop over its entries
+__global__ __launch_bounds__(256) void k_ens_lincomb(const LcDesc* __restrict__ L)
+{
+    const LcDesc& d = L[blockIdx.y];
+    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < d.count; j += (int64_t)gridDim.x * 256) {
+        double v = d.a != 0.0 ? d.a * d.x[j] : 0.0;
+        if (d.b != 0.0) v += d.b * d.y[j];
+        if (d.c != 0.0) v += d.c * d.z[j];
+        d.out[j] = v;
+    }
+}
+
 template <typename T, typename Ops>
 kevd_status evd_sinkhorn(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd_evd_result* res)
 {
@@ -3441,101 +3513,113 @@ kevd_status evd_sinkhorn(kevd_ctx* ctx, Ops& ops, const kevd_evd_params* p, kevd
     KEVD_TRY(ctx_buf_t(ctx, "e_lam", r, &lam));
     KEVD_TRY(ctx_buf_t(ctx, "e_isq", k, &isq));
     unsigned gn = (unsigned)((n + 255) / 256);
-    // start from s = 1 / sqrt(mean(K^ 1)) so that s K^ s is O(1); with
-    // u = F^T 1, mean(K^ 1) = 1^T F F^T 1 / n = |u|^2 / n.
     double mm[2];
-    KEVD_TRY(ops.colsum(nullptr, u));
-    std::vector<double> hu((size_t)r);
-    KEVD_HIP(ctx, hipMemcpyAsync(hu.data(), u, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    double mean = 0.0;
-    for (double v : hu) mean += v * v;
-    mean /= (double)n;
-    if (!(mean > 0.0)) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
     int it = 0;
     double dev = INFINITY, smin_fused = NAN;
     int maxit = p->sinkhorn_max_iter > 0 ? p->sinkhorn_max_iter : 1000;
     double tol = p->sinkhorn_tol > 0.0 ? p->sinkhorn_tol : 1e-12;
-    if (sk_fused_ok<T>(r)) {
-        // One fused pass per iteration.  Near the fixed point s* the error
-        // e in s = s* (1 + e) maps under s <- 1 / (K^ s) to -P e, with
-        // P = S* K^ S* bistochastic and positive semidefinite (K^ = F F^T).
-        // Its eigenvalue 1 (the common scale of s) is removed exactly by
-        // T(s) = sqrt(mean(s .* K^ s)) / (K^ s), which is invariant under
-        // s -> c s; every other mode of the map lies in [-lam_2, 0].  The
-        // steps are SkCheb's: two plain steps s <- T(s), then the Chebyshev
-        // semi-iteration for the spectrum [0, lam_2] (three-term recurrence
-        // on s and, by linearity, on u = F^T s).  Three rises of the
-        // deviation in a row switch to the damped update s <- sqrt(s / K^ s)
-        // (rate <= 1/2).
-        double* st3;
-        KEVD_TRY(ctx_buf_t(ctx, "e_skst3", SK_NSTAT, &st3));
-        // iterate s_j in Sb[j % 3] (built by k_lincomb3 before its pass),
-        // q_j in q; u_j = F^T s_j in ucur, u_{j-1} in uprev, v_j = F^T q_j
-        // in uv
-        double *s3, *q, *uprev, *uv, *un;
-        KEVD_TRY(ctx_buf_t(ctx, "e_s3", n, &s3));
-        KEVD_TRY(ctx_buf_t(ctx, "e_q0", n, &q));
-        KEVD_TRY(ctx_buf_t(ctx, "e_u2", r, &uprev));
-        KEVD_TRY(ctx_buf_t(ctx, "e_uv", r, &uv));
-        KEVD_TRY(ctx_buf_t(ctx, "e_un", r, &un));
-        double* Sb[3] = {s, s2, s3};
-        double* ucur = u;
-        unsigned gr = (unsigned)((r + 255) / 256);
-        // s_0 = c0 * 1 and u_0 = c0 F^T 1
-        double c0 = 1.0 / std::sqrt(mean);
-        k_fill_d<<<gn, 256, 0, ctx->stream>>>(Sb[0], n, c0);
-        k_lincomb3<<<gr, 256, 0, ctx->stream>>>(r, c0, u, 0.0, nullptr, 0.0, nullptr, ucur);
-        int damped = 0;
-        SkCheb cheb;
-        int rises = 0;
-        double prev = INFINITY, st[SK_NSTAT];
-        int j = 0;
-        for (it = 1; it <= maxit; ++it, ++j) {
-            KEVD_TRY(ops.skpass(ucur, Sb[j % 3], damped, q, uv, st3));
-            KEVD_HIP(ctx, hipGetLastError());
-            KEVD_HIP(ctx, hipMemcpyAsync(st, st3, SK_NSTAT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-            KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            if (!(st[0] > 0.0)) {
-                // the first pass sees K^ 1 up to a positive constant: a degree
-                if (it == 1) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
-                return kevd_fail(ctx, KEVD_ENONPOS_Q, "Sinkhorn: non-positive K^ s");
-            }
-            dev = st[1];
-            smin_fused = st[3];
-            s = Sb[j % 3];  // the iterate this pass measured
-            if (dev <= tol || it == maxit) break;
-            bool was_damped = damped != 0;
-            rises = dev > prev ? rises + 1 : 0;
-            prev = dev;
-            // s_{j+1} = a1 s_j + a2 s_{j-1} + a3 q_j
-            double a1, a2, a3;
-            if (was_damped) {
-                a1 = 0.0, a2 = 0.0, a3 = 1.0;
-            } else {
-                double m = st[2] / (double)n;
-                cheb.next(std::sqrt(st[4] / (double)n), &a1, &a2, &a3);
-                a3 *= std::sqrt(m);
-            }
-            k_lincomb3<<<gr, 256, 0, ctx->stream>>>(r, a1, ucur, a2, uprev, a3, uv, un);
-            k_lincomb3<<<gn, 256, 0, ctx->stream>>>(n, a1, Sb[j % 3], a2, Sb[(j + 2) % 3], a3, q, Sb[(j + 1) % 3]);
-            std::swap(uprev, ucur);  // uprev = u_j
-            std::swap(ucur, un);     // ucur = u_{j+1}; un free
-            damped = (was_damped || rises >= 3) ? 1 : 0;
-        }
+    // kevd_ensemble_evd's batched iteration may have scaled this problem
+    // already (the same passes and schedule): take its result
+    const bool pre = ctx->sk_pre.valid;
+    ctx->sk_pre.valid = false;
+    if (pre) {
+        ctx->tm.sinkhorn_batched = 1;
+        s = ctx->sk_pre.s;
+        it = ctx->sk_pre.iters;
+        dev = ctx->sk_pre.dev;
+        smin_fused = ctx->sk_pre.smin;
     } else {
-        // rank beyond the register tiling: two passes per damped iteration
-        k_fill_d<<<gn, 256, 0, ctx->stream>>>(s, n, 1.0 / std::sqrt(mean));
-        for (it = 1; it <= maxit; ++it) {
-            KEVD_TRY(ops.colsum(s, u));
-            KEVD_TRY(ops.rowdot(u, Ks, 1, s, s2, mm));
-            if (!(mm[0] > 0.0)) {
-                if (it == 1) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
-                return kevd_fail(ctx, KEVD_ENONPOS_Q, "Sinkhorn: non-positive K^ s");
+        // start from s = 1 / sqrt(mean(K^ 1)) so that s K^ s is O(1); with
+        // u = F^T 1, mean(K^ 1) = 1^T F F^T 1 / n = |u|^2 / n.
+        KEVD_TRY(ops.colsum(nullptr, u));
+        std::vector<double> hu((size_t)r);
+        KEVD_HIP(ctx, hipMemcpyAsync(hu.data(), u, r * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        double mean = 0.0;
+        for (double v : hu) mean += v * v;
+        mean /= (double)n;
+        if (!(mean > 0.0)) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
+        if (sk_fused_ok<T>(r)) {
+            // One fused pass per iteration.  Near the fixed point s* the error
+            // e in s = s* (1 + e) maps under s <- 1 / (K^ s) to -P e, with
+            // P = S* K^ S* bistochastic and positive semidefinite (K^ = F F^T).
+            // Its eigenvalue 1 (the common scale of s) is removed exactly by
+            // T(s) = sqrt(mean(s .* K^ s)) / (K^ s), which is invariant under
+            // s -> c s; every other mode of the map lies in [-lam_2, 0].  The
+            // steps are SkCheb's: two plain steps s <- T(s), then the Chebyshev
+            // semi-iteration for the spectrum [0, lam_2] (three-term recurrence
+            // on s and, by linearity, on u = F^T s).  Three rises of the
+            // deviation in a row switch to the damped update s <- sqrt(s / K^ s)
+            // (rate <= 1/2).
+            double* st3;
+            KEVD_TRY(ctx_buf_t(ctx, "e_skst3", SK_NSTAT, &st3));
+            // iterate s_j in Sb[j % 3] (built by k_lincomb3 before its pass),
+            // q_j in q; u_j = F^T s_j in ucur, u_{j-1} in uprev, v_j = F^T q_j
+            // in uv
+            double *s3, *q, *uprev, *uv, *un;
+            KEVD_TRY(ctx_buf_t(ctx, "e_s3", n, &s3));
+            KEVD_TRY(ctx_buf_t(ctx, "e_q0", n, &q));
+            KEVD_TRY(ctx_buf_t(ctx, "e_u2", r, &uprev));
+            KEVD_TRY(ctx_buf_t(ctx, "e_uv", r, &uv));
+            KEVD_TRY(ctx_buf_t(ctx, "e_un", r, &un));
+            double* Sb[3] = {s, s2, s3};
+            double* ucur = u;
+            unsigned gr = (unsigned)((r + 255) / 256);
+            // s_0 = c0 * 1 and u_0 = c0 F^T 1
+            double c0 = 1.0 / std::sqrt(mean);
+            k_fill_d<<<gn, 256, 0, ctx->stream>>>(Sb[0], n, c0);
+            k_lincomb3<<<gr, 256, 0, ctx->stream>>>(r, c0, u, 0.0, nullptr, 0.0, nullptr, ucur);
+            int damped = 0;
+            SkCheb cheb;
+            int rises = 0;
+            double prev = INFINITY, st[SK_NSTAT];
+            int j = 0;
+            for (it = 1; it <= maxit; ++it, ++j) {
+                KEVD_TRY(ops.skpass(ucur, Sb[j % 3], damped, q, uv, st3));
+                KEVD_HIP(ctx, hipGetLastError());
+                KEVD_HIP(ctx, hipMemcpyAsync(st, st3, SK_NSTAT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+                KEVD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                if (!(st[0] > 0.0)) {
+                    // the first pass sees K^ 1 up to a positive constant: a degree
+                    if (it == 1) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
+                    return kevd_fail(ctx, KEVD_ENONPOS_Q, "Sinkhorn: non-positive K^ s");
+                }
+                dev = st[1];
+                smin_fused = st[3];
+                s = Sb[j % 3];  // the iterate this pass measured
+                if (dev <= tol || it == maxit) break;
+                bool was_damped = damped != 0;
+                rises = dev > prev ? rises + 1 : 0;
+                prev = dev;
+                // s_{j+1} = a1 s_j + a2 s_{j-1} + a3 q_j
+                double a1, a2, a3;
+                if (was_damped) {
+                    a1 = 0.0, a2 = 0.0, a3 = 1.0;
+                } else {
+                    double m = st[2] / (double)n;
+                    cheb.next(std::sqrt(st[4] / (double)n), &a1, &a2, &a3);
+                    a3 *= std::sqrt(m);
+                }
+                k_lincomb3<<<gr, 256, 0, ctx->stream>>>(r, a1, ucur, a2, uprev, a3, uv, un);
+                k_lincomb3<<<gn, 256, 0, ctx->stream>>>(n, a1, Sb[j % 3], a2, Sb[(j + 2) % 3], a3, q, Sb[(j + 1) % 3]);
+                std::swap(uprev, ucur);  // uprev = u_j
+                std::swap(ucur, un);     // ucur = u_{j+1}; un free
+                damped = (was_damped || rises >= 3) ? 1 : 0;
             }
-            dev = mm[1];
-            if (dev <= tol || it == maxit) break;
-            std::swap(s, s2);
+        } else {
+            // rank beyond the register tiling: two passes per damped iteration
+            k_fill_d<<<gn, 256, 0, ctx->stream>>>(s, n, 1.0 / std::sqrt(mean));
+            for (it = 1; it <= maxit; ++it) {
+                KEVD_TRY(ops.colsum(s, u));
+                KEVD_TRY(ops.rowdot(u, Ks, 1, s, s2, mm));
+                if (!(mm[0] > 0.0)) {
+                    if (it == 1) return kevd_fail(ctx, KEVD_ENONPOS_DEGREE, "non-positive row sum of K^");
+                    return kevd_fail(ctx, KEVD_ENONPOS_Q, "Sinkhorn: non-positive K^ s");
+                }
+                dev = mm[1];
+                if (dev <= tol || it == maxit) break;
+                std::swap(s, s2);
+            }
         }
     }
     res->sinkhorn_iters = it > maxit ? maxit : it;
@@ -3598,6 +3682,205 @@ kevd_status run_evd(kevd_ctx* ctx, bool panels, const kevd_evd_params* p, kevd_e
 
 }  // namespace
 
+// The batched Sinkhorn iteration of kevd_ensemble_evd (declared in
+// internal.h): evd_sinkhorn's fused schedule (SkCheb, the damped switch
+// after three rises, the stop at tol or the step limit) run for every
+// problem in lockstep, one launch per stage for all problems and one host
+// synchronisation per iteration for all of them instead of one per problem
+// and iteration.  The iterates live in each problem's own named buffers,
+// under the names evd_sinkhorn uses, and the scaling is left in sk_pre for
+// the problem's kevd_evd, which then skips its own iteration.
+kevd_status ens_sinkhorn(kevd_ctx** ctxs, int P, const kevd_evd_params* p, int* batched)
+{
+    *batched = 0;
+    kevd_ctx* m = ctxs[0];
+    for (int i = 0; i < P; ++i) ctxs[i]->sk_pre.valid = false;
+    if (p->norm != KEVD_NORM_SINKHORN || P < 2) return KEVD_OK;
+    for (int i = 0; i < P; ++i) {
+        kevd_ctx* c = ctxs[i];
+        const int64_t r = c->rank;
+        // the fused FP32 pass and F^T 1 from the downdate's partials
+        if (!c->have_factor || c->fprec != KEVD_MIXED || !c->cs_valid || !c->cs_part || c->cs_nblk < 1 ||
+            c->device != m->device || r < 1 || ((r + 3) / 4 + 63) / 64 > 8 || c->n < 1)
+            return KEVD_OK;
+    }
+    TraceRange tr("kEVD batched Sinkhorn");
+    KEVD_HIP(m, hipSetDevice(m->device));
+    for (int i = 0; i < P; ++i) KEVD_HIP(ctxs[i], hipStreamSynchronize(ctxs[i]->stream));
+    hipStream_t st = m->stream;
+    const int maxit = p->sinkhorn_max_iter > 0 ? p->sinkhorn_max_iter : 1000;
+    const double tol = p->sinkhorn_tol > 0.0 ? p->sinkhorn_tol : 1e-12;
+    struct Prob {
+        int64_t n, r, rpad;
+        int nb, grp;  // row blocks; register tiling VPQ = 2 << grp
+        double *Sb[3], *q, *ucur, *uprev, *uv, *un, *part, *bstat;
+        int j = 0, damped = 0, rises = 0;
+        double prev = INFINITY;
+        SkCheb cheb;
+        bool active = true, done = false;
+        double dev = INFINITY, smin = NAN, *sfin = nullptr;
+        int iters = 0;
+    };
+    std::vector<Prob> pr((size_t)P);
+    int64_t nitems = 0, ncitems = 0;
+    for (int i = 0; i < P; ++i) {
+        kevd_ctx* c = ctxs[i];
+        Prob& q = pr[i];
+        q.n = c->n;
+        q.r = c->rank;
+        q.rpad = KEVD_ROUND_UP(q.r, 4);
+        q.nb = (int)((q.n + SK_ROWS - 1) / SK_ROWS);
+        const int vpq = (int)(((q.r + 3) / 4 + 63) / 64);
+        q.grp = vpq <= 2 ? 0 : (vpq <= 4 ? 1 : 2);
+        KEVD_TRY(ctx_buf_t(c, "e_s", q.n, &q.Sb[0]));
+        KEVD_TRY(ctx_buf_t(c, "e_s2", q.n, &q.Sb[1]));
+        KEVD_TRY(ctx_buf_t(c, "e_s3", q.n, &q.Sb[2]));
+        KEVD_TRY(ctx_buf_t(c, "e_q0", q.n, &q.q));
+        KEVD_TRY(ctx_buf_t(c, "e_u", q.r, &q.ucur));
+        KEVD_TRY(ctx_buf_t(c, "e_u2", q.r, &q.uprev));
+        KEVD_TRY(ctx_buf_t(c, "e_uv", q.r, &q.uv));
+        KEVD_TRY(ctx_buf_t(c, "e_un", q.r, &q.un));
+        KEVD_TRY(ctx_buf_t(c, "e_skpart", (size_t)q.nb * q.rpad, &q.part));
+        KEVD_TRY(ctx_buf_t(c, "e_skstat", (size_t)SK_NSTAT * q.nb, &q.bstat));
+        nitems += q.nb;
+        ncitems += (q.r + 63) / 64;
+    }
+    // device and pinned staging: descriptors, work items, statistics
+    const size_t b_desc = (size_t)P * sizeof(SkDesc), b_items = (size_t)nitems * sizeof(int2),
+                 b_citems = (size_t)ncitems * sizeof(int2), b_lc = (size_t)2 * P * sizeof(LcDesc),
+                 b_stat = (size_t)P * SK_NSTAT * sizeof(double);
+    const size_t o_items = KEVD_ROUND_UP(b_desc, 256), o_citems = o_items + KEVD_ROUND_UP(b_items, 256),
+                 o_lc = o_citems + KEVD_ROUND_UP(b_citems, 256), o_stat = o_lc + KEVD_ROUND_UP(b_lc, 256),
+                 total = o_stat + KEVD_ROUND_UP(b_stat, 256);
+    unsigned char *dst, *hst;
+    KEVD_TRY(ctx_buf(m, "es_stage", total, (void**)&dst));
+    KEVD_TRY(ctx_pinned(m, total, (void**)&hst));
+    double* dstat = reinterpret_cast<double*>(dst + o_stat);
+    const double* hstat = reinterpret_cast<const double*>(hst + o_stat);
+    // u_0 = F^T 1 from the downdate's partials; mean(K^ 1) = |u_0|^2 / n
+    std::vector<std::vector<double>> hu((size_t)P);
+    for (int i = 0; i < P; ++i) {
+        kevd_ctx* c = ctxs[i];
+        Prob& q = pr[i];
+        k_colsum_reduce<<<(unsigned)((q.r + 63) / 64), 1024, 0, st>>>(c->cs_part, (int)c->cs_nblk, q.r, c->ldf,
+                                                                       q.ucur);
+        hu[i].resize((size_t)q.r);
+        KEVD_HIP(m, hipMemcpyAsync(hu[i].data(), q.ucur, q.r * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    KEVD_HIP(m, hipStreamSynchronize(st));
+    for (int i = 0; i < P; ++i) {
+        Prob& q = pr[i];
+        double mean = 0.0;
+        for (double v : hu[i]) mean += v * v;
+        mean /= (double)q.n;
+        if (!(mean > 0.0)) {  // its own kEVD reports the failure
+            q.active = false;
+            continue;
+        }
+        const double c0 = 1.0 / std::sqrt(mean);
+        k_fill_d<<<(unsigned)((q.n + 255) / 256), 256, 0, st>>>(q.Sb[0], q.n, c0);
+        k_lincomb3<<<(unsigned)((q.r + 255) / 256), 256, 0, st>>>(q.r, c0, q.ucur, 0.0, nullptr, 0.0, nullptr,
+                                                                   q.ucur);
+    }
+    KEVD_HIP(m, hipGetLastError());
+    std::vector<int> act;
+    for (int it = 1; it <= maxit; ++it) {
+        act.clear();
+        for (int i = 0; i < P; ++i)
+            if (pr[i].active) act.push_back(i);
+        if (act.empty()) break;
+        const int na = (int)act.size();
+        SkDesc* hd = reinterpret_cast<SkDesc*>(hst);
+        int2* hi = reinterpret_cast<int2*>(hst + o_items);
+        int2* hci = reinterpret_cast<int2*>(hst + o_citems);
+        int64_t ni[3] = {0, 0, 0}, nci = 0;
+        for (int a = 0; a < na; ++a) {
+            const Prob& q = pr[act[a]];
+            hd[a] = SkDesc{(const float*)ctxs[act[a]]->F, ctxs[act[a]]->ldf, q.n, q.r, q.rpad, q.ucur, q.Sb[q.j % 3],
+                           q.q, q.part, q.bstat, q.uv, dstat + (int64_t)SK_NSTAT * a, q.damped, q.nb};
+            for (int c = 0; c < (int)((q.r + 63) / 64); ++c) hci[nci++] = make_int2(a, c);
+        }
+        // row-block items grouped by register tiling, heaviest group first
+        int64_t off[3] = {0, 0, 0};
+        for (int a = 0; a < na; ++a) ni[pr[act[a]].grp] += pr[act[a]].nb;
+        off[1] = ni[0];
+        off[2] = ni[0] + ni[1];
+        int64_t fill[3] = {off[0], off[1], off[2]};
+        for (int a = 0; a < na; ++a)
+            for (int b = 0; b < pr[act[a]].nb; ++b) hi[fill[pr[act[a]].grp]++] = make_int2(a, b);
+        KEVD_HIP(m, hipMemcpyAsync(dst, hst, o_lc, hipMemcpyHostToDevice, st));
+        const SkDesc* dd = reinterpret_cast<const SkDesc*>(dst);
+        const int2* di = reinterpret_cast<const int2*>(dst + o_items);
+        if (ni[0]) k_ens_skpass<2><<<(unsigned)ni[0], 256, 0, st>>>(dd, di + off[0]);
+        if (ni[1]) k_ens_skpass<4><<<(unsigned)ni[1], 256, 0, st>>>(dd, di + off[1]);
+        if (ni[2]) k_ens_skpass<8><<<(unsigned)ni[2], 256, 0, st>>>(dd, di + off[2]);
+        k_ens_colsum_reduce<<<(unsigned)nci, 1024, 0, st>>>(dd, reinterpret_cast<const int2*>(dst + o_citems));
+        k_ens_stat_reduce<<<(unsigned)na, 256, 0, st>>>(dd);
+        KEVD_HIP(m, hipGetLastError());
+        KEVD_HIP(m, hipMemcpyAsync(hst + o_stat, dstat, (size_t)na * SK_NSTAT * sizeof(double),
+                                   hipMemcpyDeviceToHost, st));
+        KEVD_HIP(m, hipStreamSynchronize(st));
+        // each problem's step, as evd_sinkhorn's loop takes it
+        LcDesc* hl = reinterpret_cast<LcDesc*>(hst + o_lc);
+        int nl = 0;
+        int64_t lmax = 0;
+        for (int a = 0; a < na; ++a) {
+            Prob& q = pr[act[a]];
+            const double* sv = hstat + (int64_t)SK_NSTAT * a;
+            if (!(sv[0] > 0.0)) {  // its own kEVD reports the failure
+                q.active = false;
+                continue;
+            }
+            q.dev = sv[1];
+            q.smin = sv[3];
+            if (q.dev <= tol || it == maxit) {
+                q.active = false;
+                q.done = true;
+                q.sfin = q.Sb[q.j % 3];
+                q.iters = it;
+                continue;
+            }
+            const bool was_damped = q.damped != 0;
+            q.rises = q.dev > q.prev ? q.rises + 1 : 0;
+            q.prev = q.dev;
+            double a1, a2, a3;
+            if (was_damped) {
+                a1 = 0.0, a2 = 0.0, a3 = 1.0;
+            } else {
+                const double mq = sv[2] / (double)q.n;
+                q.cheb.next(std::sqrt(sv[4] / (double)q.n), &a1, &a2, &a3);
+                a3 *= std::sqrt(mq);
+            }
+            hl[nl++] = LcDesc{a1, a2, a3, q.ucur, q.uprev, q.uv, q.un, q.r};
+            hl[nl++] = LcDesc{a1, a2, a3, q.Sb[q.j % 3], q.Sb[(q.j + 2) % 3], q.q, q.Sb[(q.j + 1) % 3], q.n};
+            lmax = std::max(lmax, q.n);
+            std::swap(q.uprev, q.ucur);
+            std::swap(q.ucur, q.un);
+            q.damped = (was_damped || q.rises >= 3) ? 1 : 0;
+            ++q.j;
+        }
+        if (nl > 0) {
+            KEVD_HIP(m, hipMemcpyAsync(dst + o_lc, hst + o_lc, (size_t)nl * sizeof(LcDesc), hipMemcpyHostToDevice,
+                                       st));
+            const unsigned gx = (unsigned)std::min<int64_t>((lmax + 255) / 256, 1024);
+            k_ens_lincomb<<<dim3(gx, (unsigned)nl), 256, 0, st>>>(reinterpret_cast<const LcDesc*>(dst + o_lc));
+            KEVD_HIP(m, hipGetLastError());
+        }
+    }
+    KEVD_HIP(m, hipStreamSynchronize(st));
+    for (int i = 0; i < P; ++i) {
+        if (!pr[i].done) continue;
+        kevd_ctx::SkPre& sp = ctxs[i]->sk_pre;
+        sp.valid = true;
+        sp.s = pr[i].sfin;
+        sp.iters = pr[i].iters;
+        sp.dev = pr[i].dev;
+        sp.smin = pr[i].smin;
+        ++*batched;
+    }
+    return KEVD_OK;
+}
+
 extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_evd_result* res)
 {
     TraceRange trace_range("kevd_evd");
@@ -3640,6 +3923,7 @@ extern "C" kevd_status kevd_evd(kevd_ctx* ctx, const kevd_evd_params* p, kevd_ev
     ctx->tm.eig_full_fallback = 0;
     ctx->tm.eig_orth_passes = 0;
     ctx->tm.eig_batched = 0;
+    ctx->tm.sinkhorn_batched = 0;
     prof_reset(ctx);
     size_t ubytes = (size_t)ctx->n * p->k_out * sizeof(float);
     if (ctx->U_bytes < ubytes) {

@@ -78,6 +78,17 @@ struct kevd_ctx {
     struct EigBatch* eig_batch = nullptr;
     int eig_batch_slot = -1;
 
+    // Set by kevd_ensemble_evd when its batched Sinkhorn iteration has
+    // scaled this problem: the next Sinkhorn kEVD takes the scaling s (one
+    // of its named buffers, n entries), the iteration count, the final
+    // deviation and min(s) instead of iterating itself.
+    struct SkPre {
+        bool valid = false;
+        double* s = nullptr;
+        int iters = 0;
+        double dev = 0.0, smin = 0.0;
+    } sk_pre;
+
     // Device address of this device's fixed scratch array (KEVD_SCRATCH_BYTES),
     // resolved when the context is created; shared by every context on the
     // device.
@@ -162,6 +173,11 @@ kevd_status pivot_cholesky(kevd_ctx* ctx, double* L);
 // joining must leave (eig_batch_leave is a no-op for one that joined).
 struct EigBatch;
 EigBatch* eig_batch_create(kevd_ctx* master, int nslots, int k_out);
+// The Sinkhorn iterations of an ensemble's kEVDs as batched launches on the
+// master context's stream (FP32 factors from kevd_arpc, rank <= 2048); sets
+// each scaled problem's sk_pre.  Problems it cannot take keep their own
+// iteration.  *batched: the number it scaled.
+kevd_status ens_sinkhorn(kevd_ctx** ctxs, int nprob, const kevd_evd_params* p, int* batched);
 void eig_batch_leave(EigBatch* b, int slot);
 void eig_batch_destroy(EigBatch* b);
 kevd_status kblock_gather(kevd_ctx* ctx, const int64_t* idx_dev, int64_t count,

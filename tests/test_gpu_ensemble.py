@@ -51,6 +51,9 @@ def test_ensemble_mixed_matches_single_problem_runs():
     print("ensemble ranks", [o.rank for o in outs], "graph", used_graph, flush=True)
     assert used_graph
     evs = ensemble_evd(ctxs, [o.rank for o in outs], k_out=12, norm="sinkhorn", prec="mixed", workers=3)
+    # the Sinkhorn iterations ran batched over the problems, with each
+    # problem's own schedule (compared with its single-problem run below)
+    assert all(c.timings()["sinkhorn_batched"] == 1 for c in ctxs)
     for i, (X, o) in enumerate(zip(Xs, outs)):
         assert o.rank == ranks[i]
         with Context(0) as c:
@@ -63,6 +66,8 @@ def test_ensemble_mixed_matches_single_problem_runs():
             Fr = c.get_factor(ref.rank, "mixed")
             assert np.array_equal(Fe, Fr), "problem %d factor differs" % i
             er = c.evd(ref.rank, k_out=12, prec="mixed", norm="sinkhorn")
+            assert c.timings()["sinkhorn_batched"] == 0
+            assert evs[i].sinkhorn_iters == er.sinkhorn_iters
             np.testing.assert_allclose(evs[i].lam[-12:], er.lam[-12:], rtol=1e-10, atol=0)
             assert evs[i].rowsum_dev <= 1e-12
     for c in ctxs:
@@ -77,8 +82,11 @@ def test_ensemble_batched_eigensolve_matches_own_solves(monkeypatch):
                             prec="mixed")
     rk = [o.rank for o in outs]
     monkeypatch.setenv("KEVD_ENS_EIG_BATCH", "0")
+    monkeypatch.setenv("KEVD_ENS_SK_BATCH", "0")
     own = ensemble_evd(ctxs, rk, k_out=12, norm="sinkhorn", prec="mixed", workers=3, want_U=True)
+    assert all(c.timings()["sinkhorn_batched"] == 0 for c in ctxs)
     monkeypatch.delenv("KEVD_ENS_EIG_BATCH", raising=False)
+    monkeypatch.delenv("KEVD_ENS_SK_BATCH", raising=False)
     bat = ensemble_evd(ctxs, rk, k_out=12, norm="sinkhorn", prec="mixed", workers=3, want_U=True)
     tms = [c.timings() for c in ctxs]
     print("ranks", rk, "eig iters", [t["eig_iters"] for t in tms], "batched", [t["eig_batched"] for t in tms],
