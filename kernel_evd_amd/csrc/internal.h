@@ -69,6 +69,9 @@ struct kevd_ctx {
     // this context converged (0: none yet); the next solve runs its first
     // Rayleigh-Ritz check there.
     int eig_converged_at = 0;
+    // The same for the ensemble's batched solve, kept on its master context
+    // (the first context of kevd_ensemble_evd).
+    int ens_eig_converged_at = 0;
 
     // Tile of the bf16x6 Gram (256 or 128; 0: 256), set by kevd_autotune_tiles.
     int gram_tile = 0;
