@@ -2434,7 +2434,8 @@ bool chol_panel_only()
 // Every wave forms the reflector itself from the published column, so a
 // column costs three barriers.
 constexpr int SYTRD_LMAX = 256;
-__global__ __launch_bounds__(512) void k_sytrd_lower(double* A, int l, double* D, double* E, double* tau)
+__device__ __attribute__((always_inline)) inline void sytrd_lower_body(double* A, int l, double* D, double* E,
+                                                                       double* tau)
 {
     constexpr int NBK = SYTRD_LMAX / 32, NBLK = NBK * (NBK + 1) / 2;
     __shared__ double colv[2][SYTRD_LMAX], pv[2][SYTRD_LMAX];
@@ -2564,6 +2565,197 @@ __global__ __launch_bounds__(512) void k_sytrd_lower(double* A, int l, double* D
             }
     }
 }
+__global__ __launch_bounds__(512) void k_sytrd_lower(double* A, int l, double* D, double* E, double* tau)
+{
+    sytrd_lower_body(A, l, D, E, tau);
+}
+// matrix b = blockIdx.x at A + b l^2 (D, E, tau + b l)
+__global__ __launch_bounds__(512) void k_sytrd_lower_b(double* A, int l, double* D, double* E, double* tau)
+{
+    const int64_t b = blockIdx.x;
+    sytrd_lower_body(A + b * l * l, l, D + b * l, E + b * l, tau + b * l);
+}
+
+// Eigenpairs of the symmetric tridiagonal matrices of a batch (matrix b:
+// diagonal D + b l, sub-diagonal E + b l, order l <= TQL_LMAX), one
+// workgroup each, by implicit symmetric QR steps with the Wilkinson shift
+// (Golub and Van Loan, Matrix Computations, 4th ed., Alg. 8.3.2 and 8.3.3):
+// each step takes the bottom unreduced block [p, q] (sub-diagonal entries
+// below DBL_EPSILON (|a_i| + |a_i+1|) are set to zero first) and chases the
+// bulge of a shifted Givens rotation down it.  Wave 0 forms the
+// steps (the deflation scan by the whole wave, the rotations by lane 0); the
+// next waves apply them to the rows of Z (held in LDS, one thread per row),
+// one step behind, so a step costs one barrier.  The
+// eigenvalues go to lam + b l ascending and the eigenvectors to the
+// column-major l x l matrix Z + b l^2 in the same order; *info (+ b) = 1 if
+// the steps exceed 30 l.
+constexpr int TQL_LMAX = 128;
+__device__ inline int tql_step(double* a, double* e, int l, double* c, double* s, int* lo)
+{
+    // (called by all 64 lanes of one wave; l <= 128) deflation: lane i tests
+    // e[i] and e[i + 64], and two ballots give the bottom unreduced block
+    const int lane = threadIdx.x & 63;
+    uint64_t m[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int i = lane + 64 * h;
+        bool nz = false;
+        if (i + 1 < l) {
+            double ei = e[i];
+            if (fabs(ei) <= DBL_EPSILON * (fabs(a[i]) + fabs(a[i + 1]))) {
+                e[i] = 0.0;
+                ei = 0.0;
+            }
+            nz = ei != 0.0;
+        }
+        m[h] = __ballot(nz);
+    }
+    // q - 1: the last coupling left; p: one past the last zero below it
+    const int top = m[1] ? 127 - __builtin_clzll(m[1]) : (m[0] ? 63 - __builtin_clzll(m[0]) : -1);
+    if (top < 0) return 0;
+    const uint64_t below0 = top >= 64 ? ~0ull : (top ? (1ull << top) - 1 : 0ull);
+    const uint64_t below1 = top > 64 ? (1ull << (top - 64)) - 1 : 0ull;
+    const uint64_t z0 = ~m[0] & below0, z1 = ~m[1] & below1;
+    const int zt = z1 ? 127 - __builtin_clzll(z1) : (z0 ? 63 - __builtin_clzll(z0) : -1);
+    const int p = zt + 1, q = top + 1;
+    *lo = p;
+    if (lane != 0) return q - p;
+    // the Wilkinson shift: the eigenvalue of the trailing 2 x 2 closer to a_q
+    const double d = 0.5 * (a[q - 1] - a[q]), bq = e[q - 1];
+    const double mu = a[q] - bq * bq / (d + (d >= 0.0 ? 1.0 : -1.0) * sqrt(d * d + bq * bq));
+    // the chain of rotations carries a[k] and e[k] (as updated by the
+    // previous rotation) in registers; a[k + 1] and e[k + 1] are read as they
+    // were, so the loads need not wait for the chain
+    double x = a[p] - mu, z = e[p];
+    double ak = a[p], ek = e[p], en = 0.0;
+    for (int k = p; k < q; ++k) {
+        const double ga = a[k + 1], e1 = k + 1 < q ? e[k + 1] : 0.0;
+        // the rotation of rows/columns (k, k + 1) that zeroes z against x
+        // (one division: c = x / r and s = -z / r through 1 / r)
+        const double r = sqrt(x * x + z * z);
+        const bool nzr = r > 0.0;
+        const double ir = nzr ? 1.0 / r : 0.0;
+        const double ck = nzr ? x * ir : 1.0, sk = nzr ? -z * ir : 0.0;
+        if (k > p) e[k - 1] = r;
+        const double al = ak, be = ek;
+        a[k] = ck * ck * al - 2.0 * ck * sk * be + sk * sk * ga;
+        ak = sk * sk * al + 2.0 * ck * sk * be + ck * ck * ga;
+        en = ck * sk * (al - ga) + (ck * ck - sk * sk) * be;
+        // the bulge at (k + 2, k) and the next rotation's pair
+        z = -sk * e1;
+        ek = ck * e1;
+        x = en;
+        c[k - p] = ck;
+        s[k - p] = sk;
+    }
+    a[q] = ak;
+    e[q - 1] = en;
+    return q - p;
+}
+
+__global__ __launch_bounds__(256) void k_tridiag_eig_b(const double* __restrict__ D, const double* __restrict__ E,
+                                                       int l, double* __restrict__ Z, double* __restrict__ lam,
+                                                       int* __restrict__ info)
+{
+    const int64_t b = blockIdx.x;
+    D += b * l;
+    E += b * l;
+    Z += b * l * l;
+    lam += b * l;
+    info += b;
+    __shared__ double zs[TQL_LMAX][TQL_LMAX + 1];  // Z row-major, padded
+    __shared__ double a[TQL_LMAX], e[TQL_LMAX], rc[2][TQL_LMAX], rs[2][TQL_LMAX];
+    __shared__ int nrot[2], rlo[2], rank[TQL_LMAX];
+    const int tid = threadIdx.x, row = tid - 64;
+    for (int i = tid; i < l * l; i += 256) zs[i / l][i % l] = i / l == i % l ? 1.0 : 0.0;
+    for (int i = tid; i < l; i += 256) {
+        a[i] = D[i];
+        e[i] = i + 1 < l ? E[i] : 0.0;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        int lo = 0;
+        const int nr = tql_step(a, e, l, rc[0], rs[0], &lo);
+        if (tid == 0) {
+            nrot[0] = nr;
+            rlo[0] = lo;
+        }
+    }
+    __syncthreads();
+    int bad = 0;
+    for (int t = 0;; ++t) {
+        const int nr = nrot[t & 1], lo = rlo[t & 1];
+        if (nr == 0) break;  // every thread reads the same count: a uniform exit
+        if (t >= 30 * l) {
+            bad = 1;
+            break;
+        }
+        if (tid < 64) {
+            // the next step, into the other buffer
+            int lo2 = 0;
+            const int nr2 = tql_step(a, e, l, rc[(t + 1) & 1], rs[(t + 1) & 1], &lo2);
+            if (tid == 0) {
+                nrot[(t + 1) & 1] = nr2;
+                rlo[(t + 1) & 1] = lo2;
+            }
+        } else if (row >= 0 && row < l) {
+            // Z <- Z R_k^T for the step's rotations, on this row
+            const double* cc = rc[t & 1];
+            const double* ss = rs[t & 1];
+            double zk = zs[row][lo];
+            for (int j = 0; j < nr; ++j) {
+                const double zk1 = zs[row][lo + j + 1];
+                zs[row][lo + j] = cc[j] * zk - ss[j] * zk1;
+                zk = ss[j] * zk + cc[j] * zk1;
+            }
+            zs[row][lo + nr] = zk;
+        }
+        __syncthreads();
+    }
+    // ascending order: the rank of each eigenvalue (ties by index)
+    for (int i = tid; i < l; i += 256) {
+        int rk = 0;
+        for (int j = 0; j < l; ++j) rk += (a[j] < a[i] || (a[j] == a[i] && j < i)) ? 1 : 0;
+        rank[i] = rk;
+        lam[rk] = a[i];
+    }
+    __syncthreads();
+    for (int r = tid; r < l; r += 256)
+        for (int i = 0; i < l; ++i) Z[r + (int64_t)rank[i] * l] = zs[r][i];
+    if (tid == 0) *info = bad;
+}
+
+// Z <- Q Z for the orthogonal Q = H_0 H_1 ... H_{l-2} of k_sytrd_lower's
+// reflectors (H_i = I - tau_i v v^T, v(i+1) = 1, v(i+2:l) = A(i+2:l, i)), as
+// LAPACK dormtr('L', 'L', 'N'), for each l x l column-major Z of a batch
+// (l <= TQL_LMAX), one workgroup each: thread j holds column j, the
+// reflectors applied last to first.
+__global__ __launch_bounds__(256) void k_ormtr_b(const double* __restrict__ A, const double* __restrict__ tau, int l,
+                                                 double* __restrict__ Z)
+{
+    const int64_t b = blockIdx.x;
+    A += b * l * l;
+    tau += b * l;
+    Z += b * l * l;
+    __shared__ double zs[TQL_LMAX][TQL_LMAX + 1];
+    __shared__ double v[TQL_LMAX];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < l * l; i += 256) zs[i % l][i / l] = Z[i];
+    for (int i = l - 2; i >= 0; --i) {
+        __syncthreads();  // the previous reflector's updates are done with v
+        for (int r = tid; r < l; r += 256) v[r] = r == i + 1 ? 1.0 : (r > i + 1 ? A[r + (int64_t)i * l] : 0.0);
+        __syncthreads();
+        if (tid < l) {
+            double w = 0.0;
+            for (int r = i + 1; r < l; ++r) w += v[r] * zs[r][tid];
+            w *= tau[i];
+            for (int r = i + 1; r < l; ++r) zs[r][tid] -= v[r] * w;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < l * l; i += 256) Z[i] = zs[i % l][i / l];
+}
+
 
 // C = A^T B for column-major r x l blocks A, B (lda = ldb = r; C is l x l,
 // ldc = l), FP64.  With l x l outputs and K = r the plain GEMM runs on a
@@ -2956,6 +3148,8 @@ kevd_status eig_batch_run(EigBatch& B)
     KEVD_TRY(ctx_buf_t(m, "eb_T", (size_t)P * R * l, &T));
     KEVD_TRY(ctx_buf_t(m, "eb_tinfo", P, &tinfo));
     KEVD_TRY(ctx_buf_t(m, "eb_nsw", P, &nsw));
+    double* tau;
+    KEVD_TRY(ctx_buf_t(m, "eb_tau", (size_t)P * l, &tau));
     std::vector<int64_t> hr(P);
     std::vector<const double*> hw(P);
     for (int q = 0; q < P; ++q) hr[q] = B.slots[J[q]].r, hw[q] = B.slots[J[q]].W;
@@ -3020,12 +3214,25 @@ kevd_status eig_batch_run(EigBatch& B)
         BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, Lb, Lb, Rr, &one,
                                            Y, Rr, sRL, Y, Rr, sRL, &zero, G, Lb, sLL, Pb));
         k_symmetrize_b<<<dim3((l + 15) / 16, (l + 15) / 16, P), 256, 0, s>>>(H, l);
-        // batched Jacobi (eigenvalues ascending; its batched divide and
-        // conquer runs as thousands of per-matrix kernels), converged once
-        // the off-diagonal norm is 1e-14 of the matrix's (a tolerance of
-        // 0 asks for machine precision, which it need not reach)
-        BLAS(rocsolver_dsyevj_strided_batched(h, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_lower,
-                                              Lb, H, Lb, sLL, 1e-14, E, 100, nsw, th, sL, info, Pb));
+        if (l <= TQL_LMAX) {
+            // per block: the one-workgroup tridiagonalisation, implicit QR
+            // steps on the tridiagonal and the back-transformation, each a
+            // launch over all problems (eigenvalues ascending in th, the
+            // eigenvectors into GS, which becomes H)
+            k_sytrd_lower_b<<<P, 512, 0, s>>>(H, l, th, E, tau);
+            k_tridiag_eig_b<<<P, 256, 0, s>>>(th, E, l, GS, th, info);
+            k_ormtr_b<<<P, 256, 0, s>>>(H, tau, l, GS);
+            KEVD_HIP(m, hipGetLastError());
+            std::swap(H, GS);
+        } else {
+            // batched Jacobi (eigenvalues ascending; its batched divide and
+            // conquer runs as thousands of per-matrix kernels), converged once
+            // the off-diagonal norm is 1e-14 of the matrix's (a tolerance of
+            // 0 asks for machine precision, which it need not reach)
+            BLAS(rocsolver_dsyevj_strided_batched(h, rocblas_esort_ascending, rocblas_evect_original,
+                                                  rocblas_fill_lower, Lb, H, Lb, sLL, 1e-14, E, 100, nsw, th, sL,
+                                                  info, Pb));
+        }
         BLAS(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, Lb, Lb, Lb, &one, G, Lb,
                                            sLL, H, Lb, sLL, &zero, GS, Lb, sLL, Pb));
         k_ritz_res2_b<<<dim3(k, P), 256, 0, s>>>(H, GS, th, l, k, res2);
