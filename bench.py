@@ -241,7 +241,7 @@ def ensemble_leg(nprob=48, kout=32, seed=1, workers=8, candidates=(32, 64, 128),
 
 
 def c0_leg(n=8192, rank=512, kout=32, seed=0):
-    """N = 8192 KS snapshots (FP64), eps = the exact median
+    """The small case: N = 8192 KS snapshots (FP64), eps = the exact median
     squared distance.  GPU: all-FP64 RPCholesky with block size 1, rank 512,
     then the top-32 two-step kEVD.  CPU: the dense NumPy restatement of
     ks_reference.py:21-40 (the reference's only CPU path) on the same
@@ -284,7 +284,7 @@ def c0_leg(n=8192, rank=512, kout=32, seed=0):
 
 
 def c1_leg(X, peaks, n=262144, nrows=16384, neps=64, cpu_rows=256, with_cpu=True, seed=0):
-    """S(eps) = sum over 16,384 seeded rows x all N = 262,144 KS
+    """The bandwidth sweep: S(eps) = sum over 16,384 seeded rows x all N = 262,144 KS
     snapshots of exp(-|x_i - x_c|^2 / eps), for 64 eps log-spaced over
     [1e-3, 1e3] x the median squared distance, all from one fused launch
     (FP32 MFMA distances, v_exp_f32, FP64 sums).  The points are the first N

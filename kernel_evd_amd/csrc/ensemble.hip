@@ -320,7 +320,7 @@ kevd_status ens_rounds(kevd_ctx* m, kevd_ctx** ctxs, int nprob, const kevd_arpc_
     return status;
 }
 
-// Autotuner cache: (device, n, dp, rank, precision, tol) -> block size.
+// Autotuner cache: (device, n, dp, rank, precision, tol, eps, candidates) -> block size.
 std::mutex tune_mu;
 // key: device, n, padded dim, rank, precision, tol, eps, sorted candidate set
 typedef std::tuple<int, int64_t, int32_t, int64_t, int, double, double, std::vector<int32_t>> TuneKey;

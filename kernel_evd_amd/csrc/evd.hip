@@ -2345,10 +2345,8 @@ __global__ __launch_bounds__(CHOLR_NT) void k_potrf_reg(double* A, int l, int* i
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) e[sl][a][c] -= ur[a] * uc[c];
-                // row k + 1 (patch row dr): published, 0 up to its diagonal
-                // (a store past l goes to the spare slot CHOLR_LMAX)
-                // (a store past l goes to the spare slot CHOLR_LMAX).  The
-                // patch row holding it, (k + 1) % 4, is the same for the whole
+                // row k + 1: published, 0 up to its diagonal (a store past
+                // l goes to the spare slot CHOLR_LMAX).  The patch row holding it, (k + 1) % 4, is the same for the whole
                 // wave: a scalar switch picks it.
                 if (have[sl] && rb[sl] <= k + 1 && k + 1 < rb[sl] + 32) {  // uniform
                     if (pr == ((k + 1 - rb[sl]) >> 2)) {
@@ -2387,7 +2385,6 @@ __global__ __launch_bounds__(CHOLR_NT) void k_potrf_reg(double* A, int l, int* i
     }
 done:
     // R(i, i) = sqrt(A(i, i)) and R(i, j) = A(i, j) / R(i, i) on the final
-    // rows (all, or those above a failing pivot)
     // rows (all, or those above a failing pivot): thread (i, j0) scales row
     // i in the columns j >= i of parity j0, the column-major loads of a wave
     // contiguous
