@@ -2754,6 +2754,70 @@ __global__ __launch_bounds__(256) void k_ormtr_b(const double* __restrict__ A, c
 }
 
 
+// The same product for one l x l column-major Z with l <= SYTRD_LMAX (the
+// leading solve's Rayleigh-Ritz): the columns are independent, so each wave
+// takes one (lane holding rows lane + 64 m) and applies the reflectors last
+// to first, each as a wave-wide dot product (reduced by shuffles, lane 0's
+// sum broadcast) and an update, with the next reflector's column loaded
+// while the current one is applied.  No block barrier is taken.
+constexpr int ORMTR_WAVES = 4;
+__global__ __launch_bounds__(64 * ORMTR_WAVES) void k_ormtr_cols(const double* __restrict__ A,
+                                                                  const double* __restrict__ tau, int l,
+                                                                  double* __restrict__ Z)
+{
+    constexpr int RM = SYTRD_LMAX / 64;
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * ORMTR_WAVES + (threadIdx.x >> 6);
+    if (j >= l) return;  // a whole wave
+    double* zc = Z + (int64_t)j * l;
+    double z[RM], vn[RM];
+#pragma unroll
+    for (int m = 0; m < RM; ++m) {
+        const int r = lane + 64 * m;
+        z[m] = r < l ? zc[r] : 0.0;
+    }
+    // v of reflector i: 1 at row i + 1, A(r, i) below it, 0 above
+    auto load_v = [&](int i, double* v) {
+#pragma unroll
+        for (int m = 0; m < RM; ++m) {
+            const int r = lane + 64 * m;
+            v[m] = r == i + 1 ? 1.0 : (r > i + 1 && r < l ? A[r + (int64_t)i * l] : 0.0);
+        }
+    };
+    if (l >= 2) load_v(l - 2, vn);
+    for (int i = l - 2; i >= 0; --i) {
+        double v[RM];
+#pragma unroll
+        for (int m = 0; m < RM; ++m) v[m] = vn[m];
+        const double t = tau[i];
+        if (i > 0) load_v(i - 1, vn);
+        double w = 0.0;
+#pragma unroll
+        for (int m = 0; m < RM; ++m) w += v[m] * z[m];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) w += __shfl_xor(w, o, 64);
+        w = __shfl(w, 0, 64) * t;
+#pragma unroll
+        for (int m = 0; m < RM; ++m) z[m] -= v[m] * w;
+    }
+#pragma unroll
+    for (int m = 0; m < RM; ++m) {
+        const int r = lane + 64 * m;
+        if (r < l) zc[r] = z[m];
+    }
+}
+
+// KEVD_ROCSOLVER_ORMTR=1 in the environment keeps rocSOLVER's dormtr for the
+// leading solve's back-transformation, for comparison.
+bool rocsolver_ormtr()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("KEVD_ROCSOLVER_ORMTR");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
 // C = A^T B for column-major r x l blocks A, B (lda = ldb = r; C is l x l,
 // ldc = l), FP64.  With l x l outputs and K = r the plain GEMM runs on a
 // handful of workgroups; here K is split into GRAM_TN_SPLIT strided-batched
@@ -2923,14 +2987,20 @@ kevd_status eig_leading(kevd_ctx* ctx, double* W, int64_t r, int k, bool all, do
             KEVD_TRY(gram_tn(ctx, Y, Y, r, l, G));
             k_symmetrize<<<dim3((l + 15) / 16, (l + 15) / 16), 256, 0, ctx->stream>>>(H, l);
             if (l <= SYTRD_LMAX) {
-                // one-workgroup tridiagonalisation, then rocSOLVER's
-                // divide and conquer on T and the back-transformation;
-                // the eigenvectors land in GS, which becomes H
+                // one-workgroup tridiagonalisation, rocSOLVER's divide and
+                // conquer on T, and the back-transformation by one wave per
+                // column; the eigenvectors land in GS, which becomes H
                 k_sytrd_lower<<<1, 512, 0, ctx->stream>>>(H, l, th, E, tau);
                 KEVD_HIP(ctx, hipGetLastError());
                 BLAS(rocsolver_dstedc(h, rocblas_evect_tridiagonal, Lb, th, E, GS, Lb, info));
-                BLAS(rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, Lb, Lb, H,
-                                      Lb, tau, GS, Lb));
+                if (rocsolver_ormtr()) {
+                    BLAS(rocsolver_dormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, Lb, Lb,
+                                          H, Lb, tau, GS, Lb));
+                } else {
+                    k_ormtr_cols<<<(l + ORMTR_WAVES - 1) / ORMTR_WAVES, 64 * ORMTR_WAVES, 0, ctx->stream>>>(H, tau,
+                                                                                                         l, GS);
+                    KEVD_HIP(ctx, hipGetLastError());
+                }
                 std::swap(H, GS);
             } else {
                 BLAS(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, Lb, H, Lb, th, E, info));
