@@ -2757,9 +2757,10 @@ __global__ __launch_bounds__(256) void k_ormtr_b(const double* __restrict__ A, c
 // The same product for one l x l column-major Z with l <= SYTRD_LMAX (the
 // leading solve's Rayleigh-Ritz): the columns are independent, so each wave
 // takes one (lane holding rows lane + 64 m) and applies the reflectors last
-// to first, each as a wave-wide dot product (reduced by shuffles, lane 0's
-// sum broadcast) and an update, with the next reflector's column loaded
-// while the current one is applied.  No block barrier is taken.
+// to first, two per round: their two dot products with the column and their
+// own product are reduced by one set of shuffles (lane 0's sums broadcast),
+// and the next pair's columns load while the current pair is applied.  No
+// block barrier is taken.
 constexpr int ORMTR_WAVES = 4;
 __global__ __launch_bounds__(64 * ORMTR_WAVES) void k_ormtr_cols(const double* __restrict__ A,
                                                                   const double* __restrict__ tau, int l,
@@ -2784,21 +2785,56 @@ __global__ __launch_bounds__(64 * ORMTR_WAVES) void k_ormtr_cols(const double* _
             v[m] = r == i + 1 ? 1.0 : (r > i + 1 && r < l ? A[r + (int64_t)i * l] : 0.0);
         }
     };
-    if (l >= 2) load_v(l - 2, vn);
-    for (int i = l - 2; i >= 0; --i) {
-        double v[RM];
+    // two reflectors per round, i then i - 1: with a = v_i^T z,
+    // b = v_{i-1}^T z and c = v_{i-1}^T v_i reduced by the same shuffles,
+    // H_{i-1} H_i z = z - al v_i - be v_{i-1}, al = tau_i a,
+    // be = tau_{i-1} (b - al c)
+    double vn0[RM];
+    int i = l - 2;
+    if (i >= 1) {
+        load_v(i, vn);
+        load_v(i - 1, vn0);
+    }
+    for (; i >= 1; i -= 2) {
+        double v1[RM], v0[RM];
 #pragma unroll
-        for (int m = 0; m < RM; ++m) v[m] = vn[m];
-        const double t = tau[i];
-        if (i > 0) load_v(i - 1, vn);
-        double w = 0.0;
+        for (int m = 0; m < RM; ++m) {
+            v1[m] = vn[m];
+            v0[m] = vn0[m];
+        }
+        const double t1 = tau[i], t0 = tau[i - 1];
+        if (i >= 3) {
+            load_v(i - 2, vn);
+            load_v(i - 3, vn0);
+        }
+        double a = 0.0, b = 0.0, c = 0.0;
 #pragma unroll
-        for (int m = 0; m < RM; ++m) w += v[m] * z[m];
+        for (int m = 0; m < RM; ++m) {
+            a += v1[m] * z[m];
+            b += v0[m] * z[m];
+            c += v0[m] * v1[m];
+        }
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) w += __shfl_xor(w, o, 64);
-        w = __shfl(w, 0, 64) * t;
+        for (int o = 32; o >= 1; o >>= 1) {
+            a += __shfl_xor(a, o, 64);
+            b += __shfl_xor(b, o, 64);
+            c += __shfl_xor(c, o, 64);
+        }
+        const double al = __shfl(a, 0, 64) * t1;
+        const double be = (__shfl(b, 0, 64) - al * __shfl(c, 0, 64)) * t0;
 #pragma unroll
-        for (int m = 0; m < RM; ++m) z[m] -= v[m] * w;
+        for (int m = 0; m < RM; ++m) z[m] = z[m] - al * v1[m] - be * v0[m];
+    }
+    if (i == 0) {  // an odd count leaves reflector 0
+        load_v(0, vn);
+        double a = 0.0;
+#pragma unroll
+        for (int m = 0; m < RM; ++m) a += vn[m] * z[m];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
+        const double al = __shfl(a, 0, 64) * tau[0];
+#pragma unroll
+        for (int m = 0; m < RM; ++m) z[m] -= al * vn[m];
     }
 #pragma unroll
     for (int m = 0; m < RM; ++m) {
